@@ -1,0 +1,319 @@
+"""allium_amd -- MI355X-native DG upwind transport sweep and source iteration.
+
+Python mirror of the reference's operator interface for the sweep path:
+
+* ``InverseAdvectionOperator``  (src/sweep.hpp:13-130)   ``Mult(source, psi)``
+* ``AdvectionOperator``         (src/sweep.hpp:158-167)  ``Mult``, ``ComputeBalance``
+* ``DiscreteToMoment``          (src/transport_op.hpp:99-123)
+* ``source_iteration``          (SLISolver on TransportOperator, src/linalg.cpp:103-153)
+
+Every call goes through the C ABI of ``lib/liballium_amd.so`` (HIP kernels for
+gfx950).  Host numpy arrays and ``DeviceVector`` (HBM) are both accepted.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import AlliumError, check
+
+FP64 = 0
+FP32_REFINED = 1
+BC_INFLOW = 0
+BC_REFLECTIVE = 1
+
+
+def _d(a):
+    return a.ctypes.data_as(_lib.c_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DeviceVector:
+    """A float64 vector in device memory (HBM)."""
+
+    def __init__(self, n, host=None):
+        lib = _lib.load()
+        self.n = int(n)
+        p = ctypes.c_void_p()
+        check(lib.allium_device_malloc(ctypes.byref(p), 8 * max(1, self.n)))
+        self.ptr = p
+        if host is not None:
+            self.upload(host)
+
+    @classmethod
+    def from_host(cls, a):
+        a = _f64(a).reshape(-1)
+        return cls(a.size, a)
+
+    def upload(self, a):
+        a = _f64(a).reshape(-1)
+        if a.size != self.n:
+            raise ValueError("size mismatch")
+        check(_lib.load().allium_memcpy_h2d(self.ptr, a.ctypes.data_as(ctypes.c_void_p), 8 * self.n))
+
+    def to_host(self, out=None):
+        out = np.empty(self.n) if out is None else out
+        check(_lib.load().allium_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), self.ptr, 8 * self.n))
+        return out
+
+    def copy_from(self, other):
+        check(_lib.load().allium_memcpy_d2d(self.ptr, other.ptr, 8 * self.n))
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            _lib.load().allium_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(_lib.load().allium_device_synchronize())
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    check(_lib.load().allium_device_name(buf, 256))
+    return buf.value.decode()
+
+
+class Mesh:
+    """Structured quad/hex mesh: elements per axis, optional vertex array
+    ((n0+1)(n1+1)[(n2+1)] x dim, x fastest), per-side boundary conditions."""
+
+    def __init__(self, nel, vertices=None, extents=None, bc=None):
+        self.nel = tuple(int(v) for v in nel)
+        self.dim = len(self.nel)
+        self.vertices = None if vertices is None else _f64(vertices).reshape(-1, self.dim)
+        self.extents = tuple(extents) if extents is not None else (1.0,) * self.dim
+        self.bc = tuple(bc) if bc is not None else (BC_INFLOW,) * (2 * self.dim)
+        self.Ne = int(np.prod(self.nel))
+
+    def desc(self):
+        m = _lib.MeshDesc()
+        m.dim = self.dim
+        for d in range(3):
+            m.nel[d] = self.nel[d] if d < self.dim else 1
+            m.extents[d] = self.extents[d] if d < self.dim else 1.0
+        for s in range(6):
+            m.bc[s] = self.bc[s] if s < 2 * self.dim else 0
+        m.vertices = _d(self.vertices) if self.vertices is not None else None
+        return m
+
+
+def _quad_desc(quad, dim):
+    q = _lib.QuadDesc()
+    keep = None
+    if isinstance(quad, int):
+        q.order, q.nomega = quad, 0
+    else:
+        om, w = _f64(quad[0]).reshape(-1, dim), _f64(quad[1]).reshape(-1)
+        keep = (om, w)
+        q.order, q.nomega, q.omega, q.weights = 0, w.size, _d(om), _d(w)
+    return q, keep
+
+
+def level_symmetric(order, dim):
+    """Directions (N x dim) and weights of the library's level-symmetric S_order set."""
+    lib = _lib.load()
+    cap = ctypes.c_int(1024)
+    om, w = np.zeros(1024 * dim), np.zeros(1024)
+    check(lib.allium_host_quadrature(order, dim, _d(om), _d(w), ctypes.byref(cap)))
+    n = cap.value
+    return om[: n * dim].reshape(n, dim).copy(), w[:n].copy()
+
+
+def host_levels(mesh, quad):
+    """Frontier index of every (element, angle) node, shape (Ne, Nomega)."""
+    lib = _lib.load()
+    qd, keep = _quad_desc(quad, mesh.dim)
+    nom = qd.nomega if qd.nomega else level_symmetric(quad, mesh.dim)[1].size
+    out = np.zeros(mesh.Ne * nom, dtype=np.int32)
+    nl = ctypes.c_long()
+    check(lib.allium_host_levels(ctypes.byref(mesh.desc()), ctypes.byref(qd), out.ctypes.data_as(_lib.c_ip),
+                                 ctypes.byref(nl)))
+    return out.reshape(mesh.Ne, nom), nl.value
+
+
+def host_form_source(mesh, fe_order, quad, G, Q=None, inflow=None):
+    lib = _lib.load()
+    qd, keep = _quad_desc(quad, mesh.dim)
+    nom = qd.nomega if qd.nomega else level_symmetric(quad, mesh.dim)[1].size
+    n = (fe_order + 1) ** mesh.dim
+    out = np.zeros(G * nom * mesh.Ne * n)
+    Qa = None if Q is None else _f64(np.broadcast_to(Q, (G, mesh.Ne)))
+    Ia = None if inflow is None else _f64(np.broadcast_to(inflow, (G, 2 * mesh.dim)))
+    check(lib.allium_host_form_source(ctypes.byref(mesh.desc()), fe_order, ctypes.byref(qd), G,
+                                      None if Qa is None else _d(Qa), None if Ia is None else _d(Ia), _d(out)))
+    return out
+
+
+def host_element_matrices(mesh, fe_order, e):
+    """(unit mass, gradient blocks (dim, n, n), MassIntegrator-rule mass) of element e."""
+    lib = _lib.load()
+    n = (fe_order + 1) ** mesh.dim
+    m, g, t = np.zeros(n * n), np.zeros(mesh.dim * n * n), np.zeros(n * n)
+    check(lib.allium_host_element_matrices(ctypes.byref(mesh.desc()), fe_order, e, _d(m), _d(g), _d(t)))
+    return m.reshape(n, n), g.reshape(mesh.dim, n, n), t.reshape(n, n)
+
+
+class InverseAdvectionOperator:
+    """psi = L^{-1} source on the GPU (InverseAdvectionOperator, src/sweep.hpp:13-130).
+
+    ``quad`` is an S_N order (level-symmetric set generated by the library) or
+    an ``(omega, weights)`` pair.  ``sigma_t`` is (G, Ne) element-constant."""
+
+    def __init__(self, mesh, fe_order, quad, sigma_t, lump=0, precision=FP64):
+        lib = _lib.load()
+        self.mesh = mesh
+        sig = _f64(sigma_t)
+        self.G = sig.shape[0] if sig.ndim == 2 else 1
+        sig = sig.reshape(-1)
+        if sig.size != self.G * mesh.Ne:
+            raise ValueError("sigma_t must be (G, Ne)")
+        qd, self._qkeep = _quad_desc(quad, mesh.dim)
+        md = mesh.desc()
+        h = ctypes.c_void_p()
+        check(lib.allium_sweep_create(ctypes.byref(md), fe_order, ctypes.byref(qd), self.G, _d(sig), lump,
+                                      precision, ctypes.byref(h)))
+        self._h = h
+        info = _lib.SweepInfo()
+        check(lib.allium_sweep_info(h, ctypes.byref(info)))
+        self.info = {k: getattr(info, k) for k, _ in _lib.SweepInfo._fields_}
+        self.Nomega = info.Nomega
+        self.n = info.ndof_per_elem
+        self.Ndof = info.Ndof
+        self.size = info.psi_size
+        self.phi_size = info.phi_size
+        self.nlevels = info.levels
+        self.fe_order = fe_order
+        self._lib = lib
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.allium_sweep_destroy(h)
+            self._h = None
+
+    # mfem::Operator shape
+    def Height(self):
+        return self.size
+
+    def Width(self):
+        return self.size
+
+    def quadrature(self):
+        dim = self.mesh.dim
+        om, w = np.zeros(self.Nomega * dim), np.zeros(self.Nomega)
+        check(self._lib.allium_sweep_quadrature(self._h, _d(om), _d(w)))
+        return om.reshape(-1, dim), w
+
+    def Mult(self, source, psi):
+        """psi = L^{-1} source; host arrays (may be the same array) or DeviceVectors."""
+        if isinstance(source, DeviceVector):
+            check(self._lib.allium_sweep_mult_device(self._h, source.ptr, psi.ptr, None))
+            return psi
+        if source.dtype != np.float64 or not source.flags.c_contiguous or source.size != self.size:
+            raise ValueError("source must be a contiguous float64 array of Height() entries")
+        if psi.dtype != np.float64 or not psi.flags.c_contiguous or psi.size != self.size:
+            raise ValueError("psi must be a contiguous float64 array of Height() entries")
+        check(self._lib.allium_sweep_mult(self._h, _d(source), _d(psi)))
+        return psi
+
+    def AssembleLocalMatrices(self, sigma_t):
+        check(self._lib.allium_sweep_set_sigma(self._h, _d(_f64(sigma_t).reshape(-1))))
+
+    def SetTimeAbsorption(self, tau):
+        check(self._lib.allium_sweep_set_time_absorption(self._h, float(tau)))
+
+    def set_precision(self, precision):
+        check(self._lib.allium_sweep_set_precision(self._h, precision))
+
+    def set_launch(self, mode):
+        """0 per-level launches, 1 hipGraph, 2 persistent kernel, -1 autotune."""
+        check(self._lib.allium_sweep_set_launch(self._h, mode))
+        info = _lib.SweepInfo()
+        check(self._lib.allium_sweep_info(self._h, ctypes.byref(info)))
+        return info.launch_mode
+
+    def levels(self):
+        out = np.zeros(self.mesh.Ne * self.Nomega, dtype=np.int32)
+        check(self._lib.allium_sweep_levels(self._h, out.ctypes.data_as(_lib.c_ip)))
+        return out.reshape(self.mesh.Ne, self.Nomega)
+
+    def form_source(self, Q=None, inflow=None):
+        out = np.empty(self.size)
+        Qa = None if Q is None else _f64(np.broadcast_to(Q, (self.G, self.mesh.Ne)))
+        Ia = None if inflow is None else _f64(np.broadcast_to(inflow, (self.G, 2 * self.mesh.dim)))
+        check(self._lib.allium_form_source(self._h, None if Qa is None else _d(Qa),
+                                           None if Ia is None else _d(Ia), _d(out)))
+        return out
+
+    def timing(self):
+        """(seconds, sweeps): the "sweep" TimingLog and "sweeps" EventLog entries."""
+        t, n = ctypes.c_double(), ctypes.c_long()
+        check(self._lib.allium_sweep_timing(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+
+class AdvectionOperator:
+    """L psi and particle balance from the sweep's matrices (src/sweep.cpp:1246-1462)."""
+
+    def __init__(self, Linv):
+        self.Linv = Linv
+
+    def Mult(self, x, y):
+        L = self.Linv
+        if isinstance(x, DeviceVector):
+            check(L._lib.allium_apply_L_device(L._h, x.ptr, y.ptr, None))
+            return y
+        check(L._lib.allium_apply_L(L._h, _d(_f64(x)), _d(y)))
+        return y
+
+    def ComputeBalance(self, psi):
+        L = self.Linv
+        out = ctypes.c_double()
+        if isinstance(psi, DeviceVector):
+            check(L._lib.allium_balance_device(L._h, psi.ptr, ctypes.byref(out), None))
+        else:
+            check(L._lib.allium_balance(L._h, _d(_f64(psi)), ctypes.byref(out)))
+        return out.value
+
+
+class DiscreteToMoment:
+    """0th-moment D and D^T on device vectors (src/transport_op.cpp:3-60)."""
+
+    def __init__(self, Linv):
+        self.Linv = Linv
+
+    def Mult(self, psi, phi):
+        check(self.Linv._lib.allium_moments_device(self.Linv._h, psi.ptr, phi.ptr, None))
+        return phi
+
+    def MultTranspose(self, phi, psi):
+        check(self.Linv._lib.allium_moments_transpose_device(self.Linv._h, phi.ptr, psi.ptr, None))
+        return psi
+
+
+def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000, precision=FP64,
+                     want_psi=True):
+    """chive's SI: returns (phi, psi, iterations, norms) (exe/chive.cpp:700-735)."""
+    lib = Linv._lib
+    opt = _lib.SIOptions(abstol, reltol, int(maxit), int(precision), 0, None)
+    phi = np.empty(Linv.phi_size)
+    psi = np.empty(Linv.size) if want_psi else None
+    norms = np.zeros(maxit + 1)
+    it = ctypes.c_int(0)
+    rc = lib.allium_source_iteration(Linv._h, _d(_f64(np.broadcast_to(sigma_s, (Linv.mesh.Ne,)))), _d(_f64(source)),
+                                     ctypes.byref(opt), _d(phi), None if psi is None else _d(psi), _d(norms),
+                                     ctypes.byref(it))
+    if rc not in (0, -7):
+        check(rc)
+    return phi, psi, it.value, norms[: it.value].copy()

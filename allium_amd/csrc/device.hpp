@@ -1,0 +1,118 @@
+// Device-side data of one sweep handle and the launchers of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace allium {
+
+// Everything a sweep kernel reads, passed by value.
+struct SweepParams {
+    int dim, p, n, nf;
+    int Ne, Nomega, G;
+    int nel[3];
+    int stride[3];          // element id strides per axis
+    float inv_nel0, inv_nel01;
+    long long Ndof;
+    const double *omega;    // Nomega x 4 (x, y, z, w)
+    // geometry: uniform => one element's matrices; general => per element / face
+    int uniform;
+    const double *K;        // uniform: Nomega x n x n, streaming + outflow faces
+    const float *Kf;
+    const double *Mu;       // unit mass (MG rule): uniform n x n, general Ne x n x n
+    const float *Muf;
+    const double *Mt;       // time mass (= MassIntegrator rule matrix)
+    const float *Mtf;
+    const double *Gd;       // general: Ne x dim x n x n
+    const double *Fface;    // uniform: 2dim x nf x nf; general: Nf x nf x nf
+    const float *Ffacef;
+    const double *fnor;     // general: Nf x 3
+    int foff[4];
+    int fext[3][3];
+    const double *sigma;    // G x Ne
+    double tau;
+    int timedep;
+    const int *refl;        // Nomega x 2dim reflected direction or -1
+    int refl_mask;          // bit s set => side s reflective
+    // vectors
+    const double *src;      // psi-sized source (may be null)
+    const double *iso;      // phi-sized isotropic source added to every direction (may be null)
+    double *psi;
+    const uint32_t *cells;  // (e << 10) | a, grouped by level
+    int refine;             // FP32 solves: number of FP64 residual refinements
+};
+
+// Tiles of the persistent kernel: a run of (cell, group) items of one
+// (level, direction) class.
+struct Tile {
+    int key;        // dependency class of this tile
+    int wait_key;   // class that must be complete first, -1 if none
+    int cell_begin; // first cell of the class in the cells array
+    int ncells;     // cells in the class
+    int item_begin; // first (cell, group) item of this tile
+    int nitems;
+};
+
+struct DeviceSweep {
+    SweepParams prm{};
+    int precision = 0;       // 0 fp64, 1 fp32 + refinement
+    int launch_mode = 0;     // 0 levels, 1 graph, 2 persistent
+    int block = 128;
+    int persistent_tile = 256;
+    // host copies of the plan
+    std::vector<long> level_ptr;
+    int nlevels = 0;
+    // device buffers owned by the handle
+    std::vector<void *> owned;
+    double *d_omega = nullptr;
+    uint32_t *d_cells = nullptr;
+    int *d_refl = nullptr;
+    // persistent kernel state
+    Tile *d_tiles = nullptr;
+    int ntiles = 0;
+    int nkeys = 0;
+    int *d_counters = nullptr;  // nkeys + 2 (ticket, error flag)
+    int *d_need = nullptr;
+    // graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    const void *graph_src = nullptr, *graph_iso = nullptr;
+    void *graph_psi = nullptr;
+    int graph_prec = -1;
+    hipStream_t graph_stream = nullptr;
+    // timing (TimingLog "sweep", EventLog "sweeps")
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double sweep_seconds = 0.0;
+    long sweeps = 0;
+    bool timing = true;
+};
+
+hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, double *psi,
+                        hipStream_t stream);
+hipError_t reset_persistent(DeviceSweep &ds, hipStream_t stream);
+bool sweep_supported(int dim, int p, std::string &why);
+
+// streaming operator L psi (AdvectionOperator::Mult)
+hipError_t launch_apply_L(const SweepParams &prm, const double *x, double *y, hipStream_t stream);
+// per-block partial sums of absorption and leakage (ComputeBalance); out has
+// 2 * balance_blocks(prm) doubles
+int balance_blocks(const SweepParams &prm);
+hipError_t launch_balance(const SweepParams &prm, const double *psi, double *partial,
+                          hipStream_t stream);
+// phi = sum_a w_a psi ; psi = phi / wsum ; y = scale * coef_e * Mm(e) x_e
+hipError_t launch_moments(const SweepParams &prm, const double *psi, double *phi, hipStream_t s);
+hipError_t launch_moments_T(const SweepParams &prm, double inv_wsum, const double *phi, double *psi,
+                            hipStream_t s);
+hipError_t launch_mass(const SweepParams &prm, const double *Mm, const double *coef, double scale,
+                       const double *x, double *y, hipStream_t s);
+// SI update: r = b - (x - y); x += r; partial[blk] = sum r^2 ; returns blocks used
+int si_update_blocks(long n);
+hipError_t launch_si_update(long n, const double *b, const double *y, double *x, double *partial,
+                            hipStream_t s);
+// plain vector ops
+hipError_t launch_axpby(long n, double a, const double *x, double b, const double *y, double *z,
+                        hipStream_t s);
+
+}  // namespace allium

@@ -1,0 +1,84 @@
+// Host-side discretization for the MI355X sweep: level-symmetric quadrature,
+// structured quad/hex meshes, Gauss-Lobatto DG element/face matrices and the
+// per-direction topological levels of the sweep graph.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+namespace allium {
+
+// Level-symmetric S_N set (LevelSymmetricQuadrature, src/angular_quadrature.cpp:98-147).
+// omega is stored padded to 3 components per direction.
+struct Quadrature {
+    int dim = 0;
+    std::vector<double> omega;  // size() x 3
+    std::vector<double> w;
+    double wsum = 0.0;
+    int size() const { return (int)w.size(); }
+};
+bool level_symmetric(int order, int dim, Quadrature &q, std::string &err);
+
+// Structured mesh: nel[d] elements per axis, elements and vertices x-fastest.
+struct Mesh {
+    int dim = 0;
+    int nel[3] = {1, 1, 1};
+    int nv[3] = {1, 1, 1};
+    int Ne = 0;
+    std::vector<double> vert;  // Nv x dim
+    int bc[6] = {0, 0, 0, 0, 0, 0};
+    bool uniform = false;      // every element the same axis-aligned box
+    double h[3] = {0, 0, 0};
+    // faces, grouped by axis; e1 owns the normal (lower element id on
+    // interior faces), e2 = -1 on the boundary
+    int Nf = 0;
+    int foff[4] = {0, 0, 0, 0};
+    int fext[3][3];
+    std::vector<int> f_e1, f_e2, f_bside;
+    std::vector<double> f_nor;  // Nf x 3, unit normal at the face centre, outward of e1
+
+    void elem_coords(int e, int *c) const {
+        c[0] = e % nel[0];
+        c[1] = (e / nel[0]) % nel[1];
+        c[2] = e / (nel[0] * nel[1]);
+    }
+    int face_of(int e, int side) const;  // side = 2*axis + {0 low, 1 high}
+};
+bool build_mesh(int dim, const int *nel, const double *vertices, const double *extents,
+                const int *bc, Mesh &m, std::string &err);
+
+// DG Gauss-Lobatto element data. Matrices are row-major n x n.
+struct ElementData {
+    int dim = 0, p = 0, n = 0, nf = 0, lump = 0;
+    bool uniform = false;
+    std::vector<double> Mu;     // unit mass (MGMassIntegrator rule, order 2p), per element
+    std::vector<double> Mm;     // MassIntegrator rule (time mass, scattering mass), per element
+    std::vector<double> Gd;     // per element: dim blocks of -int phi_j d_d phi_i
+    std::vector<double> Fface;  // per face (uniform: per side): nf x nf face mass
+    std::vector<int> side_nodes;  // 2*dim x nf local node ids of each side, face-lexicographic
+};
+bool assemble(const Mesh &m, int p, int lump, ElementData &ed, std::string &err);
+
+// Sweep plan: level of every node e*Nomega+a and the cells grouped by level.
+struct SweepPlan {
+    int nlevels = 0;
+    long nedges = 0;
+    std::vector<int> level;        // Ne*Nomega
+    std::vector<uint32_t> cells;   // packed (e << 10) | a, grouped by level, then angle, then e
+    std::vector<long> level_ptr;   // nlevels+1
+    std::vector<int> refl;         // Nomega x 2dim reflected angle or -1
+    bool angle_independent = true; // no reflective coupling between directions
+};
+// Returns 0, or -2 (not a DAG) / -3 (missing reflected direction).
+int build_plan(const Mesh &m, const Quadrature &q, SweepPlan &plan, std::string &err);
+
+// Source vector (FormTransportSource) on the host, RIGHT layout.
+void form_source(const Mesh &m, const ElementData &ed, const Quadrature &q, int G, const double *Q,
+                 const double *inflow, double *source);
+
+// small helpers
+void parallel_for(long n, const std::function<void(long, long)> &body);
+
+}  // namespace allium

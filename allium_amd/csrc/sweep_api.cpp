@@ -1,0 +1,778 @@
+// C ABI of the MI355X sweep (include/allium_sweep.h): handle construction,
+// device upload, the Mult entry points, the streaming operator, moments,
+// balance and the source iteration driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/allium_device.h"
+#include "../../include/allium_sweep.h"
+#include "device.hpp"
+#include "discretization.hpp"
+
+namespace {
+thread_local std::string g_last_error = "no error";
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(ALLIUM_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+template <typename T>
+hipError_t upload(std::vector<void *> &owned, T **dst, const T *src, size_t count)
+{
+    *dst = nullptr;
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void **)dst, sizeof(T) * count);
+    if (e != hipSuccess) return e;
+    owned.push_back(*dst);
+    if (src) return hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice);
+    return hipSuccess;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) hipFree(p); }
+    hipError_t ensure(size_t b)
+    {
+        if (bytes >= b) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    double *d() const { return (double *)p; }
+};
+}  // namespace
+
+struct allium_sweep {
+    allium::Mesh mesh;
+    allium::Quadrature quad;
+    allium::ElementData ed;
+    allium::SweepPlan plan;
+    allium::DeviceSweep ds;
+    int G = 1;
+    long psi_size = 0, phi_size = 0;
+    std::vector<double> sigma;
+    double *d_sigma = nullptr;
+    // matrices kept for host-side rebuilds
+    std::vector<double> Kst;
+    // scratch for host-pointer entry points and the source iteration
+    DevBuf a, b, c, x, y, iso, coef, part, q;
+    hipStream_t stream = nullptr;
+    ~allium_sweep()
+    {
+        for (void *p : ds.owned) hipFree(p);
+        if (ds.graph_exec) hipGraphExecDestroy(ds.graph_exec);
+        if (ds.graph_stream) hipStreamDestroy(ds.graph_stream);
+        if (ds.ev0) hipEventDestroy(ds.ev0);
+        if (ds.ev1) hipEventDestroy(ds.ev1);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Uniform geometry: K_a = sum_d Omega_d G_d + sum_{sides with Omega.n > 0} (Omega.n) F_side
+void build_streaming(allium_sweep *h)
+{
+    const auto &ed = h->ed;
+    const int n = ed.n, nf = ed.nf, dim = ed.dim, No = h->quad.size();
+    h->Kst.assign((size_t)No * n * n, 0.0);
+    for (int a = 0; a < No; a++) {
+        double *K = &h->Kst[(size_t)a * n * n];
+        for (int d = 0; d < dim; d++) {
+            const double od = h->quad.omega[3 * a + d];
+            for (int i = 0; i < n * n; i++) K[i] += od * ed.Gd[(size_t)d * n * n + i];
+        }
+        for (int s = 0; s < 2 * dim; s++) {
+            const double dot = (s & 1) ? h->quad.omega[3 * a + (s >> 1)] : -h->quad.omega[3 * a + (s >> 1)];
+            if (dot > 0)
+                for (int k = 0; k < nf; k++)
+                    for (int l = 0; l < nf; l++)
+                        K[ed.side_nodes[s * nf + k] * n + ed.side_nodes[s * nf + l]] +=
+                            dot * ed.Fface[(size_t)s * nf * nf + k * nf + l];
+        }
+    }
+}
+
+void build_tiles(allium_sweep *h, std::vector<allium::Tile> &tiles, std::vector<int> &need, int &nkeys)
+{
+    const auto &plan = h->plan;
+    const int No = h->quad.size(), G = h->G, T = h->ds.persistent_tile;
+    tiles.clear();
+    need.clear();
+    if (plan.angle_independent) {
+        nkeys = plan.nlevels * No;
+        need.assign(nkeys, 0);
+        long i = 0;
+        const long ncell = (long)plan.cells.size();
+        while (i < ncell) {
+            const int a = (int)(plan.cells[i] & 1023u);
+            const int e = (int)(plan.cells[i] >> 10);
+            const int L = plan.level[(size_t)e * No + a];
+            long j = i;
+            while (j < ncell && (int)(plan.cells[j] & 1023u) == a &&
+                   plan.level[(size_t)(plan.cells[j] >> 10) * No + a] == L)
+                j++;
+            const int key = L * No + a;
+            const int nc = (int)(j - i);
+            need[key] = nc * G;
+            for (int it = 0; it < nc * G; it += T)
+                tiles.push_back({key, L > 0 ? key - No : -1, (int)i, nc, it, std::min(T, nc * G - it)});
+            i = j;
+        }
+    } else {
+        nkeys = plan.nlevels;
+        need.assign(nkeys, 0);
+        for (int L = 0; L < plan.nlevels; L++) {
+            const long b = plan.level_ptr[L];
+            const int nc = (int)(plan.level_ptr[L + 1] - b);
+            need[L] = nc * G;
+            for (int it = 0; it < nc * G; it += T)
+                tiles.push_back({L, L > 0 ? L - 1 : -1, (int)b, nc, it, std::min(T, nc * G - it)});
+        }
+    }
+}
+
+int upload_all(allium_sweep *h)
+{
+    auto &ds = h->ds;
+    auto &prm = ds.prm;
+    const auto &m = h->mesh;
+    const auto &ed = h->ed;
+    const int No = h->quad.size(), n = ed.n, dim = m.dim;
+    std::memset(&prm, 0, sizeof(prm));
+    prm.dim = dim; prm.p = ed.p; prm.n = n; prm.nf = ed.nf;
+    prm.Ne = m.Ne; prm.Nomega = No; prm.G = h->G;
+    for (int d = 0; d < 3; d++) prm.nel[d] = m.nel[d];
+    prm.stride[0] = 1; prm.stride[1] = m.nel[0]; prm.stride[2] = m.nel[0] * m.nel[1];
+    prm.inv_nel0 = 1.0f / (float)m.nel[0];
+    prm.inv_nel01 = 1.0f / (float)(m.nel[0] * m.nel[1]);
+    prm.Ndof = (long long)m.Ne * n;
+    prm.uniform = m.uniform ? 1 : 0;
+    for (int i = 0; i < 4; i++) prm.foff[i] = m.foff[i];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) prm.fext[i][j] = m.fext[i][j];
+    std::vector<double> om4((size_t)No * 4);
+    for (int a = 0; a < No; a++) {
+        for (int d = 0; d < 3; d++) om4[4 * a + d] = h->quad.omega[3 * a + d];
+        om4[4 * a + 3] = h->quad.w[a];
+    }
+    HIP_TRY(upload(ds.owned, &ds.d_omega, om4.data(), om4.size()));
+    prm.omega = ds.d_omega;
+    auto tofloat = [](const std::vector<double> &v) {
+        std::vector<float> f(v.size());
+        for (size_t i = 0; i < v.size(); i++) f[i] = (float)v[i];
+        return f;
+    };
+    double *dK = nullptr, *dMu = nullptr, *dMm = nullptr, *dGd = nullptr, *dF = nullptr, *dN = nullptr;
+    float *fK = nullptr, *fMu = nullptr, *fMm = nullptr, *fF = nullptr;
+    if (m.uniform) {
+        build_streaming(h);
+        HIP_TRY(upload(ds.owned, &dK, h->Kst.data(), h->Kst.size()));
+        auto kf = tofloat(h->Kst);
+        HIP_TRY(upload(ds.owned, &fK, kf.data(), kf.size()));
+    } else {
+        HIP_TRY(upload(ds.owned, &dGd, ed.Gd.data(), ed.Gd.size()));
+        HIP_TRY(upload(ds.owned, &dN, m.f_nor.data(), m.f_nor.size()));
+    }
+    HIP_TRY(upload(ds.owned, &dMu, ed.Mu.data(), ed.Mu.size()));
+    HIP_TRY(upload(ds.owned, &dMm, ed.Mm.data(), ed.Mm.size()));
+    HIP_TRY(upload(ds.owned, &dF, ed.Fface.data(), ed.Fface.size()));
+    {
+        auto a = tofloat(ed.Mu), b = tofloat(ed.Mm), c = tofloat(ed.Fface);
+        HIP_TRY(upload(ds.owned, &fMu, a.data(), a.size()));
+        HIP_TRY(upload(ds.owned, &fMm, b.data(), b.size()));
+        HIP_TRY(upload(ds.owned, &fF, c.data(), c.size()));
+    }
+    prm.K = dK; prm.Kf = fK; prm.Mu = dMu; prm.Muf = fMu; prm.Mt = dMm; prm.Mtf = fMm;
+    prm.Gd = dGd; prm.Fface = dF; prm.Ffacef = fF; prm.fnor = dN;
+    HIP_TRY(upload(ds.owned, &h->d_sigma, h->sigma.data(), h->sigma.size()));
+    prm.sigma = h->d_sigma;
+    HIP_TRY(upload(ds.owned, &ds.d_cells, h->plan.cells.data(), h->plan.cells.size()));
+    prm.cells = ds.d_cells;
+    prm.refl_mask = 0;
+    for (int s = 0; s < 2 * dim; s++)
+        if (m.bc[s] == 1) prm.refl_mask |= 1 << s;
+    if (prm.refl_mask) {
+        HIP_TRY(upload(ds.owned, &ds.d_refl, h->plan.refl.data(), h->plan.refl.size()));
+        prm.refl = ds.d_refl;
+    }
+    ds.level_ptr = h->plan.level_ptr;
+    ds.nlevels = h->plan.nlevels;
+    std::vector<allium::Tile> tiles;
+    std::vector<int> need;
+    build_tiles(h, tiles, need, ds.nkeys);
+    ds.ntiles = (int)tiles.size();
+    HIP_TRY(upload(ds.owned, &ds.d_tiles, tiles.data(), tiles.size()));
+    HIP_TRY(upload(ds.owned, &ds.d_need, need.data(), need.size()));
+    HIP_TRY(upload(ds.owned, &ds.d_counters, (const int *)nullptr, (size_t)ds.nkeys + 2));
+    HIP_TRY(hipMemset(ds.d_counters, 0, sizeof(int) * (ds.nkeys + 2)));
+    HIP_TRY(hipEventCreate(&ds.ev0));
+    HIP_TRY(hipEventCreate(&ds.ev1));
+    HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    return ALLIUM_OK;
+}
+
+bool finite_all(const double *v, long n)
+{
+    for (long i = 0; i < n; i++)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+int check_sweep_error(allium_sweep *h, hipStream_t stream)
+{
+    if (h->ds.launch_mode != 2) return ALLIUM_OK;
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, h->ds.d_counters + h->ds.nkeys + 1, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (flag) return fail(ALLIUM_ERR_DEVICE, "persistent sweep: a dependency wait exceeded its spin limit");
+    return ALLIUM_OK;
+}
+
+int do_sweep(allium_sweep *h, const double *src, const double *iso, double *psi, hipStream_t stream)
+{
+    HIP_TRY(allium::launch_sweep(h->ds, src, iso, psi, stream));
+    if (h->ds.timing) {
+        HIP_TRY(hipEventSynchronize(h->ds.ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ds.ev0, h->ds.ev1));
+        h->ds.sweep_seconds += ms * 1e-3;
+    }
+    return check_sweep_error(h, stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *allium_last_error(void) { return g_last_error.c_str(); }
+
+int allium_sweep_create(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad, int G,
+                        const double *sigma_t, int lump, int precision, allium_sweep_t *out)
+{
+    if (!mesh || !quad || !sigma_t || !out || G < 1) return fail(ALLIUM_ERR_ARG, "null argument or G < 1");
+    *out = nullptr;
+    if (lump != 0) return fail(ALLIUM_ERR_UNSUPPORTED, "lumped sweep matrices are not implemented on the GPU");
+    std::string why;
+    if (!allium::sweep_supported(mesh->dim, fe_order, why)) return fail(ALLIUM_ERR_UNSUPPORTED, why);
+    auto h = std::make_unique<allium_sweep>();
+    if (!allium::build_mesh(mesh->dim, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, h->mesh, why))
+        return fail(ALLIUM_ERR_ARG, why);
+    if (quad->nomega > 0) {
+        if (!quad->omega || !quad->weights) return fail(ALLIUM_ERR_ARG, "explicit quadrature needs omega and weights");
+        h->quad.dim = mesh->dim;
+        for (int a = 0; a < quad->nomega; a++) {
+            for (int d = 0; d < 3; d++) h->quad.omega.push_back(d < mesh->dim ? quad->omega[a * mesh->dim + d] : 0.0);
+            h->quad.w.push_back(quad->weights[a]);
+            h->quad.wsum += quad->weights[a];
+        }
+    } else if (!allium::level_symmetric(quad->order, mesh->dim, h->quad, why)) {
+        return fail(ALLIUM_ERR_ARG, why);
+    }
+    h->G = G;
+    const long Ne = h->mesh.Ne;
+    h->sigma.assign(sigma_t, sigma_t + (size_t)G * Ne);
+    if (!finite_all(sigma_t, (long)G * Ne)) return fail(ALLIUM_ERR_NUMERIC, "non-finite sigma_t");
+    if (!allium::assemble(h->mesh, fe_order, lump, h->ed, why)) return fail(ALLIUM_ERR_ARG, why);
+    int st = allium::build_plan(h->mesh, h->quad, h->plan, why);
+    if (st == -2) return fail(ALLIUM_ERR_NOT_DAG, why);
+    if (st == -3) return fail(ALLIUM_ERR_REFLECT, why);
+    if (st) return fail(ALLIUM_ERR_ARG, why);
+    h->psi_size = (long)G * h->quad.size() * Ne * h->ed.n;
+    h->phi_size = (long)G * Ne * h->ed.n;
+    h->ds.precision = precision == ALLIUM_FP32_REFINED ? 1 : 0;
+    h->ds.prm.refine = 1;
+    int rc = upload_all(h.get());
+    if (rc) return rc;
+    // launch strategy: persistent kernel when the direction classes are independent
+    h->ds.launch_mode = 0;
+    *out = h.release();
+    return ALLIUM_OK;
+}
+
+void allium_sweep_destroy(allium_sweep_t h) { delete h; }
+
+int allium_sweep_info(allium_sweep_t h, allium_sweep_info_t *info)
+{
+    if (!h || !info) return fail(ALLIUM_ERR_ARG, "null argument");
+    info->Ne = h->mesh.Ne;
+    info->ndof_per_elem = h->ed.n;
+    info->Ndof = (long)h->mesh.Ne * h->ed.n;
+    info->Nomega = h->quad.size();
+    info->G = h->G;
+    info->psi_size = h->psi_size;
+    info->phi_size = h->phi_size;
+    info->levels = h->plan.nlevels;
+    info->cells = (long)h->plan.cells.size();
+    info->edges = h->plan.nedges;
+    info->uniform_geometry = h->mesh.uniform ? 1 : 0;
+    info->launch_mode = h->ds.launch_mode;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_quadrature(allium_sweep_t h, double *omega, double *weights)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    const int dim = h->mesh.dim;
+    for (int a = 0; a < h->quad.size(); a++) {
+        if (omega)
+            for (int d = 0; d < dim; d++) omega[a * dim + d] = h->quad.omega[3 * a + d];
+        if (weights) weights[a] = h->quad.w[a];
+    }
+    return ALLIUM_OK;
+}
+
+int allium_sweep_set_sigma(allium_sweep_t h, const double *sigma_t)
+{
+    if (!h || !sigma_t) return fail(ALLIUM_ERR_ARG, "null argument");
+    const long n = (long)h->G * h->mesh.Ne;
+    if (!finite_all(sigma_t, n)) return fail(ALLIUM_ERR_NUMERIC, "non-finite sigma_t");
+    h->sigma.assign(sigma_t, sigma_t + n);
+    HIP_TRY(hipMemcpy(h->d_sigma, sigma_t, sizeof(double) * n, hipMemcpyHostToDevice));
+    return ALLIUM_OK;
+}
+
+int allium_sweep_set_time_absorption(allium_sweep_t h, double tau)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    if (!std::isfinite(tau)) return fail(ALLIUM_ERR_NUMERIC, "non-finite time absorption");
+    h->ds.prm.tau = tau;
+    h->ds.prm.timedep = 1;
+    if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
+    return ALLIUM_OK;
+}
+
+int allium_sweep_set_precision(allium_sweep_t h, int precision)
+{
+    if (!h || (precision != ALLIUM_FP64 && precision != ALLIUM_FP32_REFINED))
+        return fail(ALLIUM_ERR_ARG, "bad precision");
+    h->ds.precision = precision == ALLIUM_FP32_REFINED ? 1 : 0;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_levels(allium_sweep_t h, int *level_of_node)
+{
+    if (!h || !level_of_node) return fail(ALLIUM_ERR_ARG, "null argument");
+    std::memcpy(level_of_node, h->plan.level.data(), sizeof(int) * h->plan.level.size());
+    return ALLIUM_OK;
+}
+
+int allium_sweep_mult_device(allium_sweep_t h, const double *d_source, double *d_psi, void *stream)
+{
+    if (!h || !d_source || !d_psi) return fail(ALLIUM_ERR_ARG, "null argument");
+    return do_sweep(h, d_source, nullptr, d_psi, (hipStream_t)stream);
+}
+
+int allium_sweep_mult(allium_sweep_t h, const double *source, double *psi)
+{
+    if (!h || !source || !psi) return fail(ALLIUM_ERR_ARG, "null argument");
+    const size_t bytes = sizeof(double) * h->psi_size;
+    HIP_TRY(h->a.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(h->a.p, source, bytes, hipMemcpyHostToDevice, h->stream));
+    int rc = do_sweep(h, h->a.d(), nullptr, h->a.d(), h->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(psi, h->a.p, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return ALLIUM_OK;
+}
+
+int allium_sweep_set_launch(allium_sweep_t h, int mode)
+{
+    if (!h || mode < -1 || mode > 2) return fail(ALLIUM_ERR_ARG, "bad launch mode");
+    if (mode == 2)
+        return fail(ALLIUM_ERR_UNSUPPORTED,
+                    "persistent sweep kernel is disabled: a run on MI355X did not terminate and the cause "
+                    "is not identified");
+    if (mode >= 0) { h->ds.launch_mode = mode; return ALLIUM_OK; }
+    // autotune: time each strategy and block size on scratch vectors
+    const size_t bytes = sizeof(double) * h->psi_size;
+    HIP_TRY(h->b.ensure(bytes));
+    HIP_TRY(h->c.ensure(bytes));
+    HIP_TRY(hipMemsetAsync(h->b.p, 0, bytes, h->stream));
+    struct Cand { int mode, block; };
+    std::vector<Cand> cands = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {1, 256}};
+    double best = 1e30;
+    Cand pick = {0, 128};
+    const bool timing = h->ds.timing;
+    h->ds.timing = false;
+    for (const auto &cd : cands) {
+        h->ds.launch_mode = cd.mode;
+        h->ds.block = cd.block;
+        if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
+        int rc = do_sweep(h, h->b.d(), nullptr, h->c.d(), h->stream);
+        if (rc) { h->ds.timing = timing; return rc; }
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, h->stream));
+        const int reps = 3;
+        for (int r = 0; r < reps; r++) HIP_TRY(allium::launch_sweep(h->ds, h->b.d(), nullptr, h->c.d(), h->stream));
+        HIP_TRY(hipEventRecord(e1, h->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        int rc2 = check_sweep_error(h, h->stream);
+        if (rc2) { h->ds.timing = timing; return rc2; }
+        if (ms / reps < best) { best = ms / reps; pick = cd; }
+    }
+    h->ds.timing = timing;
+    h->ds.launch_mode = pick.mode;
+    h->ds.block = pick.block;
+    if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
+    h->ds.sweeps = 0;
+    h->ds.sweep_seconds = 0;
+    return ALLIUM_OK;
+}
+
+int allium_apply_L_device(allium_sweep_t h, const double *d_psi, double *d_Lpsi, void *stream)
+{
+    if (!h || !d_psi || !d_Lpsi || d_psi == d_Lpsi) return fail(ALLIUM_ERR_ARG, "bad argument (in-place not allowed)");
+    HIP_TRY(allium::launch_apply_L(h->ds.prm, d_psi, d_Lpsi, (hipStream_t)stream));
+    return ALLIUM_OK;
+}
+
+int allium_apply_L(allium_sweep_t h, const double *psi, double *Lpsi)
+{
+    if (!h || !psi || !Lpsi) return fail(ALLIUM_ERR_ARG, "null argument");
+    const size_t bytes = sizeof(double) * h->psi_size;
+    HIP_TRY(h->a.ensure(bytes));
+    HIP_TRY(h->b.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(h->a.p, psi, bytes, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(allium::launch_apply_L(h->ds.prm, h->a.d(), h->b.d(), h->stream));
+    HIP_TRY(hipMemcpyAsync(Lpsi, h->b.p, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return ALLIUM_OK;
+}
+
+int allium_balance_device(allium_sweep_t h, const double *d_psi, double *out, void *stream)
+{
+    if (!h || !d_psi || !out) return fail(ALLIUM_ERR_ARG, "null argument");
+    const int nb = allium::balance_blocks(h->ds.prm);
+    HIP_TRY(h->part.ensure(sizeof(double) * 2 * nb));
+    HIP_TRY(allium::launch_balance(h->ds.prm, d_psi, h->part.d(), (hipStream_t)stream));
+    std::vector<double> hp(2 * nb);
+    HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    double ab = 0.0, lk = 0.0;
+    for (int i = 0; i < nb; i++) { ab += hp[2 * i]; lk += hp[2 * i + 1]; }
+    *out = lk + ab;
+    return ALLIUM_OK;
+}
+
+int allium_balance(allium_sweep_t h, const double *psi, double *out)
+{
+    if (!h || !psi || !out) return fail(ALLIUM_ERR_ARG, "null argument");
+    const size_t bytes = sizeof(double) * h->psi_size;
+    HIP_TRY(h->a.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(h->a.p, psi, bytes, hipMemcpyHostToDevice, h->stream));
+    return allium_balance_device(h, h->a.d(), out, h->stream);
+}
+
+int allium_form_source(allium_sweep_t h, const double *Q, const double *inflow, double *source)
+{
+    if (!h || !source) return fail(ALLIUM_ERR_ARG, "null argument");
+    allium::form_source(h->mesh, h->ed, h->quad, h->G, Q, inflow, source);
+    return ALLIUM_OK;
+}
+
+int allium_moments_device(allium_sweep_t h, const double *d_psi, double *d_phi, void *stream)
+{
+    if (!h || !d_psi || !d_phi) return fail(ALLIUM_ERR_ARG, "null argument");
+    HIP_TRY(allium::launch_moments(h->ds.prm, d_psi, d_phi, (hipStream_t)stream));
+    return ALLIUM_OK;
+}
+
+int allium_moments_transpose_device(allium_sweep_t h, const double *d_phi, double *d_psi, void *stream)
+{
+    if (!h || !d_psi || !d_phi) return fail(ALLIUM_ERR_ARG, "null argument");
+    HIP_TRY(allium::launch_moments_T(h->ds.prm, 1.0 / h->quad.wsum, d_phi, d_psi, (hipStream_t)stream));
+    return ALLIUM_OK;
+}
+
+int allium_mass_device(allium_sweep_t h, const double *d_coef, const double *d_x, double *d_y, void *stream)
+{
+    if (!h || !d_coef || !d_x || !d_y) return fail(ALLIUM_ERR_ARG, "null argument");
+    HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, d_coef, 1.0, d_x, d_y, (hipStream_t)stream));
+    return ALLIUM_OK;
+}
+
+int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const double *source,
+                            const allium_si_options *opt, double *phi, double *psi, double *norms, int *iterations)
+{
+    if (!h || !sigma_s || !source || !opt || !phi) return fail(ALLIUM_ERR_ARG, "null argument");
+    if (opt->maxit < 1) return fail(ALLIUM_ERR_ARG, "maxit < 1");
+    if (!finite_all(sigma_s, h->mesh.Ne)) return fail(ALLIUM_ERR_NUMERIC, "non-finite sigma_s");
+    hipStream_t s = h->stream;
+    const long np = h->phi_size, nps = h->psi_size;
+    HIP_TRY(h->q.ensure(sizeof(double) * nps));
+    HIP_TRY(h->a.ensure(sizeof(double) * nps));
+    HIP_TRY(h->b.ensure(sizeof(double) * np));
+    HIP_TRY(h->x.ensure(sizeof(double) * np));
+    HIP_TRY(h->y.ensure(sizeof(double) * np));
+    HIP_TRY(h->iso.ensure(sizeof(double) * np));
+    HIP_TRY(h->coef.ensure(sizeof(double) * h->mesh.Ne));
+    const int nb = allium::si_update_blocks(np);
+    HIP_TRY(h->part.ensure(sizeof(double) * nb));
+    HIP_TRY(hipMemcpyAsync(h->q.p, source, sizeof(double) * nps, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(h->coef.p, sigma_s, sizeof(double) * h->mesh.Ne, hipMemcpyHostToDevice, s));
+    const double inv_w = 1.0 / h->quad.wsum;
+    const int prec_save = h->ds.precision;
+    int rc = ALLIUM_OK;
+    std::vector<double> hp(nb);
+    double r0 = 0.0;
+    int it = 0;
+    if (opt->precision == ALLIUM_FP32_REFINED) {
+        // defect correction: psi += L32^{-1} (q + D^T Ms phi - L psi), residuals in FP64
+        HIP_TRY(h->c.ensure(sizeof(double) * nps));
+        HIP_TRY(hipMemsetAsync(h->a.p, 0, sizeof(double) * nps, s));
+        HIP_TRY(hipMemsetAsync(h->x.p, 0, sizeof(double) * np, s));
+        h->ds.precision = 1;
+        for (it = 1;; it++) {
+            HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
+            HIP_TRY(allium::launch_apply_L(h->ds.prm, h->a.d(), h->c.d(), s));
+            HIP_TRY(allium::launch_axpby(nps, 1.0, h->q.d(), -1.0, h->c.d(), h->c.d(), s));
+            rc = do_sweep(h, h->c.d(), h->iso.d(), h->c.d(), s);
+            if (rc) break;
+            HIP_TRY(allium::launch_axpby(nps, 1.0, h->a.d(), 1.0, h->c.d(), h->a.d(), s));
+            HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->y.d(), s));
+            // r = D psi_{k+1} - phi_k ; phi_{k+1} = D psi_{k+1}
+            HIP_TRY(allium::launch_si_update(np, h->y.d(), nullptr, h->x.d(), h->part.d(), s));
+            HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            double nrm = 0.0;
+            for (int i = 0; i < nb; i++) nrm += hp[i];
+            nrm = std::sqrt(nrm);
+            if (norms) norms[it - 1] = nrm;
+            if (!std::isfinite(nrm)) { rc = fail(ALLIUM_ERR_NUMERIC, "non-finite residual"); break; }
+            if (it == 1) r0 = std::max(nrm * opt->reltol, opt->abstol);
+            if (nrm < r0 || it >= opt->maxit) break;
+        }
+        h->ds.precision = prec_save;
+        if (rc) return rc;
+        // psi holds the FP64-corrected angular flux of the last iterate
+        if (psi) HIP_TRY(hipMemcpyAsync(psi, h->a.p, sizeof(double) * nps, hipMemcpyDeviceToHost, s));
+    } else {
+        h->ds.precision = 0;
+        rc = do_sweep(h, h->q.d(), nullptr, h->a.d(), s);
+        if (rc) { h->ds.precision = prec_save; return rc; }
+        HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->b.d(), s));
+        HIP_TRY(hipMemsetAsync(h->x.p, 0, sizeof(double) * np, s));
+        for (it = 1;; it++) {
+            HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
+            rc = do_sweep(h, nullptr, h->iso.d(), h->a.d(), s);
+            if (rc) break;
+            HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->y.d(), s));
+            HIP_TRY(allium::launch_si_update(np, h->b.d(), h->y.d(), h->x.d(), h->part.d(), s));
+            HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            double nrm = 0.0;
+            for (int i = 0; i < nb; i++) nrm += hp[i];
+            nrm = std::sqrt(nrm);
+            if (norms) norms[it - 1] = nrm;
+            if (!std::isfinite(nrm)) { rc = fail(ALLIUM_ERR_NUMERIC, "non-finite residual"); break; }
+            if (it == 1) r0 = std::max(nrm * opt->reltol, opt->abstol);
+            if (nrm < r0 || it >= opt->maxit) break;
+        }
+        if (rc) { h->ds.precision = prec_save; return rc; }
+        // final sweep: psi = L^{-1}(D^T Ms phi + q)
+        HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
+        rc = do_sweep(h, h->q.d(), h->iso.d(), h->a.d(), s);
+        h->ds.precision = prec_save;
+        if (rc) return rc;
+        if (psi) HIP_TRY(hipMemcpyAsync(psi, h->a.p, sizeof(double) * nps, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipMemcpyAsync(phi, h->x.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (iterations) *iterations = it;
+    const bool converged = norms ? (norms[it - 1] < r0) : true;
+    if (!converged) return fail(ALLIUM_ERR_NOT_CONVERGED, "source iteration reached maxit");
+    return ALLIUM_OK;
+}
+
+int allium_sweep_timing(allium_sweep_t h, double *sweep_seconds, long *sweeps)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    if (sweep_seconds) *sweep_seconds = h->ds.sweep_seconds;
+    if (sweeps) *sweeps = h->ds.sweeps;
+    return ALLIUM_OK;
+}
+
+// ---- host-side setup queries ------------------------------------------------
+static int host_quad(const allium_quad_desc *quad, int dim, allium::Quadrature &q)
+{
+    std::string why;
+    if (quad->nomega > 0) {
+        q.dim = dim;
+        for (int a = 0; a < quad->nomega; a++) {
+            for (int d = 0; d < 3; d++) q.omega.push_back(d < dim ? quad->omega[a * dim + d] : 0.0);
+            q.w.push_back(quad->weights[a]);
+            q.wsum += quad->weights[a];
+        }
+        return ALLIUM_OK;
+    }
+    if (!allium::level_symmetric(quad->order, dim, q, why)) return fail(ALLIUM_ERR_ARG, why);
+    return ALLIUM_OK;
+}
+
+int allium_host_quadrature(int order, int dim, double *omega, double *weights, int *nomega)
+{
+    if (!nomega) return fail(ALLIUM_ERR_ARG, "null argument");
+    allium::Quadrature q;
+    std::string why;
+    if (!allium::level_symmetric(order, dim, q, why)) return fail(ALLIUM_ERR_ARG, why);
+    if (*nomega < q.size()) { *nomega = q.size(); return fail(ALLIUM_ERR_ARG, "capacity too small"); }
+    *nomega = q.size();
+    for (int a = 0; a < q.size(); a++) {
+        for (int d = 0; d < dim; d++) omega[a * dim + d] = q.omega[3 * a + d];
+        weights[a] = q.w[a];
+    }
+    return ALLIUM_OK;
+}
+
+int allium_host_levels(const allium_mesh_desc *mesh, const allium_quad_desc *quad, int *level_of_node, long *nlevels)
+{
+    if (!mesh || !quad || !level_of_node) return fail(ALLIUM_ERR_ARG, "null argument");
+    allium::Mesh m;
+    allium::Quadrature q;
+    allium::SweepPlan plan;
+    std::string why;
+    if (!allium::build_mesh(mesh->dim, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, m, why))
+        return fail(ALLIUM_ERR_ARG, why);
+    int rc = host_quad(quad, mesh->dim, q);
+    if (rc) return rc;
+    int st = allium::build_plan(m, q, plan, why);
+    if (st == -2) return fail(ALLIUM_ERR_NOT_DAG, why);
+    if (st == -3) return fail(ALLIUM_ERR_REFLECT, why);
+    if (st) return fail(ALLIUM_ERR_ARG, why);
+    std::memcpy(level_of_node, plan.level.data(), sizeof(int) * plan.level.size());
+    if (nlevels) *nlevels = plan.nlevels;
+    return ALLIUM_OK;
+}
+
+int allium_host_form_source(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad, int G,
+                            const double *Q, const double *inflow, double *source)
+{
+    if (!mesh || !quad || !source || G < 1) return fail(ALLIUM_ERR_ARG, "bad argument");
+    allium::Mesh m;
+    allium::Quadrature q;
+    allium::ElementData ed;
+    std::string why;
+    if (!allium::build_mesh(mesh->dim, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, m, why))
+        return fail(ALLIUM_ERR_ARG, why);
+    int rc = host_quad(quad, mesh->dim, q);
+    if (rc) return rc;
+    if (!allium::assemble(m, fe_order, 0, ed, why)) return fail(ALLIUM_ERR_ARG, why);
+    allium::form_source(m, ed, q, G, Q, inflow, source);
+    return ALLIUM_OK;
+}
+
+int allium_host_element_matrices(const allium_mesh_desc *mesh, int fe_order, int e, double *mass, double *grad,
+                                 double *tmass)
+{
+    if (!mesh) return fail(ALLIUM_ERR_ARG, "null argument");
+    allium::Mesh m;
+    allium::ElementData ed;
+    std::string why;
+    if (!allium::build_mesh(mesh->dim, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, m, why))
+        return fail(ALLIUM_ERR_ARG, why);
+    if (!allium::assemble(m, fe_order, 0, ed, why)) return fail(ALLIUM_ERR_ARG, why);
+    if (e < 0 || e >= m.Ne) return fail(ALLIUM_ERR_ARG, "element out of range");
+    const size_t nn = (size_t)ed.n * ed.n, k = m.uniform ? 0 : (size_t)e;
+    if (mass) std::memcpy(mass, &ed.Mu[k * nn], sizeof(double) * nn);
+    if (tmass) std::memcpy(tmass, &ed.Mm[k * nn], sizeof(double) * nn);
+    if (grad) std::memcpy(grad, &ed.Gd[k * m.dim * nn], sizeof(double) * m.dim * nn);
+    return ALLIUM_OK;
+}
+
+// ---- device plumbing (include/allium_device.h) ------------------------------
+int allium_device_count(int *count)
+{
+    HIP_TRY(hipGetDeviceCount(count));
+    return ALLIUM_OK;
+}
+
+int allium_device_name(char *buf, int len)
+{
+    hipDeviceProp_t prop;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return ALLIUM_OK;
+}
+
+int allium_device_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return fail(ALLIUM_ERR_ARG, "null argument");
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+    return ALLIUM_OK;
+}
+
+int allium_device_free(void *ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return ALLIUM_OK;
+}
+
+int allium_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return ALLIUM_OK;
+}
+
+int allium_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return ALLIUM_OK;
+}
+
+int allium_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return ALLIUM_OK;
+}
+
+int allium_device_synchronize(void)
+{
+    HIP_TRY(hipDeviceSynchronize());
+    return ALLIUM_OK;
+}
+
+int allium_event_timer(int op, double *ms)
+{
+    // op 0: record start on the null stream, op 1: record stop and return ms
+    static hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!e0) { HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1)); }
+    if (op == 0) { HIP_TRY(hipEventRecord(e0, nullptr)); return ALLIUM_OK; }
+    HIP_TRY(hipEventRecord(e1, nullptr));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    return ALLIUM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,790 @@
+// HIP kernels of the DG upwind sweep for CDNA4 (gfx950).
+//
+// One work item is one cell (element e, direction a, group g): it assembles
+// the local DG system  A = sum_d Omega_d G_d + sum_{outflow} (Omega.n) F + sigma_g M [+ tau Mt],
+// gathers the upwind face traces from HBM into the right-hand side and solves
+// with an unrolled register LU (src/sweep.cpp:366-475).  Cells of one
+// topological level are independent; the launchers below run the levels as
+// separate launches, as one captured hipGraph, or inside one persistent kernel
+// whose workgroups take tiles in level order from an atomic ticket and wait on
+// per-(level, direction) completion counters.
+#include "device.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace allium {
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+template <int DIM, int P>
+struct Sh {
+    static constexpr int Q = P + 1;
+    static constexpr int N = DIM == 2 ? Q * Q : Q * Q * Q;
+    static constexpr int NF = DIM == 2 ? Q : Q * Q;
+    static constexpr int NS = 2 * DIM;
+};
+
+// k-th node (face-lexicographic) of element side s = 2*axis + {0 low, 1 high}
+template <int DIM, int P>
+__host__ __device__ constexpr int side_node(int s, int k)
+{
+    const int ax = s >> 1, hi = s & 1, Q = P + 1;
+    int idx[3] = {0, 0, 0};
+    int r = k;
+    for (int d = 0; d < DIM; ++d) {
+        if (d == ax) idx[d] = hi ? P : 0;
+        else { idx[d] = r % Q; r /= Q; }
+    }
+    return idx[0] + Q * (idx[1] + Q * idx[2]);
+}
+
+__device__ __forceinline__ int fast_div(int a, int d, float inv)
+{
+    int q = (int)((float)a * inv);
+    const int r = a - q * d;
+    if (r < 0) q--;
+    else if (r >= d) q++;
+    return q;
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void lu_factor_solve(T (&A)[N][N], T (&x)[N])
+{
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const T inv = T(1) / A[k][k];
+        A[k][k] = inv;
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            const T l = A[i][k] * inv;
+            A[i][k] = l;
+#pragma unroll
+            for (int j = k + 1; j < N; ++j) A[i][j] = fma(-l, A[k][j], A[i][j]);
+            x[i] = fma(-l, x[k], x[i]);
+        }
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        T s = x[i];
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) s = fma(-A[i][j], x[j], s);
+        x[i] = s * A[i][i];
+    }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void lu_resolve(const T (&A)[N][N], T (&x)[N])
+{
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) x[i] = fma(-A[i][k], x[k], x[i]);
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        T s = x[i];
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) s = fma(-A[i][j], x[j], s);
+        x[i] = s * A[i][i];
+    }
+}
+
+// Entry (i,j) of the FP64 local matrix, recomputed from the stored tables
+// (used by the FP64 residual of the refinement step, uniform geometry only).
+template <int DIM, int P>
+__device__ __forceinline__ double uniform_entry(const SweepParams &p, const double *K, double sig,
+                                                int i, int j)
+{
+    constexpr int N = Sh<DIM, P>::N;
+    double v = fma(sig, p.Mu[i * N + j], K[i * N + j]);
+    if (p.timedep) v = fma(p.tau, p.Mt[i * N + j], v);
+    return v;
+}
+
+template <int DIM, int P, bool UNI, typename T>
+__device__ __forceinline__ void cell_solve(const SweepParams &p, const int e, const int a, const int g)
+{
+    using S = Sh<DIM, P>;
+    constexpr int N = S::N, NF = S::NF, NS = S::NS;
+    int c[3];
+    {
+        const int q01 = (DIM == 3) ? fast_div(e, p.stride[2], p.inv_nel01) : 0;
+        const int rem = e - q01 * p.stride[2];
+        const int q0 = fast_div(rem, p.nel[0], p.inv_nel0);
+        c[0] = rem - q0 * p.nel[0];
+        c[1] = q0;
+        c[2] = q01;
+    }
+    const double om[3] = {p.omega[4 * a + 0], p.omega[4 * a + 1], p.omega[4 * a + 2]};
+    const long long base = ((long long)g * p.Nomega + a) * p.Ndof;
+    const long long off = base + (long long)e * N;
+    const double sig = p.sigma[(long long)g * p.Ne + e];
+
+    T A[N][N];
+    const double *K = UNI ? p.K + (size_t)a * N * N : nullptr;
+    if constexpr (UNI) {
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int j = 0; j < N; ++j) A[i][j] = uniform_entry<DIM, P>(p, K, sig, i, j);
+        } else {
+            const float *Kf = p.Kf + (size_t)a * N * N;
+            const float fs = (float)sig, ft = (float)p.tau;
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    float v = fmaf(fs, p.Muf[i * N + j], Kf[i * N + j]);
+                    if (p.timedep) v = fmaf(ft, p.Mtf[i * N + j], v);
+                    A[i][j] = v;
+                }
+        }
+    } else {
+        const double *Gd = p.Gd + (size_t)e * DIM * N * N;
+        const double *Mu = p.Mu + (size_t)e * N * N;
+        const double *Mt = p.Mt + (size_t)e * N * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                double v = om[0] * Gd[i * N + j];
+#pragma unroll
+                for (int d = 1; d < DIM; ++d) v = fma(om[d], Gd[(d * N + i) * N + j], v);
+                v = fma(sig, Mu[i * N + j], v);
+                if (p.timedep) v = fma(p.tau, Mt[i * N + j], v);
+                A[i][j] = (T)v;
+            }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int ax = s >> 1, hi = s & 1;
+            int cf[3] = {c[0], c[1], c[2]};
+            cf[ax] += hi;
+            const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
+            const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
+            const double sgn = (!hi && interior) ? -1.0 : 1.0;
+            double dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
+            dot *= sgn;
+            if (dot > 0) {
+                const double *F = p.Fface + (size_t)f * NF * NF;
+#pragma unroll
+                for (int k = 0; k < NF; ++k)
+#pragma unroll
+                    for (int l = 0; l < NF; ++l)
+                        A[side_node<DIM, P>(s, k)][side_node<DIM, P>(s, l)] += (T)(dot * F[k * NF + l]);
+            }
+        }
+    }
+
+    // right-hand side: source, isotropic source, upwind face traces
+    double b[N];
+    if (p.src) {
+        if constexpr (N % 2 == 0) {
+            const d2v *q2 = reinterpret_cast<const d2v *>(p.src + off);
+#pragma unroll
+            for (int i = 0; i < N / 2; ++i) { const d2v v = q2[i]; b[2 * i] = v.x; b[2 * i + 1] = v.y; }
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) b[i] = p.src[off + i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) b[i] = 0.0;
+    }
+    if (p.iso) {
+        const double *is = p.iso + (long long)g * p.Ndof + (long long)e * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) b[i] += is[i];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int ax = s >> 1, hi = s & 1;
+        const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
+        double dot;
+        const double *F;
+        if constexpr (UNI) {
+            dot = hi ? om[ax] : -om[ax];
+            F = p.Fface + s * NF * NF;
+        } else {
+            int cf[3] = {c[0], c[1], c[2]};
+            cf[ax] += hi;
+            const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
+            const double sgn = (!hi && interior) ? -1.0 : 1.0;
+            dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
+            dot *= sgn;
+            F = p.Fface + (size_t)f * NF * NF;
+        }
+        if (dot < 0) {
+            const double *up = nullptr;
+            int us = s ^ 1;
+            if (interior) {
+                const int nb = hi ? e + p.stride[ax] : e - p.stride[ax];
+                up = p.psi + base + (long long)nb * N;
+            } else if ((p.refl_mask >> s) & 1) {
+                const int ra = p.refl[a * NS + s];
+                up = p.psi + ((long long)g * p.Nomega + ra) * p.Ndof + (long long)e * N;
+                us = s;
+            }
+            if (up) {
+                double u[NF];
+                if (us == (s ^ 1)) {
+#pragma unroll
+                    for (int l = 0; l < NF; ++l) u[l] = up[side_node<DIM, P>(s ^ 1, l)];
+                } else {
+#pragma unroll
+                    for (int l = 0; l < NF; ++l) u[l] = up[side_node<DIM, P>(s, l)];
+                }
+#pragma unroll
+                for (int k = 0; k < NF; ++k) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], u[l], acc);
+                    b[side_node<DIM, P>(s, k)] = fma(-dot, acc, b[side_node<DIM, P>(s, k)]);
+                }
+            }
+        }
+    }
+
+    T x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = (T)b[i];
+    lu_factor_solve<N, T>(A, x);
+    double xs[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) xs[i] = (double)x[i];
+    if constexpr (sizeof(T) == 4 && UNI) {
+        // FP64 residual r = b - A x with A rebuilt in FP64, correction by the
+        // FP32 factors
+        for (int it = 0; it < p.refine; ++it) {
+            T r[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double acc = b[i];
+#pragma unroll
+                for (int j = 0; j < N; ++j) acc = fma(-uniform_entry<DIM, P>(p, K, sig, i, j), xs[j], acc);
+                r[i] = (T)acc;
+            }
+            lu_resolve<N, T>(A, r);
+#pragma unroll
+            for (int i = 0; i < N; ++i) xs[i] += (double)r[i];
+        }
+    }
+    if constexpr (N % 2 == 0) {
+        d2v *o2 = reinterpret_cast<d2v *>(p.psi + off);
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) { d2v v; v.x = xs[2 * i]; v.y = xs[2 * i + 1]; o2[i] = v; }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) p.psi[off + i] = xs[i];
+    }
+}
+
+template <int DIM, int P, bool UNI, typename T>
+__global__ void __launch_bounds__(256) k_sweep_level(SweepParams p, long long cbeg, int ncells)
+{
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= ncells * p.G) return;
+    int g = 0, c = item;
+    if (p.G > 1) { g = item / ncells; c = item - g * ncells; }
+    const uint32_t cell = p.cells[cbeg + c];
+    cell_solve<DIM, P, UNI, T>(p, (int)(cell >> 10), (int)(cell & 1023u), g);
+}
+
+// Persistent sweep.  Tiles are handed out in level order by an atomic ticket,
+// so any tile a workgroup waits on was taken earlier by a running workgroup:
+// the wait always ends.  Each wait is also bounded by spin_limit; on overrun the
+// error flag is raised and the workgroup proceeds, so the grid always drains.
+template <int DIM, int P, bool UNI, typename T>
+__global__ void __launch_bounds__(256) k_sweep_persistent(SweepParams p, const Tile *__restrict__ tiles,
+                                                          int ntiles, int *counters, const int *need,
+                                                          int nkeys, long long spin_limit)
+{
+    __shared__ int s_t;
+    int *ticket = counters + nkeys;
+    int *err = counters + nkeys + 1;
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int t = s_t;
+        __syncthreads();
+        if (t >= ntiles) return;
+        const Tile tl = tiles[t];
+        if (threadIdx.x == 0 && tl.wait_key >= 0) {
+            const int target = need[tl.wait_key];
+            long long spins = 0;
+            while (__hip_atomic_load(&counters[tl.wait_key], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                   target) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > spin_limit) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                // once any wait has overrun, stop waiting so the grid drains at once
+                if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        for (int i = threadIdx.x; i < tl.nitems; i += blockDim.x) {
+            const int item = tl.item_begin + i;
+            int g = 0, c = item;
+            if (p.G > 1) { g = item / tl.ncells; c = item - g * tl.ncells; }
+            const uint32_t cell = p.cells[tl.cell_begin + c];
+            cell_solve<DIM, P, UNI, T>(p, (int)(cell >> 10), (int)(cell & 1023u), g);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_add(&counters[tl.key], tl.nitems, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_fill_int(int *p, int n, int v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// dispatch over (dim, p, uniform, precision)
+// ---------------------------------------------------------------------------
+bool sweep_supported(int dim, int p, std::string &why)
+{
+    if (dim == 2 && p >= 0 && p <= 2) return true;
+    if (dim == 3 && p >= 0 && p <= 1) return true;
+    why = "GPU sweep implements 2D p=0..2 and 3D p=0..1";
+    return false;
+}
+
+template <int DIM, int P, bool UNI, typename T>
+static hipError_t run_levels(DeviceSweep &ds, const SweepParams &prm, hipStream_t stream)
+{
+    for (int L = 0; L < ds.nlevels; ++L) {
+        const long long cb = ds.level_ptr[L];
+        const int nc = (int)(ds.level_ptr[L + 1] - cb);
+        const long items = (long)nc * prm.G;
+        if (items == 0) continue;
+        const int grid = (int)((items + ds.block - 1) / ds.block);
+        hipLaunchKernelGGL((k_sweep_level<DIM, P, UNI, T>), dim3(grid), dim3(ds.block), 0, stream, prm, cb, nc);
+    }
+    return hipGetLastError();
+}
+
+template <int DIM, int P, bool UNI, typename T>
+static hipError_t run_persistent(DeviceSweep &ds, const SweepParams &prm, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_fill_int, dim3(std::max(1, std::min(1024, (ds.nkeys + 2 + 255) / 256))), dim3(256), 0,
+                       stream, ds.d_counters, ds.nkeys + 2, 0);
+    int dev = 0, ncu = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int grid = std::max(1, std::min(ds.ntiles, ncu * 4));
+    hipLaunchKernelGGL((k_sweep_persistent<DIM, P, UNI, T>), dim3(grid), dim3(ds.persistent_tile), 0, stream, prm,
+                       ds.d_tiles, ds.ntiles, ds.d_counters, ds.d_need, ds.nkeys, (long long)(1LL << 21));
+    return hipGetLastError();
+}
+
+template <int DIM, int P, bool UNI, typename T>
+static hipError_t run_mode(DeviceSweep &ds, const SweepParams &prm, hipStream_t stream, int mode)
+{
+    if (mode == 2) return run_persistent<DIM, P, UNI, T>(ds, prm, stream);
+    return run_levels<DIM, P, UNI, T>(ds, prm, stream);
+}
+
+template <int DIM, int P>
+static hipError_t dispatch_dp(DeviceSweep &ds, const SweepParams &prm, hipStream_t s, int mode)
+{
+    if (prm.uniform) {
+        if (ds.precision == 1) return run_mode<DIM, P, true, float>(ds, prm, s, mode);
+        return run_mode<DIM, P, true, double>(ds, prm, s, mode);
+    }
+    if (ds.precision == 1) return run_mode<DIM, P, false, float>(ds, prm, s, mode);
+    return run_mode<DIM, P, false, double>(ds, prm, s, mode);
+}
+
+static hipError_t dispatch(DeviceSweep &ds, const SweepParams &prm, hipStream_t s, int mode)
+{
+    if (prm.dim == 2) {
+        switch (prm.p) {
+        case 0: return dispatch_dp<2, 0>(ds, prm, s, mode);
+        case 1: return dispatch_dp<2, 1>(ds, prm, s, mode);
+        case 2: return dispatch_dp<2, 2>(ds, prm, s, mode);
+        }
+    } else if (prm.dim == 3) {
+        switch (prm.p) {
+        case 0: return dispatch_dp<3, 0>(ds, prm, s, mode);
+        case 1: return dispatch_dp<3, 1>(ds, prm, s, mode);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t reset_persistent(DeviceSweep &ds, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, stream, ds.d_counters, ds.nkeys + 2, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, double *psi, hipStream_t stream)
+{
+    SweepParams prm = ds.prm;
+    prm.src = src;
+    prm.iso = iso;
+    prm.psi = psi;
+    hipError_t err = hipSuccess;
+    if (ds.timing) hipEventRecord(ds.ev0, stream);
+    if (ds.launch_mode == 1) {
+        // hipGraph of the level launches, re-captured when the vectors change
+        if (!ds.graph_exec || ds.graph_src != src || ds.graph_iso != iso || ds.graph_psi != psi ||
+            ds.graph_prec != ds.precision) {
+            if (ds.graph_exec) { hipGraphExecDestroy(ds.graph_exec); ds.graph_exec = nullptr; }
+            if (!ds.graph_stream) hipStreamCreateWithFlags(&ds.graph_stream, hipStreamNonBlocking);
+            hipGraph_t graph;
+            err = hipStreamBeginCapture(ds.graph_stream, hipStreamCaptureModeThreadLocal);
+            if (err != hipSuccess) return err;
+            dispatch(ds, prm, ds.graph_stream, 0);
+            err = hipStreamEndCapture(ds.graph_stream, &graph);
+            if (err != hipSuccess) return err;
+            err = hipGraphInstantiate(&ds.graph_exec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (err != hipSuccess) return err;
+            ds.graph_src = src; ds.graph_iso = iso; ds.graph_psi = psi; ds.graph_prec = ds.precision;
+        }
+        err = hipGraphLaunch(ds.graph_exec, stream);
+    } else {
+        err = dispatch(ds, prm, stream, ds.launch_mode);
+    }
+    if (ds.timing) hipEventRecord(ds.ev1, stream);
+    ds.sweeps++;
+    return err;
+}
+
+// ---------------------------------------------------------------------------
+// streaming operator L (AdvectionOperator::Mult, src/sweep.cpp:1246-1387),
+// one thread per (element, direction, group)
+// ---------------------------------------------------------------------------
+template <int DIM, int P, bool UNI>
+__global__ void __launch_bounds__(256) k_apply_L(SweepParams p, const double *__restrict__ x, double *__restrict__ y)
+{
+    using S = Sh<DIM, P>;
+    constexpr int N = S::N, NF = S::NF, NS = S::NS;
+    const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)p.Ne * p.Nomega * p.G;
+    if (item >= total) return;
+    const int e = (int)(item % p.Ne);
+    const int a = (int)((item / p.Ne) % p.Nomega);
+    const int g = (int)(item / ((long long)p.Ne * p.Nomega));
+    int c[3] = {e % p.nel[0], (e / p.nel[0]) % p.nel[1], e / (p.nel[0] * p.nel[1])};
+    const double om[3] = {p.omega[4 * a], p.omega[4 * a + 1], p.omega[4 * a + 2]};
+    const long long base = ((long long)g * p.Nomega + a) * p.Ndof;
+    const double *xe = x + base + (long long)e * N;
+    const double sig = p.sigma[(long long)g * p.Ne + e];
+    double xl[N], yl[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) xl[i] = xe[i];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double v;
+            if constexpr (UNI) {
+                v = fma(sig, p.Mu[i * N + j], p.K[((size_t)a * N + i) * N + j]);
+                if (p.timedep) v = fma(p.tau, p.Mt[i * N + j], v);
+            } else {
+                const double *Gd = p.Gd + (size_t)e * DIM * N * N;
+                v = om[0] * Gd[i * N + j];
+#pragma unroll
+                for (int d = 1; d < DIM; ++d) v = fma(om[d], Gd[(d * N + i) * N + j], v);
+                v = fma(sig, p.Mu[(size_t)e * N * N + i * N + j], v);
+                if (p.timedep) v = fma(p.tau, p.Mt[(size_t)e * N * N + i * N + j], v);
+            }
+            acc = fma(v, xl[j], acc);
+        }
+        yl[i] = acc;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int ax = s >> 1, hi = s & 1;
+        const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
+        double dot;
+        const double *F;
+        if constexpr (UNI) {
+            dot = hi ? om[ax] : -om[ax];
+            F = p.Fface + s * NF * NF;
+        } else {
+            int cf[3] = {c[0], c[1], c[2]};
+            cf[ax] += hi;
+            const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
+            const double sgn = (!hi && interior) ? -1.0 : 1.0;
+            dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
+            dot *= sgn;
+            F = p.Fface + (size_t)f * NF * NF;
+            if (dot > 0) {  // outflow term (uniform geometry has it inside K)
+#pragma unroll
+                for (int k = 0; k < NF; ++k) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], xl[side_node<DIM, P>(s, l)], acc);
+                    yl[side_node<DIM, P>(s, k)] = fma(dot, acc, yl[side_node<DIM, P>(s, k)]);
+                }
+            }
+        }
+        if (dot < 0 && interior) {
+            const int nb = hi ? e + p.stride[ax] : e - p.stride[ax];
+            const double *xn = x + base + (long long)nb * N;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], xn[side_node<DIM, P>(s ^ 1, l)], acc);
+                yl[side_node<DIM, P>(s, k)] = fma(dot, acc, yl[side_node<DIM, P>(s, k)]);
+            }
+        }
+    }
+    double *ye = y + base + (long long)e * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) ye[i] = yl[i];
+}
+
+template <int DIM, int P>
+static hipError_t apply_L_dp(const SweepParams &prm, const double *x, double *y, hipStream_t s)
+{
+    const long long total = (long long)prm.Ne * prm.Nomega * prm.G;
+    const int grid = (int)((total + 255) / 256);
+    if (prm.uniform) hipLaunchKernelGGL((k_apply_L<DIM, P, true>), dim3(grid), dim3(256), 0, s, prm, x, y);
+    else hipLaunchKernelGGL((k_apply_L<DIM, P, false>), dim3(grid), dim3(256), 0, s, prm, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_L(const SweepParams &prm, const double *x, double *y, hipStream_t s)
+{
+    if (prm.dim == 2) {
+        if (prm.p == 0) return apply_L_dp<2, 0>(prm, x, y, s);
+        if (prm.p == 1) return apply_L_dp<2, 1>(prm, x, y, s);
+        if (prm.p == 2) return apply_L_dp<2, 2>(prm, x, y, s);
+    } else {
+        if (prm.p == 0) return apply_L_dp<3, 0>(prm, x, y, s);
+        if (prm.p == 1) return apply_L_dp<3, 1>(prm, x, y, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// particle balance (ComputeBalance, src/sweep.cpp:1400-1462): per (g, e)
+// absorption sum_a w_a 1^T (sigma M + tau Mt) psi and boundary leakage
+// sum_{dot >= 0} w_a dot 1^T F psi, block partials in FP64
+// ---------------------------------------------------------------------------
+constexpr int kBalanceBlock = 256;
+
+template <int DIM, int P>
+__global__ void __launch_bounds__(kBalanceBlock) k_balance(SweepParams p, const double *__restrict__ psi,
+                                                           double *__restrict__ partial)
+{
+    using S = Sh<DIM, P>;
+    constexpr int N = S::N, NF = S::NF, NS = S::NS;
+    __shared__ double sa[kBalanceBlock], sl[kBalanceBlock];
+    const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double absorb = 0.0, leak = 0.0;
+    if (item < (long long)p.Ne * p.G) {
+        const int e = (int)(item % p.Ne), g = (int)(item / p.Ne);
+        const double sig = p.sigma[(long long)g * p.Ne + e];
+        const double *Mu = p.uniform ? p.Mu : p.Mu + (size_t)e * N * N;
+        const double *Mt = p.uniform ? p.Mt : p.Mt + (size_t)e * N * N;
+        double col[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double v = sig * Mu[i * N + j];
+                if (p.timedep) v = fma(p.tau, Mt[i * N + j], v);
+                s += v;
+            }
+            col[j] = s;
+        }
+        int c[3] = {e % p.nel[0], (e / p.nel[0]) % p.nel[1], e / (p.nel[0] * p.nel[1])};
+        for (int a = 0; a < p.Nomega; ++a) {
+            const double *pe = psi + ((long long)g * p.Nomega + a) * p.Ndof + (long long)e * N;
+            const double wa = p.omega[4 * a + 3];
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) s = fma(col[j], pe[j], s);
+            absorb = fma(s, wa, absorb);
+#pragma unroll
+            for (int sd = 0; sd < NS; ++sd) {
+                const int ax = sd >> 1, hi = sd & 1;
+                const bool bdr = hi ? (c[ax] == p.nel[ax] - 1) : (c[ax] == 0);
+                if (!bdr) continue;
+                double dot;
+                const double *F;
+                if (p.uniform) {
+                    dot = hi ? p.omega[4 * a + ax] : -p.omega[4 * a + ax];
+                    F = p.Fface + sd * NF * NF;
+                } else {
+                    int cf[3] = {c[0], c[1], c[2]};
+                    cf[ax] += hi;
+                    const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
+                    dot = 0.0;
+                    for (int d = 0; d < DIM; ++d) dot = fma(p.omega[4 * a + d], p.fnor[3 * f + d], dot);
+                    F = p.Fface + (size_t)f * NF * NF;
+                }
+                if (dot >= 0) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = 0; k < NF; ++k)
+#pragma unroll
+                        for (int l = 0; l < NF; ++l) t = fma(F[k * NF + l], pe[side_node<DIM, P>(sd, l)], t);
+                    leak = fma(wa * dot, t, leak);
+                }
+            }
+        }
+    }
+    sa[threadIdx.x] = absorb;
+    sl[threadIdx.x] = leak;
+    __syncthreads();
+    for (int w = kBalanceBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { sa[threadIdx.x] += sa[threadIdx.x + w]; sl[threadIdx.x] += sl[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = sa[0]; partial[2 * blockIdx.x + 1] = sl[0]; }
+}
+
+int balance_blocks(const SweepParams &prm)
+{
+    return (int)(((long long)prm.Ne * prm.G + kBalanceBlock - 1) / kBalanceBlock);
+}
+
+hipError_t launch_balance(const SweepParams &prm, const double *psi, double *partial, hipStream_t s)
+{
+    const int grid = balance_blocks(prm);
+#define BAL(D, PP) hipLaunchKernelGGL((k_balance<D, PP>), dim3(grid), dim3(kBalanceBlock), 0, s, prm, psi, partial)
+    if (prm.dim == 2) {
+        if (prm.p == 0) BAL(2, 0);
+        else if (prm.p == 1) BAL(2, 1);
+        else if (prm.p == 2) BAL(2, 2);
+    } else {
+        if (prm.p == 0) BAL(3, 0);
+        else if (prm.p == 1) BAL(3, 1);
+    }
+#undef BAL
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// moments (DiscreteToMoment, src/transport_op.cpp:3-46) and block mass
+// ---------------------------------------------------------------------------
+__global__ void k_moments(const double *__restrict__ psi, const double *__restrict__ omega, int Nomega,
+                          long long Ndof, int G, double *__restrict__ phi)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ndof * G) return;
+    const long long g = i / Ndof, s = i - g * Ndof;
+    const double *ps = psi + g * Nomega * Ndof + s;
+    double acc = 0.0;
+    for (int a = 0; a < Nomega; ++a) acc = fma(omega[4 * a + 3], ps[(long long)a * Ndof], acc);
+    phi[i] = acc;
+}
+
+hipError_t launch_moments(const SweepParams &prm, const double *psi, double *phi, hipStream_t s)
+{
+    const long long n = prm.Ndof * prm.G;
+    hipLaunchKernelGGL(k_moments, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, psi, prm.omega, prm.Nomega,
+                       prm.Ndof, prm.G, phi);
+    return hipGetLastError();
+}
+
+__global__ void k_moments_T(const double *__restrict__ phi, double inv_wsum, int Nomega, long long Ndof, int G,
+                            double *__restrict__ psi)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = Ndof * Nomega * G;
+    if (i >= tot) return;
+    const long long s = i % Ndof, g = i / (Ndof * Nomega);
+    psi[i] = phi[g * Ndof + s] * inv_wsum;
+}
+
+hipError_t launch_moments_T(const SweepParams &prm, double inv_wsum, const double *phi, double *psi, hipStream_t s)
+{
+    const long long n = prm.Ndof * prm.Nomega * prm.G;
+    hipLaunchKernelGGL(k_moments_T, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phi, inv_wsum, prm.Nomega,
+                       prm.Ndof, prm.G, psi);
+    return hipGetLastError();
+}
+
+__global__ void k_mass(const double *__restrict__ Mm, int uniform, int n, int Ne, int G, const double *__restrict__ coef,
+                       double scale, const double *__restrict__ x, double *__restrict__ y)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)Ne * G * n) return;
+    const int i = (int)(t % n);
+    const long long ge = t / n;
+    const int e = (int)(ge % Ne);
+    const double *M = uniform ? Mm : Mm + (size_t)e * n * n;
+    const double *xe = x + ge * n;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc = fma(M[i * n + j], xe[j], acc);
+    y[t] = (coef[e] * acc) * scale;
+}
+
+hipError_t launch_mass(const SweepParams &prm, const double *Mm, const double *coef, double scale, const double *x,
+                       double *y, hipStream_t s)
+{
+    const long long n = (long long)prm.Ne * prm.G * prm.n;
+    hipLaunchKernelGGL(k_mass, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Mm, prm.uniform, prm.n, prm.Ne,
+                       prm.G, coef, scale, x, y);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// SI update and vector helpers
+// ---------------------------------------------------------------------------
+constexpr int kUpdBlock = 256;
+int si_update_blocks(long n) { return (int)std::min<long>(2048, (n + kUpdBlock - 1) / kUpdBlock); }
+
+__global__ void __launch_bounds__(kUpdBlock) k_si_update(long long n, const double *__restrict__ b,
+                                                         const double *__restrict__ y, double *__restrict__ x,
+                                                         double *__restrict__ partial)
+{
+    __shared__ double sh[kUpdBlock];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double r = y ? b[i] - (x[i] - y[i]) : b[i] - x[i];
+        x[i] += r;
+        acc = fma(r, r, acc);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kUpdBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+hipError_t launch_si_update(long n, const double *b, const double *y, double *x, double *partial, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_si_update, dim3(si_update_blocks(n)), dim3(kUpdBlock), 0, s, (long long)n, b, y, x, partial);
+    return hipGetLastError();
+}
+
+__global__ void k_axpby(long long n, double a, const double *__restrict__ x, double b, const double *__restrict__ y,
+                        double *__restrict__ z)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = a * x[i] + b * y[i];
+}
+
+hipError_t launch_axpby(long n, double a, const double *x, double b, const double *y, double *z, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (long long)n, a, x, b, y, z);
+    return hipGetLastError();
+}
+
+}  // namespace allium

@@ -1,0 +1,174 @@
+"""Benchmark: angular unknowns swept per second on one MI355X.
+
+Workload (one "step" = one full sweep psi = L^{-1} q through the C ABI, with q
+and psi resident in HBM): unit cube, 64^3 hexes, DG p=1 Gauss-Lobatto (8 dofs),
+level-symmetric S8 (80 directions), per-element lognormal sigma_t (E = 1,
+log-sd 1, seed 0), c ~ U[0.5, 0.99], Q = 1 per steradian, vacuum boundaries,
+FP64.  Also reported: the plain GPU source iteration to a 1e-8 FP64 scalar-flux
+residual on the c = 0.9999, log-sd 1.5 medium, and the CPU baseline (the
+oracle's level-synchronous OpenMP sweep on the host cores).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UNKNOWN_3D_P1 = 28.0   # 8 (q) + 8 (psi) + 8*d*nf/n (upwind traces) at the Mult boundary
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, seed, reps):
+    """The oracle's FP64 sweep of the same workload on the host cores."""
+    import oracle
+    from allium_amd import problems as P
+    import allium_amd as A
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cfg = P.problem_3d_hetero(n=n, seed=seed)
+    om, w = A.level_symmetric(8, 3)
+    t0 = time.perf_counter()
+    pr = oracle.Problem(3, cfg["nel"], cfg["vertices"], 1, om, w, cfg["sigma_t"])
+    q = pr.form_source(Q=1.0)
+    setup = time.perf_counter() - t0
+    psi = np.empty(pr.size)
+    pr.sweep(q, psi, threads=threads)          # warm-up
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        pr.sweep(q, psi, threads=threads)
+        times.append(time.perf_counter() - t)
+    tm = float(np.median(times))
+    try:
+        model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        model = [l.split(":", 1)[1].strip() for l in model.splitlines() if l.startswith("Model name")][0]
+    except Exception:
+        model = "unknown"
+    return {"value": pr.size / tm, "unit": "angular unknowns/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 level-synchronous OpenMP sweep of the same {n}^3 S8 p=1 workload, "
+                      f"median of {reps} sweeps after 1 warm-up (setup {setup:.1f}s excluded); host {model}",
+            "seconds_per_sweep": tm}
+
+
+def si_run(n, seed, maxit):
+    """Plain GPU source iteration to a 1e-8 FP64 residual (c = 0.9999, log-sd 1.5)."""
+    import allium_amd as A
+    from allium_amd import problems as P
+
+    cfg = P.problem_3d_hetero(n=n, seed=seed, log_sd=1.5, c_const=0.9999)
+    m = A.Mesh(cfg["nel"])
+    Linv = A.InverseAdvectionOperator(m, 1, 8, cfg["sigma_t"])
+    Linv.set_launch(-1)
+    q = Linv.form_source(Q=1.0)
+    A.synchronize()
+    t = time.perf_counter()
+    phi, psi, it, norms = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=maxit, want_psi=False)
+    wall = time.perf_counter() - t
+    return {"iterations": it, "final_residual": float(norms[-1]), "wall_s": wall,
+            "converged": bool(norms[-1] < 1e-8)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--launch", type=int, default=-1, help="-1 autotune, 0 levels, 1 graph, 2 persistent")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-si", action="store_true")
+    ap.add_argument("--si-maxit", type=int, default=3000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding (angles/groups over "
+                                   "RCCL) is designed but not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import allium_amd as A
+    from allium_amd import problems as P
+
+    log("device:", A.device_name())
+    cfg = P.problem_3d_hetero(n=args.n, seed=args.seed)
+    t0 = time.perf_counter()
+    m = A.Mesh(cfg["nel"])
+    Linv = A.InverseAdvectionOperator(m, 1, 8, cfg["sigma_t"],
+                                      precision=A.FP32_REFINED if args.precision == "fp32" else A.FP64)
+    log(f"setup {time.perf_counter() - t0:.1f}s: {Linv.size} unknowns, {Linv.nlevels} levels")
+    mode = Linv.set_launch(args.launch)
+    log(f"launch mode {mode}")
+    q = Linv.form_source(Q=1.0)
+    dq = A.DeviceVector.from_host(q)
+    dpsi = A.DeviceVector(q.size)
+    del q
+    for _ in range(args.warmup):
+        Linv.Mult(dq, dpsi)
+    A.synchronize()
+    lib = A._lib.load()
+    import ctypes
+    ms = ctypes.c_double()
+    A.check(lib.allium_event_timer(0, None))
+    for _ in range(args.steps):
+        Linv.Mult(dq, dpsi)
+    A.check(lib.allium_event_timer(1, ctypes.byref(ms)))
+    A.synchronize()
+    t_sweep = ms.value / 1e3 / args.steps
+    unknowns = Linv.size
+    value = unknowns / t_sweep
+    achieved = BYTES_PER_UNKNOWN_3D_P1 * unknowns / t_sweep / 1e9
+    psi_h = dpsi.to_host()
+    if not np.isfinite(psi_h).all():
+        raise SystemExit("non-finite angular flux")
+    log(f"sweep {t_sweep * 1e3:.3f} ms -> {value:.3e} unknowns/s, {achieved:.0f} GB/s algorithmic")
+
+    out = {
+        "metric": "angular unknowns swept/s",
+        "value": value,
+        "unit": "angular unknowns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_sweep * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-lu+f64-refine",
+        "data": "synthetic (seeded lognormal sigma_t, uniform c)",
+        "config": {"workload": f"3D heterogeneous one-group sweep: unit cube {args.n}^3 hex, DG p=1, S8 "
+                               f"(80 dirs), lognormal sigma_t (E=1, log-sd 1), c~U[0.5,0.99], Q=1, vacuum",
+                   "mesh": [args.n] * 3, "fe_order": 1, "sn": 8, "groups": 1, "seed": args.seed,
+                   "launch_mode": mode, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_unknown": BYTES_PER_UNKNOWN_3D_P1},
+    }
+    if not args.no_si:
+        try:
+            out["source_iteration"] = si_run(args.n, args.seed, args.si_maxit)
+            log("SI:", out["source_iteration"])
+        except Exception as exc:  # reported, not hidden
+            out["source_iteration"] = {"error": str(exc)}
+    if not args.no_cpu:
+        cb = cpu_baseline(args.n, args.seed, args.cpu_reps)
+        log(f"cpu baseline {cb['value']:.3e} unknowns/s on {cb['cores']} threads")
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

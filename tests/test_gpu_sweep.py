@@ -1,0 +1,208 @@
+"""GPU parity of the HIP sweep against the CPU oracle, through the C ABI.
+
+Bar: FP64 sweeps within 1e-12 relative of the oracle;
+restated reference tests (psi == 1, L(L^-1 x) = x) on the GPU path; the
+FP32-refined path within its stated bound.  Sizes are those the oracle runs in
+seconds; the full 64^3 workload is covered by size-independent properties.
+"""
+import numpy as np
+import pytest
+
+import allium_amd as A
+import oracle
+from allium_amd import problems as P
+from tests.helpers import ls_fixture, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _pair(nel, p, sn, sigma_t, vertices=None, bc=None):
+    dim = len(nel)
+    v = P.cartesian_vertices(nel) if vertices is None else vertices
+    m = A.Mesh(nel, vertices=v, bc=bc)
+    Linv = A.InverseAdvectionOperator(m, p, sn, sigma_t)
+    om, w = ls_fixture(sn, dim)
+    pr = oracle.Problem(dim, nel, v, p, om, w, sigma_t, bc=bc)
+    return Linv, pr
+
+
+CASES = [
+    # cfg1-style 2D: 32x32 quads, p=1, S8, sigma_t = 1
+    ("2d_cfg1", (32, 32), 1, 8, None, 0.0),
+    ("2d_p2_jitter", (12, 10), 2, 6, None, 0.2),
+    ("2d_p0", (16, 16), 0, 4, None, 0.0),
+    ("3d_hetero", (12, 12, 12), 1, 8, "lognormal", 0.0),
+    ("3d_jitter", (8, 7, 6), 1, 6, "lognormal", 0.2),
+    ("3d_p0", (10, 10, 10), 0, 4, None, 0.0),
+    ("3d_reflect", (6, 6, 6), 1, 4, None, 0.0),
+]
+
+
+PERSISTENT_OFF = ("persistent kernel (launch mode 2) disabled: its first MI355X run did not terminate "
+                  "and the cause is not identified")
+
+
+@pytest.mark.parametrize("name,nel,p,sn,sig,jitter", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("mode", [0, 1, pytest.param(2, marks=pytest.mark.skip(reason=PERSISTENT_OFF))])
+def test_sweep_matches_oracle(name, nel, p, sn, sig, jitter, mode):
+    print(f"case {name} mode {mode}", flush=True)
+    dim = len(nel)
+    ne = int(np.prod(nel))
+    rng = np.random.default_rng(17)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne) if sig == "lognormal" else np.ones((1, ne))
+    v = P.jittered_vertices(nel, jitter, rng) if jitter else None
+    bc = (1, 0, 0, 0, 0, 0) if name == "3d_reflect" else None
+    Linv, pr = _pair(nel, p, sn, sigma, vertices=v, bc=bc)
+    Linv.set_launch(mode)
+    q = pr.form_source(Q=1.0) + rng.uniform(0, 0.1, pr.size)
+    ref = pr.sweep(q)
+    got = Linv.Mult(q, np.empty_like(q))
+    assert rel_err(got, ref) < FP64_TOL
+    # in place (Mult(psi, psi), src/transport_op.hpp:28)
+    buf = q.copy()
+    Linv.Mult(buf, buf)
+    assert rel_err(buf, ref) < FP64_TOL
+
+
+def test_multigroup_time_absorption_matches_oracle():
+    nel, ne, G = (6, 5, 4), 120, 3
+    rng = np.random.default_rng(3)
+    sigma = rng.uniform(0.1, 5.0, (G, ne))
+    Linv, pr = _pair(nel, 1, 6, sigma)
+    Linv.SetTimeAbsorption(7.5)
+    pr.set_time_absorption(7.5)
+    q = pr.form_source(Q=rng.uniform(0.5, 1.5, (G, ne)), inflow=np.ones((G, 6)))
+    for mode in (0, 1):
+        Linv.set_launch(mode)
+        assert rel_err(Linv.Mult(q, np.empty_like(q)), pr.sweep(q)) < FP64_TOL
+    # new cross sections (AssembleLocalMatrices)
+    sigma2 = rng.uniform(0.1, 5.0, (G, ne))
+    Linv.AssembleLocalMatrices(sigma2)
+    pr.set_sigma(sigma2)
+    assert rel_err(Linv.Mult(q, np.empty_like(q)), pr.sweep(q)) < FP64_TOL
+
+
+@pytest.mark.parametrize("nel,p", [((10, 10), 1), ((10, 10), 2), ((6, 6, 6), 1), ((10, 10, 10), 0)])
+def test_constant_solution_gpu(nel, p):
+    # test/test_sweep.cpp:7-108 on the GPU path (two groups, sigma = 0)
+    dim = len(nel)
+    ne = int(np.prod(nel))
+    m = A.Mesh(nel)
+    Linv = A.InverseAdvectionOperator(m, p, 4, np.zeros((2, ne)))
+    q = Linv.form_source(Q=0.0, inflow=np.ones((2, 2 * dim)))
+    psi = Linv.Mult(q, np.empty_like(q))
+    assert np.abs(psi - 1.0).max() < 1e-10
+
+
+@pytest.mark.parametrize("nel", [(3, 3), (3, 3, 3), (9, 8, 7)])
+def test_L_Linv_gpu(nel):
+    # test/test_sweep.cpp:289-361 through the GPU L and L^-1
+    ne = int(np.prod(nel))
+    m = A.Mesh(nel)
+    Linv = A.InverseAdvectionOperator(m, 1, 4, np.vstack([np.ones(ne), 0.5 * np.ones(ne)]))
+    L = A.AdvectionOperator(Linv)
+    x = np.random.default_rng(12345).uniform(-1, 1, Linv.size)
+    y = L.Mult(x, np.empty_like(x))
+    z = Linv.Mult(y, y)
+    assert np.linalg.norm(z - x) < 1e-12 * max(1.0, np.sqrt(x.size / 1000))
+
+
+def test_apply_L_and_balance_match_oracle():
+    nel = (7, 6, 5)
+    ne = 210
+    rng = np.random.default_rng(9)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+    v = P.jittered_vertices(nel, 0.2, rng)
+    for verts in (None, v):
+        Linv, pr = _pair(nel, 1, 6, sigma, vertices=verts)
+        x = rng.normal(size=pr.size)
+        L = A.AdvectionOperator(Linv)
+        assert rel_err(L.Mult(x, np.empty_like(x)), pr.apply_L(x)) < 1e-13
+        q = pr.form_source(Q=1.0)
+        psi = pr.sweep(q)
+        assert abs(L.ComputeBalance(psi) - pr.balance(psi)) < 1e-12 * abs(pr.balance(psi))
+        assert abs(L.ComputeBalance(psi) - 4 * np.pi) < 1e-10
+
+
+def test_device_pointer_api_and_moments():
+    nel = (10, 9, 8)
+    ne = 720
+    rng = np.random.default_rng(21)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+    Linv, pr = _pair(nel, 1, 8, sigma)
+    q = pr.form_source(Q=1.0)
+    dq = A.DeviceVector.from_host(q)
+    dpsi = A.DeviceVector(q.size)
+    Linv.Mult(dq, dpsi)
+    ref = pr.sweep(q)
+    assert rel_err(dpsi.to_host(), ref) < FP64_TOL
+    D = A.DiscreteToMoment(Linv)
+    dphi = A.DeviceVector(Linv.phi_size)
+    D.Mult(dpsi, dphi)
+    assert rel_err(dphi.to_host(), pr.D(ref)) < 1e-14
+    D.MultTranspose(dphi, dq)
+    assert rel_err(dq.to_host(), pr.DT(pr.D(ref))) < 1e-14
+
+
+def test_source_iteration_matches_oracle():
+    # cfg1: 2D 32x32, p=1, S8, sigma_t = 1, c = 0.9, Q = 1, vacuum, SI to 1e-8
+    d = P.problem_2d_cfg1()
+    Linv, pr = _pair(d["nel"], 1, 8, d["sigma_t"])
+    q = pr.form_source(Q=1.0)
+    phi_o, psi_o, it_o, norms_o = pr.source_iteration(d["sigma_s"], q, abstol=1e-8, maxit=400)
+    for mode in (0, 1):
+        Linv.set_launch(mode)
+        phi, psi, it, norms = A.source_iteration(Linv, d["sigma_s"], q, abstol=1e-8, maxit=400)
+        assert it == it_o
+        np.testing.assert_allclose(norms, norms_o, rtol=1e-9)
+        assert rel_err(phi, phi_o) < 1e-11
+        assert rel_err(psi, psi_o) < 1e-11
+
+
+def test_fp32_refined_sweep_error_bound():
+    """FP32 local LU with one FP64 residual refinement per cell: error vs the
+    FP64 oracle sweep on the 3D lognormal medium, printed and bounded."""
+    nel = (12, 12, 12)
+    ne = 1728
+    rng = np.random.default_rng(23)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+    Linv, pr = _pair(nel, 1, 8, sigma)
+    q = pr.form_source(Q=1.0)
+    ref = pr.sweep(q)
+    Linv.set_precision(A.FP32_REFINED)
+    got = Linv.Mult(q, np.empty_like(q))
+    err = rel_err(got, ref)
+    print(f"fp32-refined sweep rel err {err:.3e}")
+    assert err < 1e-5
+
+
+def test_fp32_refined_source_iteration_reaches_fp64_solution():
+    """Defect-correction SI with FP32 sweeps converges to the FP64 SI solution."""
+    nel = (10, 10, 10)
+    ne = 1000
+    d = P.problem_3d_hetero(n=10, seed=4)
+    Linv, pr = _pair(nel, 1, 6, d["sigma_t"])
+    q = pr.form_source(Q=1.0)
+    phi_o, psi_o, it_o, _ = pr.source_iteration(d["sigma_s"], q, abstol=1e-10, maxit=400)
+    phi, psi, it, norms = A.source_iteration(Linv, d["sigma_s"], q, abstol=1e-10, maxit=400,
+                                             precision=A.FP32_REFINED)
+    assert norms[-1] < 1e-10
+    assert rel_err(phi, phi_o) < 1e-9
+    assert abs(it - it_o) <= 2
+
+
+def test_errors_are_reported_not_aborted():
+    m = A.Mesh((4, 4), bc=(1, 1, 0, 0))
+    with pytest.raises(A.AlliumError, match="DAG"):
+        A.InverseAdvectionOperator(m, 1, 4, np.ones((1, 16)))
+    with pytest.raises(A.AlliumError, match="non-finite"):
+        A.InverseAdvectionOperator(A.Mesh((4, 4)), 1, 4, np.full((1, 16), np.nan))
+    with pytest.raises(A.AlliumError):
+        A.InverseAdvectionOperator(A.Mesh((2, 2, 2)), 3, 4, np.ones((1, 8)))
+    Linv = A.InverseAdvectionOperator(A.Mesh((4, 4)), 1, 4, np.ones((1, 16)))
+    with pytest.raises(ValueError):
+        Linv.Mult(np.zeros(3), np.zeros(3))
+    with pytest.raises(A.AlliumError, match="disabled"):
+        Linv.set_launch(2)
