@@ -243,6 +243,15 @@ class InverseAdvectionOperator:
         check(self._lib.allium_sweep_info(self._h, ctypes.byref(info)))
         return info.launch_mode
 
+    def set_tuning(self, mode, block, lanes):
+        """Explicit launch mode, workgroup size and lanes per cell."""
+        check(self._lib.allium_sweep_set_tuning(self._h, mode, block, lanes))
+
+    def tuning(self):
+        m, b, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self._lib.allium_sweep_get_tuning(self._h, ctypes.byref(m), ctypes.byref(b), ctypes.byref(l)))
+        return {"mode": m.value, "block": b.value, "lanes": l.value}
+
     def levels(self):
         out = np.zeros(self.mesh.Ne * self.Nomega, dtype=np.int32)
         check(self._lib.allium_sweep_levels(self._h, out.ctypes.data_as(_lib.c_ip)))

@@ -20,7 +20,7 @@ struct SweepParams {
     const double *omega;    // Nomega x 4 (x, y, z, w)
     // geometry: uniform => one element's matrices; general => per element / face
     int uniform;
-    const double *K;        // uniform: Nomega x n x n, streaming + outflow faces
+    const double *K;        // uniform: Nomega x n x n, streaming + outflow faces + tau * Mt
     const float *Kf;
     const double *Mu;       // unit mass (MG rule): uniform n x n, general Ne x n x n
     const float *Muf;
@@ -42,7 +42,14 @@ struct SweepParams {
     const double *iso;      // phi-sized isotropic source added to every direction (may be null)
     double *psi;
     const uint32_t *cells;  // (e << 10) | a, grouped by level
-    int refine;             // FP32 solves: number of FP64 residual refinements
+};
+
+// One level of the direction-uniform wave plan: offset of the level in the
+// padded cell array and the prefix of 64-cell waves per XCD bin.
+struct WaveLevel {
+    long long base;   // offset of the level in the padded cell array
+    int nparts;       // octant groups in the plan (1)
+    int pb[33];       // prefix of waves over (group, bin)
 };
 
 // Tiles of the persistent kernel: a run of (cell, group) items of one
@@ -58,9 +65,12 @@ struct Tile {
 
 struct DeviceSweep {
     SweepParams prm{};
-    int precision = 0;       // 0 fp64, 1 fp32 + refinement
-    int launch_mode = 0;     // 0 levels, 1 graph, 2 persistent
+    int precision = 0;       // 0 fp64 cell solves, 1 fp32 cell solves
+    int launch_mode = 0;     // 0 levels, 1 graph, 2 persistent, 3 direction sweeps
+    int dir_block = 512;     // workgroup size of the direction-sweep kernel
     int block = 128;
+    int lanes = 4;           // lanes per cell in the level kernel (1, 2 or 4); 0 = direction-uniform
+                             // waves over XCD bins (uniform geometry)
     int persistent_tile = 256;
     // host copies of the plan
     std::vector<long> level_ptr;
@@ -69,6 +79,13 @@ struct DeviceSweep {
     std::vector<void *> owned;
     double *d_omega = nullptr;
     uint32_t *d_cells = nullptr;
+    uint32_t *d_wcells = nullptr;
+    long long *d_cls_ptr = nullptr;  // nlevels x Nomega + 1: start of each (level, direction) class
+    bool angle_independent = true;
+    std::vector<long> wlevel_base;
+    std::vector<int> wpb;
+    int nparts = 1;
+
     int *d_refl = nullptr;
     // persistent kernel state
     Tile *d_tiles = nullptr;

@@ -661,6 +661,56 @@ int build_plan(const Mesh &m, const Quadrature &q, SweepPlan &plan, std::string 
     return 0;
 }
 
+void build_wave_plan(const Mesh &m, const Quadrature &q, SweepPlan &plan, int block, int nparts)
+{
+    const int Ne = m.Ne, nl = plan.nlevels, No = q.size();
+    constexpr int W = 64, NB = 8;
+    const int NP = plan.angle_independent ? nparts : 1;
+    plan.nparts = NP;
+    std::vector<uint8_t> bins(Ne);
+    for (int e = 0; e < Ne; e++) {
+        int c[3];
+        m.elem_coords(e, c);
+        int b = 0;
+        for (int d = 0; d < 3; d++) b |= ((c[d] / block) & 1) << d;
+        bins[e] = (uint8_t)b;
+    }
+    // part of a direction: in 3D its octant pair (octants differing only in
+    // the x sign), in 2D its quadrant
+    std::vector<int> part(No, 0);
+    if (NP > 1)
+        for (int a = 0; a < No; a++) {
+            const int oct = (q.omega[3 * a] < 0 ? 1 : 0) | (q.omega[3 * a + 1] < 0 ? 2 : 0) |
+                            (q.omega[3 * a + 2] < 0 ? 4 : 0);
+            part[a] = (m.dim == 3 ? oct >> 1 : oct) % NP;
+        }
+    const int NPB = NP * NB;
+    auto key_of = [&](int L, int e, int a) {
+        return (((size_t)L * NP + part[a]) * NB + bins[e]) * No + a;
+    };
+    const size_t nkey = (size_t)nl * NPB * No;
+    std::vector<long> cnt(nkey, 0);
+    for (int e = 0; e < Ne; e++)
+        for (int a = 0; a < No; a++) cnt[key_of(plan.level[(size_t)e * No + a], e, a)]++;
+    std::vector<long> off(nkey + 1, 0);
+    for (size_t k = 0; k < nkey; k++) off[k + 1] = off[k] + (cnt[k] + W - 1) / W * W;
+    plan.wcells.assign(off[nkey], SweepPlan::kPad);
+    plan.wlevel_base.assign(nl + 1, 0);
+    plan.wpb.assign((size_t)nl * (NPB + 1), 0);
+    for (int L = 0; L <= nl; L++) plan.wlevel_base[L] = off[(size_t)L * NPB * No];
+    for (int L = 0; L < nl; L++) {
+        int *w = &plan.wpb[(size_t)L * (NPB + 1)];
+        for (int pb = 0; pb < NPB; pb++) {
+            const size_t k0 = ((size_t)L * NPB + pb) * No;
+            w[pb + 1] = w[pb] + (int)((off[k0 + No] - off[k0]) / W);
+        }
+    }
+    std::vector<long> pos(off.begin(), off.end() - 1);
+    for (int a = 0; a < No; a++)
+        for (int e = 0; e < Ne; e++)
+            plan.wcells[pos[key_of(plan.level[(size_t)e * No + a], e, a)]++] = ((uint32_t)e << 10) | (uint32_t)a;
+}
+
 // ---------------------------------------------------------------------------
 // FormTransportSource (src/sweep.cpp:1160-1185): DomainLFIntegrator(Q) plus
 // BoundaryFlowIntegrator(inflow, Omega, -1, -0.5), both of order 2p

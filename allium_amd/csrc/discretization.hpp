@@ -70,7 +70,18 @@ struct SweepPlan {
     std::vector<long> level_ptr;   // nlevels+1
     std::vector<int> refl;         // Nomega x 2dim reflected angle or -1
     bool angle_independent = true; // no reflective coupling between directions
+    // wave plan: per level, part and XCD bin, runs of 64 cells sharing one
+    // direction (tails padded with kPad).  Parts are groups of octants that
+    // never couple (4 when directions are independent, else 1); bins are
+    // spatial 8-colour checkerboard blocks.
+    static constexpr uint32_t kPad = 0xFFFFFFFFu;
+    int nparts = 1;
+    std::vector<uint32_t> wcells;
+    std::vector<long> wlevel_base;   // nlevels+1, offset of each level in wcells
+    std::vector<int> wpb;            // nlevels x (nparts*8+1): prefix of waves per (part, bin)
 };
+// fill the wave plan from level[] (bin = checkerboard colour of blocks of `block` elements)
+void build_wave_plan(const Mesh &m, const Quadrature &q, SweepPlan &plan, int block, int nparts);
 // Returns 0, or -2 (not a DAG) / -3 (missing reflected direction).
 int build_plan(const Mesh &m, const Quadrature &q, SweepPlan &plan, std::string &err);
 

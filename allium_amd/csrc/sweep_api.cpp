@@ -90,14 +90,18 @@ struct allium_sweep {
 
 namespace {
 
-// Uniform geometry: K_a = sum_d Omega_d G_d + sum_{sides with Omega.n > 0} (Omega.n) F_side
+// Uniform geometry: K_a = sum_d Omega_d G_d + sum_{sides with Omega.n > 0}
+// (Omega.n) F_side [+ tau Mt], the same for every element
 void build_streaming(allium_sweep *h)
 {
     const auto &ed = h->ed;
     const int n = ed.n, nf = ed.nf, dim = ed.dim, No = h->quad.size();
+    const double tau = h->ds.prm.timedep ? h->ds.prm.tau : 0.0;
     h->Kst.assign((size_t)No * n * n, 0.0);
     for (int a = 0; a < No; a++) {
         double *K = &h->Kst[(size_t)a * n * n];
+        if (tau != 0.0)
+            for (int i = 0; i < n * n; i++) K[i] += tau * ed.Mm[i];
         for (int d = 0; d < dim; d++) {
             const double od = h->quad.omega[3 * a + d];
             for (int i = 0; i < n * n; i++) K[i] += od * ed.Gd[(size_t)d * n * n + i];
@@ -216,6 +220,28 @@ int upload_all(allium_sweep *h)
         HIP_TRY(upload(ds.owned, &ds.d_refl, h->plan.refl.data(), h->plan.refl.size()));
         prm.refl = ds.d_refl;
     }
+    if (m.uniform) {
+        int nmax = std::max(m.nel[0], std::max(m.nel[1], m.nel[2]));
+        allium::build_wave_plan(m, h->quad, h->plan, std::max(2, (nmax + 7) / 8), 1);
+        HIP_TRY(upload(ds.owned, &ds.d_wcells, h->plan.wcells.data(), h->plan.wcells.size()));
+        ds.wlevel_base = h->plan.wlevel_base;
+        ds.wpb = h->plan.wpb;
+        ds.nparts = 1;
+        std::vector<uint32_t>().swap(h->plan.wcells);
+    }
+    {
+        // start of every (level, direction) class in the cells array
+        const int nl = h->plan.nlevels;
+        std::vector<long long> cls((size_t)nl * No + 1, 0);
+        std::vector<long long> cnt((size_t)nl * No, 0);
+        for (uint32_t c : h->plan.cells) {
+            const int e = (int)(c >> 10), a = (int)(c & 1023u);
+            cnt[(size_t)h->plan.level[(size_t)e * No + a] * No + a]++;
+        }
+        for (size_t k = 0; k < cnt.size(); k++) cls[k + 1] = cls[k] + cnt[k];
+        HIP_TRY(upload(ds.owned, &ds.d_cls_ptr, cls.data(), cls.size()));
+        ds.angle_independent = h->plan.angle_independent;
+    }
     ds.level_ptr = h->plan.level_ptr;
     ds.nlevels = h->plan.nlevels;
     std::vector<allium::Tile> tiles;
@@ -301,11 +327,11 @@ int allium_sweep_create(const allium_mesh_desc *mesh, int fe_order, const allium
     h->psi_size = (long)G * h->quad.size() * Ne * h->ed.n;
     h->phi_size = (long)G * Ne * h->ed.n;
     h->ds.precision = precision == ALLIUM_FP32_REFINED ? 1 : 0;
-    h->ds.prm.refine = 1;
     int rc = upload_all(h.get());
     if (rc) return rc;
     // launch strategy: persistent kernel when the direction classes are independent
     h->ds.launch_mode = 0;
+    h->ds.lanes = h->ds.d_wcells ? 0 : 1;
     *out = h.release();
     return ALLIUM_OK;
 }
@@ -358,6 +384,14 @@ int allium_sweep_set_time_absorption(allium_sweep_t h, double tau)
     if (!std::isfinite(tau)) return fail(ALLIUM_ERR_NUMERIC, "non-finite time absorption");
     h->ds.prm.tau = tau;
     h->ds.prm.timedep = 1;
+    if (h->mesh.uniform) {
+        // the uniform streaming tables carry tau * Mt
+        build_streaming(h);
+        std::vector<float> kf(h->Kst.size());
+        for (size_t i = 0; i < kf.size(); i++) kf[i] = (float)h->Kst[i];
+        HIP_TRY(hipMemcpy((void *)h->ds.prm.K, h->Kst.data(), sizeof(double) * h->Kst.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy((void *)h->ds.prm.Kf, kf.data(), sizeof(float) * kf.size(), hipMemcpyHostToDevice));
+    }
     if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
     return ALLIUM_OK;
 }
@@ -398,7 +432,9 @@ int allium_sweep_mult(allium_sweep_t h, const double *source, double *psi)
 
 int allium_sweep_set_launch(allium_sweep_t h, int mode)
 {
-    if (!h || mode < -1 || mode > 2) return fail(ALLIUM_ERR_ARG, "bad launch mode");
+    if (!h || mode < -1 || mode > 3) return fail(ALLIUM_ERR_ARG, "bad launch mode");
+    if (mode == 3 && !h->ds.angle_independent)
+        return fail(ALLIUM_ERR_UNSUPPORTED, "direction sweeps need directions that do not couple (no reflective side)");
     if (mode == 2)
         return fail(ALLIUM_ERR_UNSUPPORTED,
                     "persistent sweep kernel is disabled: a run on MI355X did not terminate and the cause "
@@ -409,15 +445,26 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
     HIP_TRY(h->b.ensure(bytes));
     HIP_TRY(h->c.ensure(bytes));
     HIP_TRY(hipMemsetAsync(h->b.p, 0, bytes, h->stream));
-    struct Cand { int mode, block; };
-    std::vector<Cand> cands = {{0, 64}, {0, 128}, {0, 256}, {1, 64}, {1, 128}, {1, 256}};
+    struct Cand { int mode, block, lanes; };
+    std::vector<Cand> cands;
+    if (h->ds.angle_independent)
+        for (int bl : {256, 512}) cands.push_back({3, bl, 1});
+    for (int md : {0, 1})
+        for (int bl : {64, 128, 256})
+            for (int ln : {0, 1, 2, 4}) {
+                if (ln == 0 && !h->ds.d_wcells) continue;
+                if (ln > 0 && h->ed.n % ln) continue;
+                cands.push_back({md, bl, ln});
+            }
     double best = 1e30;
-    Cand pick = {0, 128};
+    Cand pick = {0, 128, 1};
     const bool timing = h->ds.timing;
     h->ds.timing = false;
     for (const auto &cd : cands) {
         h->ds.launch_mode = cd.mode;
-        h->ds.block = cd.block;
+        if (cd.mode == 3) h->ds.dir_block = cd.block;
+        else h->ds.block = cd.block;
+        h->ds.lanes = cd.lanes;
         if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
         int rc = do_sweep(h, h->b.d(), nullptr, h->c.d(), h->stream);
         if (rc) { h->ds.timing = timing; return rc; }
@@ -440,10 +487,42 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
     }
     h->ds.timing = timing;
     h->ds.launch_mode = pick.mode;
-    h->ds.block = pick.block;
+    if (pick.mode == 3) h->ds.dir_block = pick.block;
+    else h->ds.block = pick.block;
+    h->ds.lanes = pick.lanes;
     if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
     h->ds.sweeps = 0;
     h->ds.sweep_seconds = 0;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_set_tuning(allium_sweep_t h, int mode, int block, int lanes)
+{
+    // the level kernels are compiled with __launch_bounds__(256), the
+    // direction-sweep kernel (mode 3) with __launch_bounds__(512)
+    const int maxb = mode == 3 ? 512 : 256;
+    if (!h || mode < 0 || mode > 3 || block < 64 || block > maxb || block % 64 ||
+        (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4))
+        return fail(ALLIUM_ERR_ARG, "bad tuning (mode 0..3, block 64..256 step 64 (512 for mode 3), lanes 0/1/2/4)");
+    if (mode == 3 && lanes != 1) return fail(ALLIUM_ERR_ARG, "bad tuning: direction sweeps use one lane per cell");
+    if (lanes == 0 && !h->ds.d_wcells)
+        return fail(ALLIUM_ERR_ARG, "direction-uniform waves need uniform geometry");
+    if (lanes > 0 && h->ed.n % lanes) return fail(ALLIUM_ERR_ARG, "lanes per cell must divide the dofs per element");
+    int rc = allium_sweep_set_launch(h, mode);
+    if (rc) return rc;
+    if (mode == 3) h->ds.dir_block = block;
+    else h->ds.block = block;
+    h->ds.lanes = lanes;
+    if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
+    return ALLIUM_OK;
+}
+
+int allium_sweep_get_tuning(allium_sweep_t h, int *mode, int *block, int *lanes)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    if (mode) *mode = h->ds.launch_mode;
+    if (block) *block = h->ds.launch_mode == 3 ? h->ds.dir_block : h->ds.block;
+    if (lanes) *lanes = h->ds.lanes;
     return ALLIUM_OK;
 }
 

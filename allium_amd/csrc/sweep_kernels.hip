@@ -16,6 +16,10 @@
 namespace allium {
 
 typedef double d2v __attribute__((ext_vector_type(2)));
+// read-only tables seen through the constant address space: uniform
+// addresses become scalar loads, non-uniform ones stay vector loads
+typedef const __attribute__((address_space(4))) double *cdptr;
+__device__ __forceinline__ cdptr as_const(const double *p) { return (cdptr)p; }
 
 template <int DIM, int P>
 struct Sh {
@@ -48,12 +52,27 @@ __device__ __forceinline__ int fast_div(int a, int d, float inv)
     return q;
 }
 
+__device__ __forceinline__ double fast_rcp(double a)
+{
+    double r = __builtin_amdgcn_rcp(a);
+    double e = fma(-a, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-a, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ float fast_rcp(float a)
+{
+    const float r = __builtin_amdgcn_rcpf(a);
+    return fmaf(r, fmaf(-a, r, 1.0f), r);
+}
+
 template <int N, typename T>
 __device__ __forceinline__ void lu_factor_solve(T (&A)[N][N], T (&x)[N])
 {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        const T inv = T(1) / A[k][k];
+        const T inv = fast_rcp(A[k][k]);
         A[k][k] = inv;
 #pragma unroll
         for (int i = k + 1; i < N; ++i) {
@@ -89,19 +108,13 @@ __device__ __forceinline__ void lu_resolve(const T (&A)[N][N], T (&x)[N])
     }
 }
 
-// Entry (i,j) of the FP64 local matrix, recomputed from the stored tables
-// (used by the FP64 residual of the refinement step, uniform geometry only).
-template <int DIM, int P>
-__device__ __forceinline__ double uniform_entry(const SweepParams &p, const double *K, double sig,
-                                                int i, int j)
-{
-    constexpr int N = Sh<DIM, P>::N;
-    double v = fma(sig, p.Mu[i * N + j], K[i * N + j]);
-    if (p.timedep) v = fma(p.tau, p.Mt[i * N + j], v);
-    return v;
-}
-
-template <int DIM, int P, bool UNI, typename T>
+// One cell (e, a, g).  Every global load (source, isotropic source, upwind
+// traces, general-geometry matrices) is issued before any of them is used and
+// without data-dependent branches, so a wave makes one memory round trip
+// before its register LU.  OCT >= 0 fixes the direction's octant at compile
+// time (uniform geometry, direction-uniform waves): only the dim inflow sides
+// are gathered.  OCT < 0 decides inflow per lane with selects.
+template <int DIM, int P, bool UNI, typename T, int OCT>
 __device__ __forceinline__ void cell_solve(const SweepParams &p, const int e, const int a, const int g)
 {
     using S = Sh<DIM, P>;
@@ -115,70 +128,14 @@ __device__ __forceinline__ void cell_solve(const SweepParams &p, const int e, co
         c[1] = q0;
         c[2] = q01;
     }
-    const double om[3] = {p.omega[4 * a + 0], p.omega[4 * a + 1], p.omega[4 * a + 2]};
+    const cdptr omg = as_const(p.omega);
+    const double om[3] = {omg[4 * a + 0], omg[4 * a + 1], omg[4 * a + 2]};
     const long long base = ((long long)g * p.Nomega + a) * p.Ndof;
     const long long off = base + (long long)e * N;
+    const double *self = p.psi + off;
+
+    // ---- phase 1: issue loads -------------------------------------------
     const double sig = p.sigma[(long long)g * p.Ne + e];
-
-    T A[N][N];
-    const double *K = UNI ? p.K + (size_t)a * N * N : nullptr;
-    if constexpr (UNI) {
-        if constexpr (sizeof(T) == 8) {
-#pragma unroll
-            for (int i = 0; i < N; ++i)
-#pragma unroll
-                for (int j = 0; j < N; ++j) A[i][j] = uniform_entry<DIM, P>(p, K, sig, i, j);
-        } else {
-            const float *Kf = p.Kf + (size_t)a * N * N;
-            const float fs = (float)sig, ft = (float)p.tau;
-#pragma unroll
-            for (int i = 0; i < N; ++i)
-#pragma unroll
-                for (int j = 0; j < N; ++j) {
-                    float v = fmaf(fs, p.Muf[i * N + j], Kf[i * N + j]);
-                    if (p.timedep) v = fmaf(ft, p.Mtf[i * N + j], v);
-                    A[i][j] = v;
-                }
-        }
-    } else {
-        const double *Gd = p.Gd + (size_t)e * DIM * N * N;
-        const double *Mu = p.Mu + (size_t)e * N * N;
-        const double *Mt = p.Mt + (size_t)e * N * N;
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-            for (int j = 0; j < N; ++j) {
-                double v = om[0] * Gd[i * N + j];
-#pragma unroll
-                for (int d = 1; d < DIM; ++d) v = fma(om[d], Gd[(d * N + i) * N + j], v);
-                v = fma(sig, Mu[i * N + j], v);
-                if (p.timedep) v = fma(p.tau, Mt[i * N + j], v);
-                A[i][j] = (T)v;
-            }
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int ax = s >> 1, hi = s & 1;
-            int cf[3] = {c[0], c[1], c[2]};
-            cf[ax] += hi;
-            const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
-            const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
-            const double sgn = (!hi && interior) ? -1.0 : 1.0;
-            double dot = 0.0;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
-            dot *= sgn;
-            if (dot > 0) {
-                const double *F = p.Fface + (size_t)f * NF * NF;
-#pragma unroll
-                for (int k = 0; k < NF; ++k)
-#pragma unroll
-                    for (int l = 0; l < NF; ++l)
-                        A[side_node<DIM, P>(s, k)][side_node<DIM, P>(s, l)] += (T)(dot * F[k * NF + l]);
-            }
-        }
-    }
-
-    // right-hand side: source, isotropic source, upwind face traces
     double b[N];
     if (p.src) {
         if constexpr (N % 2 == 0) {
@@ -193,10 +150,294 @@ __device__ __forceinline__ void cell_solve(const SweepParams &p, const int e, co
 #pragma unroll
         for (int i = 0; i < N; ++i) b[i] = 0.0;
     }
+    double isv[N];
     if (p.iso) {
         const double *is = p.iso + (long long)g * p.Ndof + (long long)e * N;
 #pragma unroll
-        for (int i = 0; i < N; ++i) b[i] += is[i];
+        for (int i = 0; i < N; ++i) isv[i] = is[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) isv[i] = 0.0;
+    }
+    // upwind traces: u[s][m] and weight -dot (0 when side s is not an inflow side)
+    double u[NS][NF], wgt[NS], dots[NS];
+    int fid[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int ax = s >> 1, hi = s & 1;
+        const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
+        double dot;
+        if constexpr (UNI) {
+            dot = hi ? om[ax] : -om[ax];
+            fid[s] = 0;
+        } else {
+            int cf[3] = {c[0], c[1], c[2]};
+            cf[ax] += hi;
+            const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
+            fid[s] = f;
+            dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
+            if (!hi && interior) dot = -dot;
+        }
+        dots[s] = dot;
+        // compile-time octant: inflow sides are the low side of each positive axis
+        bool inflow_known = true, inflow = false;
+        if constexpr (OCT >= 0) inflow = ((OCT >> ax) & 1) ? (hi == 1) : (hi == 0);
+        else inflow_known = false;
+        if (inflow_known && !inflow) { wgt[s] = 0.0; continue; }
+        const bool in = inflow_known ? true : (dot < 0);
+        const bool refl = !interior && ((p.refl_mask >> s) & 1);
+        const bool use_nb = in && interior, use_rf = in && refl;
+        wgt[s] = (use_nb || use_rf) ? -dot : 0.0;
+        const double *nbp = p.psi + base + (long long)(hi ? e + p.stride[ax] : e - p.stride[ax]) * N;
+        const double *rfp = self;
+        if (refl) rfp = p.psi + ((long long)g * p.Nomega + p.refl[a * NS + s]) * p.Ndof + (long long)e * N;
+#pragma unroll
+        for (int m = 0; m < NF; ++m) {
+            const double *addr = use_nb ? nbp + side_node<DIM, P>(s ^ 1, m) : rfp + side_node<DIM, P>(s, m);
+            u[s][m] = *addr;
+        }
+    }
+
+    // ---- phase 2: right-hand side (the traces die here) ----------------------------------------
+#pragma unroll
+    for (int i = 0; i < N; ++i) b[i] += isv[i];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if constexpr (OCT >= 0) {
+            const int ax = s >> 1, hi = s & 1;
+            const bool inflow = ((OCT >> ax) & 1) ? (hi == 1) : (hi == 0);
+            if (!inflow) continue;
+        }
+        const cdptr F = UNI ? as_const(p.Fface + s * NF * NF) : as_const(p.Fface + (size_t)fid[s] * NF * NF);
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            double acc = 0.0;
+#pragma unroll
+            for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], u[s][l], acc);
+            // select, not multiply by zero: a side that is not an inflow side read
+            // an arbitrary (possibly non-finite) value
+            const double nb = fma(wgt[s], acc, b[side_node<DIM, P>(s, k)]);
+            b[side_node<DIM, P>(s, k)] = wgt[s] != 0.0 ? nb : b[side_node<DIM, P>(s, k)];
+        }
+    }
+
+    // ---- phase 3: local matrix -------------------------------------------
+    T A[N][N];
+    if constexpr (UNI) {
+        const cdptr K = as_const(p.K + (size_t)a * N * N);   // includes tau * Mt
+        const cdptr Mu = as_const(p.Mu);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) A[i][j] = (T)fma(sig, Mu[i * N + j], K[i * N + j]);
+    } else {
+        const double *Gd = p.Gd + (size_t)e * DIM * N * N;
+        const double *Mu = p.Mu + (size_t)e * N * N;
+        const double *Mt = p.Mt + (size_t)e * N * N;
+        const double tau = p.timedep ? p.tau : 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                double v = om[0] * Gd[i * N + j];
+#pragma unroll
+                for (int d = 1; d < DIM; ++d) v = fma(om[d], Gd[(d * N + i) * N + j], v);
+                v = fma(sig, Mu[i * N + j], v);
+                v = fma(tau, Mt[i * N + j], v);
+                A[i][j] = (T)v;
+            }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const double w = dots[s] > 0 ? dots[s] : 0.0;   // outflow faces
+            const double *F = p.Fface + (size_t)fid[s] * NF * NF;
+#pragma unroll
+            for (int k = 0; k < NF; ++k)
+#pragma unroll
+                for (int l = 0; l < NF; ++l)
+                    A[side_node<DIM, P>(s, k)][side_node<DIM, P>(s, l)] += (T)(w * F[k * NF + l]);
+        }
+    }
+
+    // ---- phase 4: solve and store ----------------------------------------
+    T x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = (T)b[i];
+    lu_factor_solve<N, T>(A, x);
+    if constexpr (N % 2 == 0) {
+        d2v *o2 = reinterpret_cast<d2v *>(p.psi + off);
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) { d2v v; v.x = (double)x[2 * i]; v.y = (double)x[2 * i + 1]; o2[i] = v; }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) p.psi[off + i] = (double)x[i];
+    }
+}
+
+// octant sign bits of direction a (bit d set when Omega_d < 0)
+__device__ __forceinline__ int octant_of(const SweepParams &p, int a)
+{
+    return (p.omega[4 * a] < 0 ? 1 : 0) | (p.omega[4 * a + 1] < 0 ? 2 : 0) | (p.omega[4 * a + 2] < 0 ? 4 : 0);
+}
+
+// dispatch a direction-uniform cell solve on the (wave-uniform) octant
+template <int DIM, int P, typename T>
+__device__ __forceinline__ void cell_solve_octant(const SweepParams &p, int e, int a, int g, int oct)
+{
+    switch (oct) {
+    case 0: cell_solve<DIM, P, true, T, 0>(p, e, a, g); break;
+    case 1: cell_solve<DIM, P, true, T, 1>(p, e, a, g); break;
+    case 2: cell_solve<DIM, P, true, T, 2>(p, e, a, g); break;
+    case 3: cell_solve<DIM, P, true, T, 3>(p, e, a, g); break;
+    case 4: if constexpr (DIM == 3) cell_solve<DIM, P, true, T, 4>(p, e, a, g); break;
+    case 5: if constexpr (DIM == 3) cell_solve<DIM, P, true, T, 5>(p, e, a, g); break;
+    case 6: if constexpr (DIM == 3) cell_solve<DIM, P, true, T, 6>(p, e, a, g); break;
+    default: if constexpr (DIM == 3) cell_solve<DIM, P, true, T, 7>(p, e, a, g); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-lane cell solve: LPC lanes (a pair or a quad) share one cell; lane l
+// owns rows [l*RPL, (l+1)*RPL) of the local system.  Loads and stores of q and
+// psi are 8*RPL contiguous bytes per lane, so one instruction of a wave covers
+// 64/LPC whole cells; pivot rows and the back-substituted unknowns move between
+// the lanes of a group with DPP quad_perm broadcasts (no LDS traffic).
+// ---------------------------------------------------------------------------
+template <int LPC, int SRC>
+__device__ __forceinline__ constexpr int bcast_ctrl()
+{
+    // quad_perm(s0,s1,s2,s3) = s0 | s1<<2 | s2<<4 | s3<<6
+    return LPC == 4 ? SRC * 0x55 : (SRC | (SRC << 2) | ((2 + SRC) << 4) | ((2 + SRC) << 6));
+}
+
+template <int LPC, int SRC>
+__device__ __forceinline__ double group_bcast(double v)
+{
+    if constexpr (LPC == 1) {
+        return v;
+    } else {
+        constexpr int ctrl = bcast_ctrl<LPC, SRC>();
+        const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xF, 0xF, false);
+        return __hiloint2double(hi, lo);
+    }
+}
+
+template <int LPC, int SRC>
+__device__ __forceinline__ float group_bcast(float v)
+{
+    if constexpr (LPC == 1) {
+        return v;
+    } else {
+        constexpr int ctrl = bcast_ctrl<LPC, SRC>();
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false));
+    }
+}
+
+// is local node r on side s, and its face-lexicographic index there
+template <int DIM, int P>
+__device__ __forceinline__ bool on_side(int r, int s, int &k)
+{
+    constexpr int Q = P + 1;
+    const int i0 = r % Q, i1 = (r / Q) % Q, i2 = r / (Q * Q);
+    const int ax = s >> 1, want = (s & 1) ? P : 0;
+    const int ia = ax == 0 ? i0 : (ax == 1 ? i1 : i2);
+    if (DIM == 2) k = ax == 0 ? i1 : i0;
+    else k = ax == 0 ? i1 + Q * i2 : (ax == 1 ? i0 + Q * i2 : i0 + Q * i1);
+    return ia == want;
+}
+
+template <int DIM, int P, bool UNI, typename T, int LPC>
+__device__ __forceinline__ void group_cell_solve(const SweepParams &p, const int e, const int a, const int g,
+                                                 const int l)
+{
+    using S = Sh<DIM, P>;
+    constexpr int N = S::N, NF = S::NF, NS = S::NS, RPL = N / LPC;
+    static_assert(N % LPC == 0, "lanes per cell must divide the dofs per cell");
+    int c[3];
+    {
+        const int q01 = (DIM == 3) ? fast_div(e, p.stride[2], p.inv_nel01) : 0;
+        const int rem = e - q01 * p.stride[2];
+        const int q0 = fast_div(rem, p.nel[0], p.inv_nel0);
+        c[0] = rem - q0 * p.nel[0];
+        c[1] = q0;
+        c[2] = q01;
+    }
+    const double om[3] = {p.omega[4 * a + 0], p.omega[4 * a + 1], p.omega[4 * a + 2]};
+    const long long base = ((long long)g * p.Nomega + a) * p.Ndof;
+    const long long off = base + (long long)e * N;
+    const double sig = p.sigma[(long long)g * p.Ne + e];
+    const int r0 = l * RPL;
+
+    T A[RPL][N];
+    if constexpr (UNI) {
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+            const double *Kr = p.K + ((size_t)a * N + r0 + j) * N;   // includes tau * Mt
+            const double *Mr = p.Mu + (size_t)(r0 + j) * N;
+#pragma unroll
+            for (int cc = 0; cc < N; ++cc) A[j][cc] = (T)fma(sig, Mr[cc], Kr[cc]);
+        }
+    } else {
+        const double *Gd = p.Gd + (size_t)e * DIM * N * N;
+        const double *Mu = p.Mu + (size_t)e * N * N;
+        const double *Mt = p.Mt + (size_t)e * N * N;
+#pragma unroll
+        for (int j = 0; j < RPL; ++j)
+#pragma unroll
+            for (int cc = 0; cc < N; ++cc) {
+                const int ij = (r0 + j) * N + cc;
+                double v = om[0] * Gd[ij];
+#pragma unroll
+                for (int d = 1; d < DIM; ++d) v = fma(om[d], Gd[d * N * N + ij], v);
+                v = fma(sig, Mu[ij], v);
+                if (p.timedep) v = fma(p.tau, Mt[ij], v);
+                A[j][cc] = (T)v;
+            }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int ax = s >> 1, hi = s & 1;
+            int cf[3] = {c[0], c[1], c[2]};
+            cf[ax] += hi;
+            const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
+            const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
+            double dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
+            if (!hi && interior) dot = -dot;
+            if (dot > 0) {
+                const double *F = p.Fface + (size_t)f * NF * NF;
+#pragma unroll
+                for (int j = 0; j < RPL; ++j) {
+                    int k;
+                    if (on_side<DIM, P>(r0 + j, s, k)) {
+#pragma unroll
+                        for (int m = 0; m < NF; ++m)
+                            A[j][side_node<DIM, P>(s, m)] += (T)(dot * F[k * NF + m]);
+                    }
+                }
+            }
+        }
+    }
+
+    double b[RPL];
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) b[j] = 0.0;
+    if (p.src) {
+        if constexpr (RPL % 2 == 0) {
+            const d2v *q2 = reinterpret_cast<const d2v *>(p.src + off + r0);
+#pragma unroll
+            for (int j = 0; j < RPL / 2; ++j) { const d2v v = q2[j]; b[2 * j] = v.x; b[2 * j + 1] = v.y; }
+        } else {
+#pragma unroll
+            for (int j = 0; j < RPL; ++j) b[j] = p.src[off + r0 + j];
+        }
+    }
+    if (p.iso) {
+        const double *is = p.iso + (long long)g * p.Ndof + (long long)e * N + r0;
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) b[j] += is[j];
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -211,75 +452,157 @@ __device__ __forceinline__ void cell_solve(const SweepParams &p, const int e, co
             int cf[3] = {c[0], c[1], c[2]};
             cf[ax] += hi;
             const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
-            const double sgn = (!hi && interior) ? -1.0 : 1.0;
             dot = 0.0;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
-            dot *= sgn;
+            if (!hi && interior) dot = -dot;
             F = p.Fface + (size_t)f * NF * NF;
         }
         if (dot < 0) {
             const double *up = nullptr;
-            int us = s ^ 1;
+            bool same = false;
             if (interior) {
                 const int nb = hi ? e + p.stride[ax] : e - p.stride[ax];
                 up = p.psi + base + (long long)nb * N;
             } else if ((p.refl_mask >> s) & 1) {
                 const int ra = p.refl[a * NS + s];
                 up = p.psi + ((long long)g * p.Nomega + ra) * p.Ndof + (long long)e * N;
-                us = s;
+                same = true;
             }
             if (up) {
                 double u[NF];
-                if (us == (s ^ 1)) {
+                if (same) {
 #pragma unroll
-                    for (int l = 0; l < NF; ++l) u[l] = up[side_node<DIM, P>(s ^ 1, l)];
+                    for (int m = 0; m < NF; ++m) u[m] = up[side_node<DIM, P>(s, m)];
                 } else {
 #pragma unroll
-                    for (int l = 0; l < NF; ++l) u[l] = up[side_node<DIM, P>(s, l)];
+                    for (int m = 0; m < NF; ++m) u[m] = up[side_node<DIM, P>(s ^ 1, m)];
                 }
 #pragma unroll
-                for (int k = 0; k < NF; ++k) {
-                    double acc = 0.0;
+                for (int j = 0; j < RPL; ++j) {
+                    int k;
+                    if (on_side<DIM, P>(r0 + j, s, k)) {
+                        double acc = 0.0;
 #pragma unroll
-                    for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], u[l], acc);
-                    b[side_node<DIM, P>(s, k)] = fma(-dot, acc, b[side_node<DIM, P>(s, k)]);
+                        for (int m = 0; m < NF; ++m) acc = fma(F[k * NF + m], u[m], acc);
+                        b[j] = fma(-dot, acc, b[j]);
+                    }
                 }
             }
         }
     }
 
-    T x[N];
+    // LU without pivoting across the lanes of the group
+    T x[RPL];
 #pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = (T)b[i];
-    lu_factor_solve<N, T>(A, x);
-    double xs[N];
+    for (int j = 0; j < RPL; ++j) x[j] = (T)b[j];
+    T inv[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) xs[i] = (double)x[i];
-    if constexpr (sizeof(T) == 4 && UNI) {
-        // FP64 residual r = b - A x with A rebuilt in FP64, correction by the
-        // FP32 factors
-        for (int it = 0; it < p.refine; ++it) {
-            T r[N];
+    for (int k = 0; k < N; ++k) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int kl = k / RPL, kj = k % RPL;
+        T piv[N];
+        T pb;
+        // the owner lane kl broadcasts row kj of its block (compile-time lane)
+#define ALLIUM_BCAST_ROW(LANE)                                                        \
+    if (kl == LANE) {                                                                 \
+        _Pragma("unroll") for (int cc = 0; cc < N; ++cc) if (cc >= k) piv[cc] =        \
+            group_bcast<LPC, LANE>(A[kj][cc]);                                        \
+        pb = group_bcast<LPC, LANE>(x[kj]);                                           \
+    }
+        ALLIUM_BCAST_ROW(0)
+        if constexpr (LPC > 1) { ALLIUM_BCAST_ROW(1) }
+        if constexpr (LPC > 2) { ALLIUM_BCAST_ROW(2) ALLIUM_BCAST_ROW(3) }
+#undef ALLIUM_BCAST_ROW
+        inv[k] = fast_rcp(piv[k]);
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                double acc = b[i];
+        for (int j = 0; j < RPL; ++j) {
+            const T lm = (r0 + j > k) ? A[j][k] * inv[k] : T(0);
 #pragma unroll
-                for (int j = 0; j < N; ++j) acc = fma(-uniform_entry<DIM, P>(p, K, sig, i, j), xs[j], acc);
-                r[i] = (T)acc;
-            }
-            lu_resolve<N, T>(A, r);
-#pragma unroll
-            for (int i = 0; i < N; ++i) xs[i] += (double)r[i];
+            for (int cc = 0; cc < N; ++cc)
+                if (cc > k) A[j][cc] = fma(-lm, piv[cc], A[j][cc]);
+            x[j] = fma(-lm, pb, x[j]);
         }
     }
-    if constexpr (N % 2 == 0) {
-        d2v *o2 = reinterpret_cast<d2v *>(p.psi + off);
 #pragma unroll
-        for (int i = 0; i < N / 2; ++i) { d2v v; v.x = xs[2 * i]; v.y = xs[2 * i + 1]; o2[i] = v; }
+    for (int k = N - 1; k >= 0; --k) {
+        const int kl = k / RPL, kj = k % RPL;
+        T xk = T(0);
+#define ALLIUM_BCAST_X(LANE) if (kl == LANE) xk = group_bcast<LPC, LANE>(x[kj]);
+        ALLIUM_BCAST_X(0)
+        if constexpr (LPC > 1) { ALLIUM_BCAST_X(1) }
+        if constexpr (LPC > 2) { ALLIUM_BCAST_X(2) ALLIUM_BCAST_X(3) }
+#undef ALLIUM_BCAST_X
+        xk *= inv[k];
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+            if (r0 + j < k) x[j] = fma(-A[j][k], xk, x[j]);
+            else if (r0 + j == k) x[j] = xk;
+        }
+    }
+    if constexpr (RPL % 2 == 0) {
+        d2v *o2 = reinterpret_cast<d2v *>(p.psi + off + r0);
+#pragma unroll
+        for (int j = 0; j < RPL / 2; ++j) { d2v v; v.x = (double)x[2 * j]; v.y = (double)x[2 * j + 1]; o2[j] = v; }
     } else {
 #pragma unroll
-        for (int i = 0; i < N; ++i) p.psi[off + i] = xs[i];
+        for (int j = 0; j < RPL; ++j) p.psi[off + r0 + j] = (double)x[j];
+    }
+}
+
+template <int DIM, int P, bool UNI, typename T, int LPC>
+__global__ void __launch_bounds__(256) k_sweep_level_group(SweepParams p, long long cbeg, int ncells)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int item = t / LPC, l = t % LPC;
+    if (item >= ncells * p.G) return;
+    int g = 0, c = item;
+    if (p.G > 1) { g = item / ncells; c = item - g * ncells; }
+    const uint32_t cell = p.cells[cbeg + c];
+    group_cell_solve<DIM, P, UNI, T, LPC>(p, (int)(cell >> 10), (int)(cell & 1023u), g, l);
+}
+
+// Direction-uniform waves over XCD bins: every wave holds 64 cells of one
+// (level, direction) class, so the direction index is wave-uniform and the
+// direction's streaming matrix K[a] is read with scalar loads; workgroup
+// blockIdx.x % 8 serves spatial bin blockIdx.x % 8, so on MI355X (workgroups
+// dispatched round-robin over the 8 XCDs) the upwind traces a cell reads were
+// written through the same XCD's L2.
+template <int DIM, int P, typename T>
+__global__ void __launch_bounds__(256) k_sweep_wave(SweepParams p, const uint32_t *__restrict__ wcells, WaveLevel wl)
+{
+    const int bin = blockIdx.x & 7;
+    const int rest = blockIdx.x >> 3;
+    const int part = rest % wl.nparts, chunk = rest / wl.nparts;
+    const int pb = part * 8 + bin;
+    const int w = chunk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    if (w >= wl.pb[pb + 1] - wl.pb[pb]) return;
+    const long long idx = wl.base + (long long)(wl.pb[pb] + w) * 64 + (threadIdx.x & 63);
+    const uint32_t cell = wcells[idx];
+    const int a = __builtin_amdgcn_readfirstlane((int)(cell & 1023u));
+    if (cell == 0xFFFFFFFFu) return;
+    cell_solve_octant<DIM, P, T>(p, (int)(cell >> 10), a, (int)blockIdx.y, octant_of(p, a));
+}
+
+// Direction sweeps: workgroup (a, g) sweeps direction a of group g through
+// all of its levels, separated by workgroup barriers.  Directions never
+// couple without reflective sides, so workgroups share no data: every upwind
+// trace is written and read through the same CU, the direction is uniform for
+// the whole workgroup (its streaming matrix comes through scalar loads) and a
+// sweep is one launch.
+template <int DIM, int P, bool UNI, typename T>
+__global__ void __launch_bounds__(512) k_sweep_direction(SweepParams p, const long long *__restrict__ cls_ptr,
+                                                         int nlevels)
+{
+    const int a = blockIdx.x, g = blockIdx.y, No = p.Nomega;
+    for (int L = 0; L < nlevels; ++L) {
+        const long long b = cls_ptr[(long long)L * No + a], e = cls_ptr[(long long)L * No + a + 1];
+        for (long long i = b + threadIdx.x; i < e; i += blockDim.x) {
+            if constexpr (UNI) cell_solve_octant<DIM, P, T>(p, (int)(p.cells[i] >> 10), a, g, octant_of(p, a));
+            else cell_solve<DIM, P, false, T, -1>(p, (int)(p.cells[i] >> 10), a, g);
+        }
+        __syncthreads();
     }
 }
 
@@ -291,7 +614,7 @@ __global__ void __launch_bounds__(256) k_sweep_level(SweepParams p, long long cb
     int g = 0, c = item;
     if (p.G > 1) { g = item / ncells; c = item - g * ncells; }
     const uint32_t cell = p.cells[cbeg + c];
-    cell_solve<DIM, P, UNI, T>(p, (int)(cell >> 10), (int)(cell & 1023u), g);
+    cell_solve<DIM, P, UNI, T, -1>(p, (int)(cell >> 10), (int)(cell & 1023u), g);
 }
 
 // Persistent sweep.  Tiles are handed out in level order by an atomic ticket,
@@ -335,7 +658,7 @@ __global__ void __launch_bounds__(256) k_sweep_persistent(SweepParams p, const T
             int g = 0, c = item;
             if (p.G > 1) { g = item / tl.ncells; c = item - g * tl.ncells; }
             const uint32_t cell = p.cells[tl.cell_begin + c];
-            cell_solve<DIM, P, UNI, T>(p, (int)(cell >> 10), (int)(cell & 1023u), g);
+            cell_solve<DIM, P, UNI, T, -1>(p, (int)(cell >> 10), (int)(cell & 1023u), g);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
@@ -360,16 +683,58 @@ bool sweep_supported(int dim, int p, std::string &why)
     return false;
 }
 
+template <int DIM, int P, bool UNI, typename T, int LPC>
+static void launch_level(const DeviceSweep &ds, const SweepParams &prm, hipStream_t stream, long long cb, int nc)
+{
+    const long threads = (long)nc * prm.G * LPC;
+    const int grid = (int)((threads + ds.block - 1) / ds.block);
+    if constexpr (LPC == 1)
+        hipLaunchKernelGGL((k_sweep_level<DIM, P, UNI, T>), dim3(grid), dim3(ds.block), 0, stream, prm, cb, nc);
+    else
+        hipLaunchKernelGGL((k_sweep_level_group<DIM, P, UNI, T, LPC>), dim3(grid), dim3(ds.block), 0, stream, prm,
+                           cb, nc);
+}
+
+template <int DIM, int P>
+constexpr bool lanes_ok(int lpc)
+{
+    return lpc == 1 || (Sh<DIM, P>::N % lpc == 0 && Sh<DIM, P>::N / lpc >= 1);
+}
+
 template <int DIM, int P, bool UNI, typename T>
 static hipError_t run_levels(DeviceSweep &ds, const SweepParams &prm, hipStream_t stream)
 {
+    if constexpr (UNI) {
+        if (ds.lanes == 0 && ds.d_wcells) {
+            const int wpb = ds.block / 64;
+            for (int L = 0; L < ds.nlevels; ++L) {
+                const int *pb = &ds.wpb[(size_t)L * 9];
+                WaveLevel wl;
+                wl.nparts = 1;
+                wl.base = ds.wlevel_base[L];
+                int maxw = 0;
+                for (int k = 0; k <= 8; ++k) wl.pb[k] = pb[k];
+                for (int k = 0; k < 8; ++k) maxw = std::max(maxw, wl.pb[k + 1] - wl.pb[k]);
+                if (maxw == 0) continue;
+                const int chunks = (maxw + wpb - 1) / wpb;
+                hipLaunchKernelGGL((k_sweep_wave<DIM, P, T>), dim3(8 * chunks, prm.G), dim3(ds.block), 0, stream, prm,
+                                   ds.d_wcells, wl);
+            }
+            return hipGetLastError();
+        }
+    }
+    const int lpc = (ds.lanes == 4 && lanes_ok<DIM, P>(4)) ? 4 : (ds.lanes == 2 && lanes_ok<DIM, P>(2)) ? 2 : 1;
     for (int L = 0; L < ds.nlevels; ++L) {
         const long long cb = ds.level_ptr[L];
         const int nc = (int)(ds.level_ptr[L + 1] - cb);
-        const long items = (long)nc * prm.G;
-        if (items == 0) continue;
-        const int grid = (int)((items + ds.block - 1) / ds.block);
-        hipLaunchKernelGGL((k_sweep_level<DIM, P, UNI, T>), dim3(grid), dim3(ds.block), 0, stream, prm, cb, nc);
+        if ((long)nc * prm.G == 0) continue;
+        if constexpr (Sh<DIM, P>::N % 4 == 0) {
+            if (lpc == 4) { launch_level<DIM, P, UNI, T, 4>(ds, prm, stream, cb, nc); continue; }
+        }
+        if constexpr (Sh<DIM, P>::N % 2 == 0) {
+            if (lpc == 2) { launch_level<DIM, P, UNI, T, 2>(ds, prm, stream, cb, nc); continue; }
+        }
+        launch_level<DIM, P, UNI, T, 1>(ds, prm, stream, cb, nc);
     }
     return hipGetLastError();
 }
@@ -389,8 +754,18 @@ static hipError_t run_persistent(DeviceSweep &ds, const SweepParams &prm, hipStr
 }
 
 template <int DIM, int P, bool UNI, typename T>
+static hipError_t run_directions(DeviceSweep &ds, const SweepParams &prm, hipStream_t stream)
+{
+    const int block = std::max(64, std::min(512, ds.dir_block));
+    hipLaunchKernelGGL((k_sweep_direction<DIM, P, UNI, T>), dim3(prm.Nomega, prm.G), dim3(block), 0, stream, prm,
+                       ds.d_cls_ptr, ds.nlevels);
+    return hipGetLastError();
+}
+
+template <int DIM, int P, bool UNI, typename T>
 static hipError_t run_mode(DeviceSweep &ds, const SweepParams &prm, hipStream_t stream, int mode)
 {
+    if (mode == 3) return run_directions<DIM, P, UNI, T>(ds, prm, stream);
     if (mode == 2) return run_persistent<DIM, P, UNI, T>(ds, prm, stream);
     return run_levels<DIM, P, UNI, T>(ds, prm, stream);
 }
@@ -493,8 +868,7 @@ __global__ void __launch_bounds__(256) k_apply_L(SweepParams p, const double *__
         for (int j = 0; j < N; ++j) {
             double v;
             if constexpr (UNI) {
-                v = fma(sig, p.Mu[i * N + j], p.K[((size_t)a * N + i) * N + j]);
-                if (p.timedep) v = fma(p.tau, p.Mt[i * N + j], v);
+                v = fma(sig, p.Mu[i * N + j], p.K[((size_t)a * N + i) * N + j]);   // K includes tau * Mt
             } else {
                 const double *Gd = p.Gd + (size_t)e * DIM * N * N;
                 v = om[0] * Gd[i * N + j];

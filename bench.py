@@ -62,6 +62,51 @@ def cpu_baseline(n, seed, reps):
             "seconds_per_sweep": tm}
 
 
+def measure_traffic(args, tuning):
+    """HBM bytes per sweep from two rocprofv3 --pmc child runs (FETCH_SIZE,
+    then WRITE_SIZE), each scaled by what the counter reads for a device copy
+    of known size recorded in the same run.  Returns (bytes, detail) or
+    (None, reason)."""
+    import csv
+    import shutil
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    copy_mib = 256
+    tun = f"{tuning['mode']},{tuning['block']},{tuning['lanes']}"
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="allium_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run",
+               "--", sys.executable, os.path.abspath(__file__), "--n", str(args.n), "--seed", str(args.seed),
+               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-traffic", "--tuning", tun,
+               "--copy-mib", str(copy_mib), "--precision", args.precision]
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=os.environ.get("TMPDIR", "/tmp"))
+        path = os.path.join(d, "run_counter_collection.csv")
+        if r.returncode != 0 or not os.path.exists(path):
+            return None, f"rocprofv3 {counter} run failed ({r.returncode})"
+        sweep_kib, copy_kib = 0.0, 0.0
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                if row["Counter_Name"] != counter:
+                    continue
+                if "sweep" in row["Kernel_Name"]:
+                    sweep_kib += float(row["Counter_Value"])
+                elif "copyBuffer" in row["Kernel_Name"]:
+                    copy_kib = max(copy_kib, float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if copy_kib <= 0:
+            return None, "no calibration copy in the counter run"
+        factor = copy_mib * 1024.0 / copy_kib
+        out[counter] = (sweep_kib * 1024.0 * factor / 2.0, factor)   # 2 timed sweeps
+    total = out["FETCH_SIZE"][0] + out["WRITE_SIZE"][0]
+    detail = {"fetch_bytes": out["FETCH_SIZE"][0], "write_bytes": out["WRITE_SIZE"][0],
+              "fetch_scale": out["FETCH_SIZE"][1], "write_scale": out["WRITE_SIZE"][1]}
+    return total, detail
+
+
 def si_run(n, seed, maxit):
     """Plain GPU source iteration to a 1e-8 FP64 residual (c = 0.9999, log-sd 1.5)."""
     import allium_amd as A
@@ -93,6 +138,10 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-si", action="store_true")
     ap.add_argument("--si-maxit", type=int, default=3000)
+    ap.add_argument("--tuning", default=None, help="mode,block,lanes (skips the autotuner)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic child runs")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="device-to-device copy of this many MiB before timing (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding (angles/groups over "
@@ -109,12 +158,27 @@ def main():
     Linv = A.InverseAdvectionOperator(m, 1, 8, cfg["sigma_t"],
                                       precision=A.FP32_REFINED if args.precision == "fp32" else A.FP64)
     log(f"setup {time.perf_counter() - t0:.1f}s: {Linv.size} unknowns, {Linv.nlevels} levels")
-    mode = Linv.set_launch(args.launch)
-    log(f"launch mode {mode}")
+    if args.tuning:
+        md, bl, ln = (int(v) for v in args.tuning.split(","))
+        Linv.set_tuning(md, bl, ln)
+    else:
+        Linv.set_launch(args.launch)
+    tuning = Linv.tuning()
+    log(f"tuning {tuning}")
+    traffic, traffic_detail = (None, "skipped")
+    if not args.no_traffic:
+        traffic, traffic_detail = measure_traffic(args, tuning)
+        log(f"traffic per sweep: {traffic} ({traffic_detail})")
     q = Linv.form_source(Q=1.0)
     dq = A.DeviceVector.from_host(q)
     dpsi = A.DeviceVector(q.size)
     del q
+    if args.copy_mib:
+        nd = args.copy_mib * (1 << 20) // 8
+        ca, cb = A.DeviceVector(nd), A.DeviceVector(nd)
+        cb.copy_from(ca)
+        A.synchronize()
+        del ca, cb
     for _ in range(args.warmup):
         Linv.Mult(dq, dpsi)
     A.synchronize()
@@ -151,9 +215,12 @@ def main():
         "config": {"workload": f"3D heterogeneous one-group sweep: unit cube {args.n}^3 hex, DG p=1, S8 "
                                f"(80 dirs), lognormal sigma_t (E=1, log-sd 1), c~U[0.5,0.99], Q=1, vacuum",
                    "mesh": [args.n] * 3, "fe_order": 1, "sn": 8, "groups": 1, "seed": args.seed,
-                   "launch_mode": mode, "parallelism": "single GPU"},
+                   "tuning": tuning, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_unit": "bytes per sweep (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "traffic_detail": traffic_detail,
+                     "algorithmic_bytes_per_sweep": BYTES_PER_UNKNOWN_3D_P1 * Linv.size,
                      "bytes_per_unknown": BYTES_PER_UNKNOWN_3D_P1},
     }
     if not args.no_si:
