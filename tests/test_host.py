@@ -109,3 +109,41 @@ def test_problem_generators_are_seeded():
     # boundary nodes stay on their planes
     on_x0 = np.isclose(c[:, 0], 0.0)
     assert np.all(v[on_x0, 0] == 0.0)
+
+
+def test_cpp_operator_adapter_compiles_and_links(tmp_path):
+    """include/allium_operator.hpp builds against the library with g++ and its
+    host-only calls work without a GPU (level-symmetric S8 set in 3D)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not found")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "adapter.cpp"
+    src.write_text(r'''
+#include "allium_operator.hpp"
+#include <cstdio>
+int main() {
+    double om[80 * 3], w[80];
+    int n = 80;
+    allium::check(allium_host_quadrature(8, 3, om, w, &n));
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += w[i];
+    try {
+        allium::check(allium_host_quadrature(7, 3, om, w, &n));
+        return 2;
+    } catch (const allium::Error &e) {
+        if (e.code() != ALLIUM_ERR_ARG) return 3;
+    }
+    std::printf("%d %.15f\n", n, s);
+    return 0;
+}
+''')
+    exe = tmp_path / "adapter"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(root, "include"), str(src),
+                    "-L", libdir, "-lallium_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert int(out[0]) == 80
+    assert abs(float(out[1]) - 4 * np.pi) < 1e-12
