@@ -237,7 +237,8 @@ class InverseAdvectionOperator:
         check(self._lib.allium_sweep_set_precision(self._h, precision))
 
     def set_launch(self, mode):
-        """0 per-level launches, 1 hipGraph, 2 persistent kernel, -1 autotune."""
+        """0 per-level launches, 1 hipGraph, 2 persistent kernel (disabled),
+        3 direction sweeps, 4 box sweeps, 5 box sweeps in a hipGraph, -1 autotune."""
         check(self._lib.allium_sweep_set_launch(self._h, mode))
         info = _lib.SweepInfo()
         check(self._lib.allium_sweep_info(self._h, ctypes.byref(info)))
@@ -251,6 +252,16 @@ class InverseAdvectionOperator:
         m, b, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(self._lib.allium_sweep_get_tuning(self._h, ctypes.byref(m), ctypes.byref(b), ctypes.byref(l)))
         return {"mode": m.value, "block": b.value, "lanes": l.value}
+
+    def set_box_tiling(self, bx, by, bz=1, dirs_per_group=5, threads=256, prefetch=2):
+        """Box extents, directions of one octant per workgroup, workgroup size and
+        input prefetch (0/1/2) of box sweeps."""
+        check(self._lib.allium_sweep_set_box_tiling(self._h, bx, by, bz, dirs_per_group, threads, prefetch))
+
+    def box_tiling(self):
+        v = [ctypes.c_int() for _ in range(6)]
+        check(self._lib.allium_sweep_get_box_tiling(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("bx", "by", "bz", "dirs_per_group", "threads", "prefetch"), (x.value for x in v)))
 
     def levels(self):
         out = np.zeros(self.mesh.Ne * self.Nomega, dtype=np.int32)

@@ -50,6 +50,9 @@ SIGNATURES = {
     "allium_sweep_set_launch": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "allium_sweep_set_tuning": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "allium_sweep_get_tuning": (ctypes.c_int, [c_vp, c_ip, c_ip, c_ip]),
+    "allium_sweep_set_box_tiling": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_int]),
+    "allium_sweep_get_box_tiling": (ctypes.c_int, [c_vp, c_ip, c_ip, c_ip, c_ip, c_ip, c_ip]),
     "allium_sweep_mult": (ctypes.c_int, [c_vp, c_dp, c_dp]),
     "allium_sweep_mult_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "allium_sweep_levels": (ctypes.c_int, [c_vp, c_ip]),

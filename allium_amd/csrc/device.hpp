@@ -63,15 +63,47 @@ struct Tile {
     int nitems;
 };
 
+// Box sweeps (uniform geometry, directions that do not couple).  The mesh is
+// cut into boxes of bs[0] x bs[1] x bs[2] elements; a workgroup sweeps one box
+// for nd directions of one octant and one group, passing the face traces
+// between the box's cells through LDS planes.  Boxes on one anti-diagonal of
+// the box grid (in the octant's orientation) are independent: one launch per
+// box level.
+struct BoxPlan {
+    int bs[3];               // box extents in elements (1 beyond dim)
+    int nb[3];               // boxes per axis
+    int nd;                  // directions per workgroup
+    int nchunk;              // workgroups per (box, octant, group)
+    int maxoct;              // directions per octant row of oct_dirs
+    int noct;                // 2^dim
+    int nl;                  // levels inside a full box
+    int kstride;             // doubles per direction's matrix in LDS
+    int psz[3];              // LDS plane slots per axis (product of the other extents)
+    int lds_bytes;           // planes, matrices, |Omega|, directions, cells, level offsets
+    double vol;              // element volume: Mu = vol * (tensor product of 1D unit masses)
+    double farea[3];         // area of the element faces normal to each axis
+    const int *oct_dirs;     // noct x maxoct direction ids, -1 padded
+    const uint32_t *lcells;  // cells of a full box by level: li | lj << 10 | lk << 20
+    const int *lptr;         // nl + 1
+    const uint32_t *boxes;   // oriented box coordinates by box level: bi | bj << 10 | bk << 20
+};
+
 struct DeviceSweep {
     SweepParams prm{};
     int precision = 0;       // 0 fp64 cell solves, 1 fp32 cell solves
-    int launch_mode = 0;     // 0 levels, 1 graph, 2 persistent, 3 direction sweeps
+    int launch_mode = 0;     // 0 levels, 1 graph, 2 persistent, 3 direction sweeps, 4 box sweeps,
+                             // 5 box sweeps in a graph
     int dir_block = 512;     // workgroup size of the direction-sweep kernel
     int block = 128;
     int lanes = 4;           // lanes per cell in the level kernel (1, 2 or 4); 0 = direction-uniform
                              // waves over XCD bins (uniform geometry)
     int persistent_tile = 256;
+    // box sweeps (launch modes 4 and 5)
+    BoxPlan box{};
+    std::vector<int> box_ptr;    // box level -> first entry of box.boxes
+    int box_threads = 256;
+    int box_prefetch = 2;        // 0 none, 2 next level's inputs loaded during the current solve
+    void *box_buf = nullptr;     // device memory of the box plan tables
     // host copies of the plan
     std::vector<long> level_ptr;
     int nlevels = 0;
@@ -109,6 +141,11 @@ struct DeviceSweep {
 hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, double *psi,
                         hipStream_t stream);
 hipError_t reset_persistent(DeviceSweep &ds, hipStream_t stream);
+// one sweep as box-level launches (modes 4/5); the box plan must be built
+hipError_t launch_box_sweep(const DeviceSweep &ds, const SweepParams &prm, hipStream_t stream);
+bool box_sweep_supported(int dim, int p);
+// entry (i, j) of the unit-volume tensor-product mass the box kernel uses
+double host_unit_mass(int dim, int p, int i, int j);
 bool sweep_supported(int dim, int p, std::string &why);
 
 // streaming operator L psi (AdvectionOperator::Mult)

@@ -80,6 +80,7 @@ struct allium_sweep {
     ~allium_sweep()
     {
         for (void *p : ds.owned) hipFree(p);
+        if (ds.box_buf) hipFree(ds.box_buf);
         if (ds.graph_exec) hipGraphExecDestroy(ds.graph_exec);
         if (ds.graph_stream) hipStreamDestroy(ds.graph_stream);
         if (ds.ev0) hipEventDestroy(ds.ev0);
@@ -258,6 +259,159 @@ int upload_all(allium_sweep *h)
     return ALLIUM_OK;
 }
 
+// Box-sweep plan (launch modes 4/5): oriented box order, the level order of the
+// cells inside a full box and the directions of each octant.
+int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads, int prefetch)
+{
+    auto &ds = h->ds;
+    const int dim = h->mesh.dim, n = h->ed.n, nf = h->ed.nf;
+    if (!h->mesh.uniform) return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps need uniform geometry");
+    if (!ds.angle_independent)
+        return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps need directions that do not couple (no reflective side)");
+    if (!allium::box_sweep_supported(dim, h->ed.p))
+        return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps implement 2D p=0..2 and 3D p=0..1");
+    int bs[3] = {bx, by, dim == 3 ? bz : 1};
+    for (int d = 0; d < dim; d++) {
+        if (bs[d] < 1 || bs[d] > 1023) return fail(ALLIUM_ERR_ARG, "box extents must be 1..1023");
+        bs[d] = std::min(bs[d], h->mesh.nel[d]);
+    }
+    if (nd < 1 || nd > 64) return fail(ALLIUM_ERR_ARG, "directions per workgroup must be 1..64");
+    if (threads < 64 || threads > 256 || threads % 64)
+        return fail(ALLIUM_ERR_ARG, "box sweep workgroup size must be 64..256, a multiple of 64");
+    if (prefetch != 0 && prefetch != 2) return fail(ALLIUM_ERR_ARG, "box sweep prefetch must be 0 or 2");
+    const int No = h->quad.size(), noct = 1 << dim;
+    std::vector<std::vector<int>> oct(noct);
+    for (int a = 0; a < No; a++) {
+        int o = 0;
+        for (int d = 0; d < dim; d++)
+            if (h->quad.omega[3 * a + d] < 0) o |= 1 << d;
+        oct[o].push_back(a);
+    }
+    int maxoct = 1;
+    for (auto &v : oct) maxoct = std::max(maxoct, (int)v.size());
+    nd = std::min(nd, maxoct);
+    allium::BoxPlan bp{};
+    for (int d = 0; d < 3; d++) {
+        bp.bs[d] = bs[d];
+        bp.nb[d] = d < dim ? (h->mesh.nel[d] + bs[d] - 1) / bs[d] : 1;
+    }
+    bp.nd = nd;
+    bp.nchunk = (maxoct + nd - 1) / nd;
+    bp.maxoct = bp.nchunk * nd;
+    bp.noct = noct;
+    bp.nl = bs[0] + bs[1] + bs[2] - 2;
+    bp.kstride = n * n + 1;   // odd stride: the directions' matrices start in different LDS banks
+    long plane = 0;
+    for (int ax = 0; ax < 3; ax++) {
+        int v = 1;
+        for (int o = 0; o < dim; o++)
+            if (o != ax) v *= bs[o];
+        bp.psz[ax] = ax < dim ? v : 0;
+        plane += bp.psz[ax];
+    }
+    const long vol = (long)bs[0] * bs[1] * bs[2];
+    const long lds = 8L * ((long)nd * nf * plane + (long)nd * bp.kstride + 3L * nd) + 4L * (nd + vol + bp.nl + 1);
+    // the box kernel builds Mu and the face masses as tensor products of the
+    // exact 1D masses: check that the assembled matrices are those
+    {
+        const auto &m = h->mesh;
+        bp.vol = 1.0;
+        for (int d = 0; d < dim; d++) bp.vol *= m.h[d];
+        for (int ax = 0; ax < 3; ax++) {
+            bp.farea[ax] = 1.0;
+            for (int d = 0; d < dim; d++)
+                if (d != ax) bp.farea[ax] *= m.h[d];
+        }
+        double err = 0.0, ref = 0.0, errf = 0.0, reff = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                const double c = bp.vol * allium::host_unit_mass(dim, h->ed.p, i, j);
+                err = std::max(err, std::fabs(h->ed.Mu[(size_t)i * n + j] - c));
+                ref = std::max(ref, std::fabs(c));
+            }
+        for (int sd = 0; sd < 2 * dim; sd++)
+            for (int k = 0; k < nf; k++)
+                for (int l = 0; l < nf; l++) {
+                    const double c = bp.farea[sd >> 1] * allium::host_unit_mass(dim - 1, h->ed.p, k, l);
+                    errf = std::max(errf, std::fabs(h->ed.Fface[((size_t)sd * nf + k) * nf + l] - c));
+                    reff = std::max(reff, std::fabs(c));
+                }
+        if (!(err <= 1e-13 * ref) || !(errf <= 1e-13 * reff))
+            return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps: element matrices are not the tensor-product masses "
+                                                "of an axis-aligned box");
+    }
+    int dev = 0, maxlds = 65536;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (lds > std::min(maxlds, 65536))
+        return fail(ALLIUM_ERR_ARG, "box tiling needs " + std::to_string(lds) +
+                                        " bytes of LDS per workgroup (limit 65536): use smaller boxes or fewer "
+                                        "directions per workgroup");
+    bp.lds_bytes = (int)lds;
+    std::vector<uint32_t> lcells, boxes;
+    std::vector<int> lptr(1, 0), bptr(1, 0), odirs((size_t)noct * bp.maxoct, -1);
+    for (int l = 0; l < bp.nl; l++) {
+        for (int k = 0; k < bs[2]; k++)
+            for (int j = 0; j < bs[1]; j++) {
+                const int i = l - j - k;
+                if (i >= 0 && i < bs[0]) lcells.push_back((uint32_t)i | (uint32_t)j << 10 | (uint32_t)k << 20);
+            }
+        lptr.push_back((int)lcells.size());
+    }
+    const int nbl = bp.nb[0] + bp.nb[1] + bp.nb[2] - 2;
+    for (int l = 0; l < nbl; l++) {
+        for (int k = 0; k < bp.nb[2]; k++)
+            for (int j = 0; j < bp.nb[1]; j++) {
+                const int i = l - j - k;
+                if (i >= 0 && i < bp.nb[0]) boxes.push_back((uint32_t)i | (uint32_t)j << 10 | (uint32_t)k << 20);
+            }
+        bptr.push_back((int)boxes.size());
+    }
+    for (int o = 0; o < noct; o++)
+        for (size_t i = 0; i < oct[o].size(); i++) odirs[(size_t)o * bp.maxoct + i] = oct[o][i];
+    // one device buffer: oct_dirs | lptr | lcells | boxes (all 32-bit)
+    std::vector<uint32_t> buf;
+    buf.insert(buf.end(), odirs.begin(), odirs.end());
+    const size_t o_lptr = buf.size();
+    buf.insert(buf.end(), lptr.begin(), lptr.end());
+    const size_t o_lcells = buf.size();
+    buf.insert(buf.end(), lcells.begin(), lcells.end());
+    const size_t o_boxes = buf.size();
+    buf.insert(buf.end(), boxes.begin(), boxes.end());
+    void *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(uint32_t) * buf.size()));
+    hipError_t e = hipMemcpy(d, buf.data(), sizeof(uint32_t) * buf.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return fail(ALLIUM_ERR_DEVICE, std::string("box plan upload: ") + hipGetErrorString(e));
+    }
+    if (ds.box_buf) HIP_TRY(hipFree(ds.box_buf));
+    ds.box_buf = d;
+    const uint32_t *db = (const uint32_t *)d;
+    bp.oct_dirs = (const int *)db;
+    bp.lptr = (const int *)(db + o_lptr);
+    bp.lcells = db + o_lcells;
+    bp.boxes = db + o_boxes;
+    ds.box = bp;
+    ds.box_ptr = bptr;
+    ds.box_threads = threads;
+    ds.box_prefetch = prefetch;
+    if (ds.graph_exec) { hipGraphExecDestroy(ds.graph_exec); ds.graph_exec = nullptr; }
+    return ALLIUM_OK;
+}
+
+// default box tiling: 8-element boxes, directions per workgroup as LDS allows
+int default_box_plan(allium_sweep *h)
+{
+    const int dim = h->mesh.dim;
+    int rc = ALLIUM_ERR_ARG;
+    for (int nd : {5, 4, 3, 2, 1}) {
+        rc = build_box_plan(h, 8, 8, dim == 3 ? 8 : 1, nd, 256, 2);
+        if (rc != ALLIUM_ERR_ARG) return rc;
+    }
+    return rc;
+}
+
 bool finite_all(const double *v, long n)
 {
     for (long i = 0; i < n; i++)
@@ -432,9 +586,13 @@ int allium_sweep_mult(allium_sweep_t h, const double *source, double *psi)
 
 int allium_sweep_set_launch(allium_sweep_t h, int mode)
 {
-    if (!h || mode < -1 || mode > 3) return fail(ALLIUM_ERR_ARG, "bad launch mode");
+    if (!h || mode < -1 || mode > 5) return fail(ALLIUM_ERR_ARG, "bad launch mode");
     if (mode == 3 && !h->ds.angle_independent)
         return fail(ALLIUM_ERR_UNSUPPORTED, "direction sweeps need directions that do not couple (no reflective side)");
+    if ((mode == 4 || mode == 5) && !h->ds.box_buf) {
+        int rc = default_box_plan(h);
+        if (rc) return rc;
+    }
     if (mode == 2)
         return fail(ALLIUM_ERR_UNSUPPORTED,
                     "persistent sweep kernel is disabled: a run on MI355X did not terminate and the cause "
@@ -445,25 +603,41 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
     HIP_TRY(h->b.ensure(bytes));
     HIP_TRY(h->c.ensure(bytes));
     HIP_TRY(hipMemsetAsync(h->b.p, 0, bytes, h->stream));
-    struct Cand { int mode, block, lanes; };
+    struct Cand { int mode, block, lanes, bx, by, bz, nd, pf; };
     std::vector<Cand> cands;
     if (h->ds.angle_independent)
-        for (int bl : {256, 512}) cands.push_back({3, bl, 1});
+        for (int bl : {256, 512}) cands.push_back({3, bl, 1, 0, 0, 0, 0, 0});
     for (int md : {0, 1})
         for (int bl : {64, 128, 256})
             for (int ln : {0, 1, 2, 4}) {
                 if (ln == 0 && !h->ds.d_wcells) continue;
                 if (ln > 0 && h->ed.n % ln) continue;
-                cands.push_back({md, bl, ln});
+                cands.push_back({md, bl, ln, 0, 0, 0, 0, 0});
             }
+    const bool boxes = h->mesh.uniform && h->ds.angle_independent && allium::box_sweep_supported(h->mesh.dim, h->ed.p);
+    if (boxes) {
+        const bool d3 = h->mesh.dim == 3;
+        for (int b : d3 ? std::vector<int>{4, 8, 16} : std::vector<int>{8, 16, 32})
+            for (int nd : {1, 2, 3, 5, 10})
+                for (int th : {128, 256})
+                    for (int pf : {0, 2}) cands.push_back({5, th, 1, b, b, d3 ? b : 1, nd, pf});
+    }
+    // keep the caller's box tiling to restore it when a non-box mode wins
+    const allium::BoxPlan keep_box = h->ds.box;
+    const int keep_threads = h->ds.box_threads, keep_pf = h->ds.box_prefetch;
     double best = 1e30;
-    Cand pick = {0, 128, 1};
+    Cand pick = {0, 128, 1, 0, 0, 0, 0, 0};
     const bool timing = h->ds.timing;
     h->ds.timing = false;
     for (const auto &cd : cands) {
+        if (cd.mode >= 4) {
+            int rc = build_box_plan(h, cd.bx, cd.by, cd.bz, cd.nd, cd.block, cd.pf);
+            if (rc == ALLIUM_ERR_ARG) continue;   // LDS does not fit this tiling
+            if (rc) { h->ds.timing = timing; return rc; }
+        }
         h->ds.launch_mode = cd.mode;
         if (cd.mode == 3) h->ds.dir_block = cd.block;
-        else h->ds.block = cd.block;
+        else if (cd.mode < 3) h->ds.block = cd.block;
         h->ds.lanes = cd.lanes;
         if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
         int rc = do_sweep(h, h->b.d(), nullptr, h->c.d(), h->stream);
@@ -488,8 +662,15 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
     h->ds.timing = timing;
     h->ds.launch_mode = pick.mode;
     if (pick.mode == 3) h->ds.dir_block = pick.block;
-    else h->ds.block = pick.block;
+    else if (pick.mode < 3) h->ds.block = pick.block;
     h->ds.lanes = pick.lanes;
+    if (pick.mode >= 4) {
+        int rc = build_box_plan(h, pick.bx, pick.by, pick.bz, pick.nd, pick.block, pick.pf);
+        if (rc) return rc;
+    } else if (boxes && keep_box.nd > 0) {
+        int rc = build_box_plan(h, keep_box.bs[0], keep_box.bs[1], keep_box.bs[2], keep_box.nd, keep_threads, keep_pf);
+        if (rc) return rc;
+    }
     if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
     h->ds.sweeps = 0;
     h->ds.sweep_seconds = 0;
@@ -498,13 +679,20 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
 
 int allium_sweep_set_tuning(allium_sweep_t h, int mode, int block, int lanes)
 {
-    // the level kernels are compiled with __launch_bounds__(256), the
+    // the level and box kernels are compiled with __launch_bounds__(256), the
     // direction-sweep kernel (mode 3) with __launch_bounds__(512)
     const int maxb = mode == 3 ? 512 : 256;
-    if (!h || mode < 0 || mode > 3 || block < 64 || block > maxb || block % 64 ||
+    if (!h || mode < 0 || mode > 5 || block < 64 || block > maxb || block % 64 ||
         (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4))
-        return fail(ALLIUM_ERR_ARG, "bad tuning (mode 0..3, block 64..256 step 64 (512 for mode 3), lanes 0/1/2/4)");
-    if (mode == 3 && lanes != 1) return fail(ALLIUM_ERR_ARG, "bad tuning: direction sweeps use one lane per cell");
+        return fail(ALLIUM_ERR_ARG, "bad tuning (mode 0..5, block 64..256 step 64 (512 for mode 3), lanes 0/1/2/4)");
+    if (mode >= 3 && lanes != 1)
+        return fail(ALLIUM_ERR_ARG, "bad tuning: direction and box sweeps use one lane per cell");
+    if (mode >= 4) {
+        int rc = allium_sweep_set_launch(h, mode);
+        if (rc) return rc;
+        const allium::BoxPlan &bp = h->ds.box;
+        return build_box_plan(h, bp.bs[0], bp.bs[1], bp.bs[2], bp.nd, block, h->ds.box_prefetch);
+    }
     if (lanes == 0 && !h->ds.d_wcells)
         return fail(ALLIUM_ERR_ARG, "direction-uniform waves need uniform geometry");
     if (lanes > 0 && h->ed.n % lanes) return fail(ALLIUM_ERR_ARG, "lanes per cell must divide the dofs per element");
@@ -521,8 +709,31 @@ int allium_sweep_get_tuning(allium_sweep_t h, int *mode, int *block, int *lanes)
 {
     if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
     if (mode) *mode = h->ds.launch_mode;
-    if (block) *block = h->ds.launch_mode == 3 ? h->ds.dir_block : h->ds.block;
-    if (lanes) *lanes = h->ds.lanes;
+    if (block)
+        *block = h->ds.launch_mode == 3 ? h->ds.dir_block : h->ds.launch_mode >= 4 ? h->ds.box_threads : h->ds.block;
+    if (lanes) *lanes = h->ds.launch_mode >= 3 ? 1 : h->ds.lanes;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_set_box_tiling(allium_sweep_t h, int bx, int by, int bz, int dirs_per_group, int threads,
+                                int prefetch)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    return build_box_plan(h, bx, by, bz, dirs_per_group, threads, prefetch);
+}
+
+int allium_sweep_get_box_tiling(allium_sweep_t h, int *bx, int *by, int *bz, int *dirs_per_group, int *threads,
+                                int *prefetch)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    if (!h->ds.box_buf) return fail(ALLIUM_ERR_UNSUPPORTED, "no box tiling (box sweeps not set up)");
+    const allium::BoxPlan &bp = h->ds.box;
+    if (bx) *bx = bp.bs[0];
+    if (by) *by = bp.bs[1];
+    if (bz) *bz = bp.bs[2];
+    if (dirs_per_group) *dirs_per_group = bp.nd;
+    if (threads) *threads = h->ds.box_threads;
+    if (prefetch) *prefetch = h->ds.box_prefetch;
     return ALLIUM_OK;
 }
 

@@ -9,104 +9,12 @@
 // whose workgroups take tiles in level order from an atomic ticket and wait on
 // per-(level, direction) completion counters.
 #include "device.hpp"
+#include "kernel_common.hpp"
 
 #include <algorithm>
 #include <cstdio>
 
 namespace allium {
-
-typedef double d2v __attribute__((ext_vector_type(2)));
-// read-only tables seen through the constant address space: uniform
-// addresses become scalar loads, non-uniform ones stay vector loads
-typedef const __attribute__((address_space(4))) double *cdptr;
-__device__ __forceinline__ cdptr as_const(const double *p) { return (cdptr)p; }
-
-template <int DIM, int P>
-struct Sh {
-    static constexpr int Q = P + 1;
-    static constexpr int N = DIM == 2 ? Q * Q : Q * Q * Q;
-    static constexpr int NF = DIM == 2 ? Q : Q * Q;
-    static constexpr int NS = 2 * DIM;
-};
-
-// k-th node (face-lexicographic) of element side s = 2*axis + {0 low, 1 high}
-template <int DIM, int P>
-__host__ __device__ constexpr int side_node(int s, int k)
-{
-    const int ax = s >> 1, hi = s & 1, Q = P + 1;
-    int idx[3] = {0, 0, 0};
-    int r = k;
-    for (int d = 0; d < DIM; ++d) {
-        if (d == ax) idx[d] = hi ? P : 0;
-        else { idx[d] = r % Q; r /= Q; }
-    }
-    return idx[0] + Q * (idx[1] + Q * idx[2]);
-}
-
-__device__ __forceinline__ int fast_div(int a, int d, float inv)
-{
-    int q = (int)((float)a * inv);
-    const int r = a - q * d;
-    if (r < 0) q--;
-    else if (r >= d) q++;
-    return q;
-}
-
-__device__ __forceinline__ double fast_rcp(double a)
-{
-    double r = __builtin_amdgcn_rcp(a);
-    double e = fma(-a, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-a, r, 1.0);
-    return fma(r, e, r);
-}
-
-__device__ __forceinline__ float fast_rcp(float a)
-{
-    const float r = __builtin_amdgcn_rcpf(a);
-    return fmaf(r, fmaf(-a, r, 1.0f), r);
-}
-
-template <int N, typename T>
-__device__ __forceinline__ void lu_factor_solve(T (&A)[N][N], T (&x)[N])
-{
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const T inv = fast_rcp(A[k][k]);
-        A[k][k] = inv;
-#pragma unroll
-        for (int i = k + 1; i < N; ++i) {
-            const T l = A[i][k] * inv;
-            A[i][k] = l;
-#pragma unroll
-            for (int j = k + 1; j < N; ++j) A[i][j] = fma(-l, A[k][j], A[i][j]);
-            x[i] = fma(-l, x[k], x[i]);
-        }
-    }
-#pragma unroll
-    for (int i = N - 1; i >= 0; --i) {
-        T s = x[i];
-#pragma unroll
-        for (int j = i + 1; j < N; ++j) s = fma(-A[i][j], x[j], s);
-        x[i] = s * A[i][i];
-    }
-}
-
-template <int N, typename T>
-__device__ __forceinline__ void lu_resolve(const T (&A)[N][N], T (&x)[N])
-{
-#pragma unroll
-    for (int k = 0; k < N; ++k)
-#pragma unroll
-        for (int i = k + 1; i < N; ++i) x[i] = fma(-A[i][k], x[k], x[i]);
-#pragma unroll
-    for (int i = N - 1; i >= 0; --i) {
-        T s = x[i];
-#pragma unroll
-        for (int j = i + 1; j < N; ++j) s = fma(-A[i][j], x[j], s);
-        x[i] = s * A[i][i];
-    }
-}
 
 // One cell (e, a, g).  Every global load (source, isotropic source, upwind
 // traces, general-geometry matrices) is issued before any of them is used and
@@ -812,8 +720,9 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
     prm.psi = psi;
     hipError_t err = hipSuccess;
     if (ds.timing) hipEventRecord(ds.ev0, stream);
-    if (ds.launch_mode == 1) {
-        // hipGraph of the level launches, re-captured when the vectors change
+    if (ds.launch_mode == 1 || ds.launch_mode == 5) {
+        // hipGraph of the level (1) or box-level (5) launches, re-captured when
+        // the vectors change
         if (!ds.graph_exec || ds.graph_src != src || ds.graph_iso != iso || ds.graph_psi != psi ||
             ds.graph_prec != ds.precision) {
             if (ds.graph_exec) { hipGraphExecDestroy(ds.graph_exec); ds.graph_exec = nullptr; }
@@ -821,8 +730,13 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
             hipGraph_t graph;
             err = hipStreamBeginCapture(ds.graph_stream, hipStreamCaptureModeThreadLocal);
             if (err != hipSuccess) return err;
-            dispatch(ds, prm, ds.graph_stream, 0);
+            const hipError_t lerr = ds.launch_mode == 5 ? launch_box_sweep(ds, prm, ds.graph_stream)
+                                                        : dispatch(ds, prm, ds.graph_stream, 0);
             err = hipStreamEndCapture(ds.graph_stream, &graph);
+            if (lerr != hipSuccess) {
+                if (err == hipSuccess) hipGraphDestroy(graph);
+                return lerr;
+            }
             if (err != hipSuccess) return err;
             err = hipGraphInstantiate(&ds.graph_exec, graph, nullptr, nullptr, 0);
             hipGraphDestroy(graph);
@@ -830,6 +744,8 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
             ds.graph_src = src; ds.graph_iso = iso; ds.graph_psi = psi; ds.graph_prec = ds.precision;
         }
         err = hipGraphLaunch(ds.graph_exec, stream);
+    } else if (ds.launch_mode == 4) {
+        err = launch_box_sweep(ds, prm, stream);
     } else {
         err = dispatch(ds, prm, stream, ds.launch_mode);
     }

@@ -76,13 +76,17 @@ def measure_traffic(args, tuning):
         return None, "rocprofv3 not found"
     copy_mib = 256
     tun = f"{tuning['mode']},{tuning['block']},{tuning['lanes']}"
+    box = []
+    if "box" in tuning:
+        b = tuning["box"]
+        box = ["--box", f"{b['bx']},{b['by']},{b['bz']},{b['dirs_per_group']},{b['threads']},{b['prefetch']}"]
     out = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="allium_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run",
                "--", sys.executable, os.path.abspath(__file__), "--n", str(args.n), "--seed", str(args.seed),
                "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-traffic", "--tuning", tun,
-               "--copy-mib", str(copy_mib), "--precision", args.precision]
+               "--copy-mib", str(copy_mib), "--precision", args.precision, *box]
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=os.environ.get("TMPDIR", "/tmp"))
         path = os.path.join(d, "run_counter_collection.csv")
         if r.returncode != 0 or not os.path.exists(path):
@@ -132,13 +136,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--launch", type=int, default=-1, help="-1 autotune, 0 levels, 1 graph, 2 persistent")
+    ap.add_argument("--launch", type=int, default=-1,
+                    help="-1 autotune, 0 levels, 1 graph, 3 direction sweeps, 4 box sweeps, 5 box sweeps in a graph")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-si", action="store_true")
     ap.add_argument("--si-maxit", type=int, default=3000)
     ap.add_argument("--tuning", default=None, help="mode,block,lanes (skips the autotuner)")
+    ap.add_argument("--box", default=None, help="bx,by,bz,dirs_per_group,threads,prefetch of box sweeps (modes 4/5)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic child runs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="device-to-device copy of this many MiB before timing (counter calibration)")
@@ -160,10 +166,15 @@ def main():
     log(f"setup {time.perf_counter() - t0:.1f}s: {Linv.size} unknowns, {Linv.nlevels} levels")
     if args.tuning:
         md, bl, ln = (int(v) for v in args.tuning.split(","))
+        if args.box:
+            bx, by, bz, nd, th, pf = (int(v) for v in args.box.split(","))
+            Linv.set_box_tiling(bx, by, bz, dirs_per_group=nd, threads=th, prefetch=pf)
         Linv.set_tuning(md, bl, ln)
     else:
         Linv.set_launch(args.launch)
     tuning = Linv.tuning()
+    if tuning["mode"] >= 4:
+        tuning["box"] = Linv.box_tiling()
     log(f"tuning {tuning}")
     traffic, traffic_detail = (None, "skipped")
     if not args.no_traffic:

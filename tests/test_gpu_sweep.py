@@ -45,7 +45,7 @@ PERSISTENT_OFF = ("persistent kernel (launch mode 2) disabled: its first MI355X 
 
 
 @pytest.mark.parametrize("name,nel,p,sn,sig,jitter", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("mode", [0, 1, pytest.param(2, marks=pytest.mark.skip(reason=PERSISTENT_OFF)), 3])
+@pytest.mark.parametrize("mode", [0, 1, pytest.param(2, marks=pytest.mark.skip(reason=PERSISTENT_OFF)), 3, 4, 5])
 def test_sweep_matches_oracle(name, nel, p, sn, sig, jitter, mode):
     print(f"case {name} mode {mode}", flush=True)
     dim = len(nel)
@@ -55,8 +55,12 @@ def test_sweep_matches_oracle(name, nel, p, sn, sig, jitter, mode):
     v = P.jittered_vertices(nel, jitter, rng) if jitter else None
     bc = (1, 0, 0, 0, 0, 0) if name == "3d_reflect" else None
     Linv, pr = _pair(nel, p, sn, sigma, vertices=v, bc=bc)
-    if mode == 3 and bc is not None:
+    if mode in (3, 4, 5) and bc is not None:
         with pytest.raises(A.AlliumError, match="reflective"):
+            Linv.set_launch(mode)
+        return
+    if mode in (4, 5) and jitter:
+        with pytest.raises(A.AlliumError, match="uniform geometry"):
             Linv.set_launch(mode)
         return
     Linv.set_launch(mode)
@@ -113,6 +117,88 @@ def test_direction_sweeps_block_sizes(block):
         assert rel_err(Linv.Mult(q, np.empty_like(q)), pr.sweep(q)) < 1e-4
 
 
+BOX_TILINGS = [
+    # (mesh, p, sn, box extents, directions per workgroup, threads)
+    ((14, 13, 12), 1, 8, (4, 5, 3), 3, 64),
+    ((14, 13, 12), 1, 8, (8, 8, 8), 5, 256),
+    ((14, 13, 12), 1, 8, (16, 16, 16), 2, 128),
+    ((14, 13, 12), 1, 8, (1, 1, 1), 10, 128),
+    ((7, 6, 5), 0, 6, (3, 2, 4), 4, 192),
+    ((9, 7), 2, 8, (4, 3), 3, 64),
+    ((33, 31), 1, 8, (16, 16), 10, 256),
+    ((12, 10), 0, 4, (5, 2), 1, 64),
+]
+
+
+@pytest.mark.parametrize("nel,p,sn,box,nd,threads", BOX_TILINGS,
+                         ids=[f"{'x'.join(map(str, c[0]))}_p{c[1]}_b{'x'.join(map(str, c[3]))}_d{c[4]}_t{c[5]}"
+                              for c in BOX_TILINGS])
+def test_box_tilings_match_oracle(nel, p, sn, box, nd, threads):
+    """Box sweeps on ragged meshes (boxes that do not divide the mesh), every
+    directions-per-workgroup split and workgroup size, FP64 and FP32."""
+    print(f"box case {nel} p{p} S{sn} box {box} nd {nd} threads {threads}", flush=True)
+    ne = int(np.prod(nel))
+    rng = np.random.default_rng(53)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+    Linv, pr = _pair(nel, p, sn, sigma)
+    q = pr.form_source(Q=1.0, inflow=np.ones((1, 2 * len(nel)))) + rng.uniform(0, 0.1, pr.size)
+    ref = pr.sweep(q)
+    for pf in (0, 2):
+        if len(nel) == 3:
+            Linv.set_box_tiling(*box, dirs_per_group=nd, threads=threads, prefetch=pf)
+        else:
+            Linv.set_box_tiling(box[0], box[1], 1, dirs_per_group=nd, threads=threads, prefetch=pf)
+        for mode in (4, 5):
+            Linv.set_launch(mode)
+            assert rel_err(Linv.Mult(q, np.empty_like(q)), ref) < FP64_TOL
+        buf = q.copy()
+        Linv.Mult(buf, buf)   # in place: prefetched sources of later cells are not overwritten
+        assert rel_err(buf, ref) < FP64_TOL
+    Linv.set_precision(A.FP32_REFINED)
+    assert rel_err(Linv.Mult(q, np.empty_like(q)), ref) < 1e-4
+
+
+def test_box_tiling_errors():
+    nel = (16, 16, 16)
+    Linv, _ = _pair(nel, 1, 8, np.ones((1, 4096)))
+    with pytest.raises(A.AlliumError, match="LDS"):
+        Linv.set_box_tiling(16, 16, 16, dirs_per_group=10, threads=256)
+    with pytest.raises(A.AlliumError, match="workgroup size"):
+        Linv.set_box_tiling(8, 8, 8, dirs_per_group=5, threads=512)
+    Linv.set_box_tiling(8, 8, 8, dirs_per_group=5, threads=128)
+    assert Linv.box_tiling() == {"bx": 8, "by": 8, "bz": 8, "dirs_per_group": 5, "threads": 128, "prefetch": 2}
+    with pytest.raises(A.AlliumError, match="prefetch"):
+        Linv.set_box_tiling(8, 8, 8, dirs_per_group=5, threads=128, prefetch=1)
+
+
+def test_full_size_box_sweep_properties():
+    """64^3 hex, p=1, S8, lognormal sigma_t: size-independent checks at the
+    benchmark size.  The box sweep, the level waves and the direction sweeps
+    agree; L (L^{-1} q) = q through the independently assembled streaming
+    operator; particle balance equals the source (src/sweep.cpp:1400-1462)."""
+    nel = (64, 64, 64)
+    ne = 64 ** 3
+    sigma = P.lognormal_sigma(ne, 1.0, np.random.default_rng(0)).reshape(1, ne)
+    Linv = A.InverseAdvectionOperator(A.Mesh(nel), 1, 8, sigma)
+    q = Linv.form_source(Q=1.0)
+    dq = A.DeviceVector.from_host(q)
+    out = {}
+    for mode in (5, 0, 3):
+        Linv.set_launch(mode)
+        dpsi = A.DeviceVector(q.size)
+        Linv.Mult(dq, dpsi)
+        out[mode] = dpsi
+    ref = out[0].to_host()
+    for mode in (5, 3):
+        assert rel_err(out[mode].to_host(), ref) < FP64_TOL
+    L = A.AdvectionOperator(Linv)
+    dLpsi = A.DeviceVector(q.size)
+    L.Mult(out[5], dLpsi)
+    assert rel_err(dLpsi.to_host(), q) < 1e-12
+    total_source = 4 * np.pi * 1.0   # Q = 1 per steradian over the unit cube
+    assert abs(L.ComputeBalance(out[5]) - total_source) < 1e-10 * total_source
+
+
 def test_autotune_picks_a_valid_variant():
     nel = (16, 16, 16)
     ne = 4096
@@ -121,7 +207,8 @@ def test_autotune_picks_a_valid_variant():
     Linv, pr = _pair(nel, 1, 8, sigma)
     mode = Linv.set_launch(-1)
     t = Linv.tuning()
-    assert mode in (0, 1, 3) and t["lanes"] in (0, 1, 2, 4)
+    print("autotune picked", t, Linv.box_tiling() if mode >= 4 else "")
+    assert mode in (0, 1, 3, 4, 5) and t["lanes"] in (0, 1, 2, 4)
     q = pr.form_source(Q=1.0)
     assert rel_err(Linv.Mult(q, np.empty_like(q)), pr.sweep(q)) < FP64_TOL
 
@@ -134,7 +221,7 @@ def test_multigroup_time_absorption_matches_oracle():
     Linv.SetTimeAbsorption(7.5)
     pr.set_time_absorption(7.5)
     q = pr.form_source(Q=rng.uniform(0.5, 1.5, (G, ne)), inflow=np.ones((G, 6)))
-    for mode in (0, 1, 3):
+    for mode in (0, 1, 3, 4, 5):
         Linv.set_launch(mode)
         assert rel_err(Linv.Mult(q, np.empty_like(q)), pr.sweep(q)) < FP64_TOL
     # new cross sections (AssembleLocalMatrices)
