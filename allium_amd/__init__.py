@@ -323,10 +323,24 @@ class DiscreteToMoment:
 
 
 def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000, precision=FP64,
-                     want_psi=True):
-    """chive's SI: returns (phi, psi, iterations, norms) (exe/chive.cpp:700-735)."""
+                     want_psi=True, accel=None, history=None, prolong=1, patience=2):
+    """chive's SI: returns (phi, psi, iterations, norms) (exe/chive.cpp:700-735).
+
+    ``accel``: a cnn.FluxCorrector applied after every iteration (switched off
+    once the residual has failed to drop ``patience`` times in a row);
+    ``prolong`` 1 spreads its element corrections continuously over the
+    vertices (p = 1), 0 keeps them element-constant.  ``history``: a
+    (maxit, 2, Ne) float64 array that receives, for every iteration but the
+    last, the element-averaged scalar flux after it and the element-averaged
+    change it made before the correction."""
     lib = Linv._lib
-    opt = _lib.SIOptions(abstol, reltol, int(maxit), int(precision), 0, None)
+    hist_p = None
+    if history is not None:
+        if history.dtype != np.float64 or not history.flags.c_contiguous or history.size < 2 * maxit * Linv.mesh.Ne:
+            raise ValueError("history must be a contiguous float64 array of maxit x 2 x Ne")
+        hist_p = _d(history)
+    opt = _lib.SIOptions(abstol, reltol, int(maxit), int(precision), 0 if accel is None else 1,
+                         None if accel is None else accel.handle, int(prolong), int(patience), hist_p)
     phi = np.empty(Linv.phi_size)
     psi = np.empty(Linv.size) if want_psi else None
     norms = np.zeros(maxit + 1)

@@ -33,7 +33,8 @@ class SweepInfo(ctypes.Structure):
 
 class SIOptions(ctypes.Structure):
     _fields_ = [("abstol", ctypes.c_double), ("reltol", ctypes.c_double), ("maxit", ctypes.c_int),
-                ("precision", ctypes.c_int), ("accel", ctypes.c_int), ("accel_handle", c_vp)]
+                ("precision", ctypes.c_int), ("accel", ctypes.c_int), ("accel_handle", c_vp),
+                ("accel_prolong", ctypes.c_int), ("accel_patience", ctypes.c_int), ("history", c_dp)]
 
 
 # name -> (restype, argtypes)
@@ -74,6 +75,18 @@ SIGNATURES = {
     "allium_host_element_matrices": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.c_int, c_dp,
                                                     c_dp, c_dp]),
     "allium_last_error": (ctypes.c_char_p, []),
+    # allium_cnn.h
+    "allium_cnn_create": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.POINTER(c_vp)]),
+    "allium_cnn_destroy": (None, [c_vp]),
+    "allium_cnn_num_params": (ctypes.c_long, []),
+    "allium_cnn_get_params": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
+    "allium_cnn_set_params": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
+    "allium_cnn_forward_device": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp,
+                                                 c_vp, c_vp]),
+    "allium_cnn_gradient_device": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp,
+                                                  c_vp, ctypes.POINTER(ctypes.c_float), c_dp]),
+    "allium_cnn_train_step_device": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    c_vp, c_vp, ctypes.c_float, c_dp]),
     # allium_device.h
     "allium_device_count": (ctypes.c_int, [c_ip]),
     "allium_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),

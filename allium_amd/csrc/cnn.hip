@@ -1,0 +1,734 @@
+// Learned low-order flux correction: a 5-layer 3x3x3 convolutional network
+// on the element-centred grid, with BF16 MFMA forward and backward kernels
+// (FP32 accumulation) for gfx950 and an FP32 Adam optimiser.  No counterpart
+// in the reference; it stands in for the diffusion synthetic acceleration of
+// src/transport_op.hpp:38-59 inside the source iteration.
+//
+// Convolution as an implicit GEMM on v_mfma_f32_16x16x32_bf16: a wave owns 16
+// consecutive voxels (rows of the A tile) and all output channels (N tiles of
+// 16); the K dimension runs over (kernel offset, input channel) in chunks of
+// 8 channels of one offset, so every A operand is one 16-byte load of a
+// channels-last voxel row.  Lane l holds A[l % 16][8 (l / 16) + t],
+// B[8 (l / 16) + t][l % 16] and D[4 (l / 16) + r][l % 16].
+//
+// Weight gradients: a workgroup owns one (sample, z) plane; for each row and
+// 32-voxel chunk it stages the output gradient (transposed, channel-major)
+// and the 9 neighbouring input rows with a one-voxel x halo in LDS, then
+// accumulates all 27 offsets of its (output-tile, input-tile) pair with
+// voxels as the K dimension.  Per-plane partials are reduced in a second pass
+// in a fixed order, so gradients are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/allium_cnn.h"
+#include "cnn_internal.hpp"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NL = 5;
+constexpr int CI_[NL] = {3, 32, 32, 32, 32};
+constexpr int CO_[NL] = {32, 32, 32, 32, 1};
+constexpr int CIP_[NL] = {8, 32, 32, 32, 32};
+constexpr int COP_[NL] = {32, 32, 32, 32, 16};
+constexpr int HID = 32;
+
+struct Geo {
+    int B, nz, ny, nx;
+    long long T;   // voxels
+};
+
+// ---------------------------------------------------------------------------
+// convolution (forward, and backward-data with flipped/transposed weights)
+// MODE 0: + bias, ReLU, bf16 out [T][COP]
+// MODE 1: + bias, FP32 out [T] (channel 0)
+// MODE 2: masked by (mask[T][COP] > 0), bf16 out [T][COP]
+// ---------------------------------------------------------------------------
+template <int CIP, int COP, int MODE>
+__global__ void __launch_bounds__(256) k_conv(Geo g, const bf16 *__restrict__ in, const bf16 *__restrict__ W,
+                                              const float *__restrict__ bias, bf16 *__restrict__ out_bf,
+                                              float *__restrict__ out_f, const bf16 *__restrict__ mask)
+{
+    constexpr int CH = CIP / 8, NCH = 27 * CH, KS = (NCH + 3) / 4, NT = COP / 16;
+    static_assert(CIP % 8 == 0 && COP % 16 == 0, "padded channel counts");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, grp = lane >> 4;
+    const long long m0 = ((long long)blockIdx.x * 4 + w) * 16;
+    if (m0 >= g.T) return;   // wave-uniform; this kernel has no barriers
+    const long long m = m0 + (lane & 15);
+    const bool mv = m < g.T;
+    int x = 0, y = 0, z = 0;
+    if (mv) {
+        long long r = m;
+        x = (int)(r % g.nx);
+        r /= g.nx;
+        y = (int)(r % g.ny);
+        r /= g.ny;
+        z = (int)(r % g.nz);
+    }
+    f32x4 acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < KS; ++s) {
+        const int c = 4 * s + grp;
+        const bool cv = c < NCH;
+        const int o = cv ? c / CH : 0;
+        const int ci0 = cv ? (c - o * CH) * 8 : 0;
+        bf16x8 a;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) a[t] = (bf16)0.f;
+        if (cv && mv) {
+            const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+            const int xx = x + dx, yy = y + dy, zz = z + dz;
+            if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz) {
+                const long long mn = m + dx + (long long)g.nx * (dy + (long long)g.ny * dz);
+                a = *reinterpret_cast<const bf16x8 *>(in + mn * CIP + ci0);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            bf16x8 b;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) b[t] = (bf16)0.f;
+            if (cv) b = *reinterpret_cast<const bf16x8 *>(W + ((long long)o * COP + nt * 16 + (lane & 15)) * CIP + ci0);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int j = nt * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long long mo = m0 + 4 * grp + r;
+            if (mo >= g.T) continue;
+            float v = acc[nt][r];
+            if constexpr (MODE == 0) {
+                v = fmaxf(v + bias[j], 0.f);
+                out_bf[mo * COP + j] = (bf16)v;
+            } else if constexpr (MODE == 1) {
+                if (j == 0) out_f[mo] = v + bias[0];
+            } else {
+                v = (float)mask[mo * COP + j] > 0.f ? v : 0.f;
+                out_bf[mo * COP + j] = (bf16)v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient partials: part[plane][27][COP][CIT]
+// ---------------------------------------------------------------------------
+template <int CIP, int COP>
+__global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ gz, const bf16 *__restrict__ a,
+                                               float *__restrict__ part)
+{
+    constexpr int NTO = COP / 16, NTI = (CIP + 15) / 16, CIT = NTI * 16, NPAIR = NTO * NTI;
+    constexpr int XW = 32;        // voxels per K step
+    constexpr int XS = XW + 8;    // staged x range (one-voxel halo each side), padded
+    static_assert(NPAIR <= 4, "one tile pair per wave");
+    __shared__ __attribute__((aligned(16))) bf16 gT[COP][XW];
+    __shared__ __attribute__((aligned(16))) bf16 aT[9][CIP][XS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, grp = lane >> 4;
+    const int plane = blockIdx.x;
+    const int b = plane / g.nz, z = plane - b * g.nz;
+    const bool active = w < NPAIR;
+    const int cot = active ? w / NTI : 0, cit = active ? w % NTI : 0;
+    f32x4 acc[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) acc[o] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const long long plane_base = ((long long)b * g.nz + z) * g.ny * g.nx;
+    for (int y = 0; y < g.ny; ++y) {
+        for (int x0 = 0; x0 < g.nx; x0 += XW) {
+            __syncthreads();
+            const long long row = plane_base + (long long)y * g.nx;
+            for (int i = threadIdx.x; i < XW * COP; i += blockDim.x) {
+                const int xi = i / COP, co = i - xi * COP;
+                const int xx = x0 + xi;
+                gT[co][xi] = xx < g.nx ? gz[(row + xx) * COP + co] : (bf16)0.f;
+            }
+            for (int i = threadIdx.x; i < 9 * (XW + 2) * CIP; i += blockDim.x) {
+                const int ci = i % CIP;
+                const int rest = i / CIP;
+                const int xi = rest % (XW + 2), r9 = rest / (XW + 2);
+                const int dz = r9 / 3 - 1, dy = r9 % 3 - 1;
+                const int xx = x0 - 1 + xi, yy = y + dy, zz = z + dz;
+                bf16 v = (bf16)0.f;
+                if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
+                    v = a[((((long long)b * g.nz + zz) * g.ny + yy) * g.nx + xx) * CIP + ci];
+                aT[r9][ci][xi] = v;
+            }
+            __syncthreads();
+            if (active) {
+                const bf16x8 av = *reinterpret_cast<const bf16x8 *>(&gT[cot * 16 + (lane & 15)][8 * grp]);
+                const int ci = cit * 16 + (lane & 15);
+#pragma unroll
+                for (int o = 0; o < 27; ++o) {
+                    const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+                    const int r9 = (dz + 1) * 3 + (dy + 1);
+                    bf16x8 bv;
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) bv[t] = ci < CIP ? aT[r9][ci][1 + dx + 8 * grp + t] : (bf16)0.f;
+                    acc[o] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[o], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int o = 0; o < 27; ++o)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = cot * 16 + 4 * grp + r, ci = cit * 16 + (lane & 15);
+            part[(((long long)plane * 27 + o) * COP + co) * CIT + ci] = acc[o][r];
+        }
+}
+
+// grad[o][co][ci] (packed, real sizes) = sum over planes of the partials
+__global__ void k_wgrad_reduce(const float *__restrict__ part, int nplane, int COP, int CIT, int CO, int CI,
+                               float *__restrict__ grad)
+{
+    const int n = 27 * CO * CI;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int o = i / (CO * CI), rem = i - o * CO * CI, co = rem / CI, ci = rem - co * CI;
+        float s = 0.f;
+        for (int p = 0; p < nplane; ++p) s += part[(((long long)p * 27 + o) * COP + co) * CIT + ci];
+        grad[i] = s;
+    }
+}
+
+// bias gradient partials: bpart[block][COP] = sum over the block's voxels
+template <int COP>
+__global__ void __launch_bounds__(256) k_bias_grad(long long T, const bf16 *__restrict__ gz, float *__restrict__ bpart)
+{
+    constexpr int VPB = 256 / COP;
+    __shared__ float red[256];
+    const int co = threadIdx.x % COP, vi = threadIdx.x / COP;
+    float s = 0.f;
+    for (long long v = (long long)blockIdx.x * VPB + vi; v < T; v += (long long)gridDim.x * VPB)
+        s += (float)gz[v * COP + co];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < COP) {
+        float t = 0.f;
+        for (int k = 0; k < VPB; ++k) t += red[k * COP + threadIdx.x];
+        bpart[(long long)blockIdx.x * COP + threadIdx.x] = t;
+    }
+}
+
+__global__ void k_bias_reduce(const float *__restrict__ bpart, int nblk, int COP, int CO, float *__restrict__ grad)
+{
+    const int co = threadIdx.x;
+    if (co >= CO) return;
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += bpart[(long long)b * COP + co];
+    grad[co] = s;
+}
+
+// x [T][3] float -> [T][8] bf16
+__global__ void k_pack_input(long long T, const float *__restrict__ x, bf16 *__restrict__ xb)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < T * 8; i += (long long)gridDim.x * blockDim.x) {
+        const long long v = i >> 3;
+        const int c = (int)(i & 7);
+        xb[i] = c < 3 ? (bf16)x[v * 3 + c] : (bf16)0.f;
+    }
+}
+
+// master FP32 layer (W[27][CO][CI], b[CO]) -> bf16 forward [27][COP][CIP],
+// bf16 backward-data [27][CIP][COP] (offset flipped), FP32 bias [COP]
+__global__ void k_pack_layer(const float *__restrict__ Wm, const float *__restrict__ bm, int CO, int CI, int COP,
+                             int CIP, bf16 *__restrict__ Wf, bf16 *__restrict__ Wb, float *__restrict__ bias)
+{
+    const int n = 27 * COP * CIP;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int o = i / (COP * CIP), rem = i - o * COP * CIP, co = rem / CIP, ci = rem - co * CIP;
+        const float v = (co < CO && ci < CI) ? Wm[(o * CO + co) * CI + ci] : 0.f;
+        Wf[i] = (bf16)v;
+        Wb[((26 - o) * CIP + ci) * COP + co] = (bf16)v;
+        if (i < COP) bias[i] = i < CO ? bm[i] : 0.f;
+    }
+}
+
+// loss: g[T][16] bf16 (channel 0 = 2 (out - target) / T), lpart[block] = sum diff^2
+__global__ void __launch_bounds__(256) k_loss(long long T, const float *__restrict__ out, const float *__restrict__ tgt,
+                                              bf16 *__restrict__ g, double *__restrict__ lpart)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    const float scale = 2.0f / (float)T;
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < T; v += (long long)gridDim.x * blockDim.x) {
+        const float d = out[v] - tgt[v];
+        s += (double)d * d;
+        g[v * 16] = (bf16)(scale * d);
+#pragma unroll
+        for (int c = 1; c < 16; ++c) g[v * 16 + c] = (bf16)0.f;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+}
+
+__global__ void k_adam(long n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// ---- source-iteration correction -------------------------------------------
+// ravg[e] = sum_i w_i r[e n + i] ; part[block] = sum ravg^2
+__global__ void __launch_bounds__(256) k_elem_avg(long long Ne, int n, const double *__restrict__ w,
+                                                  const double *__restrict__ r, double *__restrict__ ravg,
+                                                  double *__restrict__ part)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < Ne; e += (long long)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int i = 0; i < n; ++i) v = fma(w[i], r[e * n + i], v);
+        ravg[e] = v;
+        s += v * v;
+    }
+    if (!part) return;   // block-uniform
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// scale = sqrt(mean ravg^2) (one block)
+__global__ void k_scale(const double *__restrict__ part, int nb, long long Ne, double *__restrict__ scale)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) scale[0] = sqrt(red[0] / (double)Ne);
+}
+
+// network input [Ne][8] bf16: (ravg / scale, log sigma_t, sigma_s / sigma_t)
+__global__ void k_make_input(long long Ne, const double *__restrict__ ravg, const double *__restrict__ scale,
+                             const double *__restrict__ sigt, const double *__restrict__ sigs, bf16 *__restrict__ xb)
+{
+    const double s = scale[0];
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < Ne; e += (long long)gridDim.x * blockDim.x) {
+        const double st = sigt[e];
+        xb[e * 8 + 0] = (bf16)(float)(ravg[e] * inv);
+        xb[e * 8 + 1] = (bf16)(float)log(st);
+        xb[e * 8 + 2] = (bf16)(float)(sigs[e] / st);
+#pragma unroll
+        for (int c = 3; c < 8; ++c) xb[e * 8 + c] = (bf16)0.f;
+    }
+}
+
+// phi[e n + i] += scale * out[e]
+__global__ void k_apply_corr(long long Ne, int n, const float *__restrict__ out, const double *__restrict__ scale,
+                             double *__restrict__ phi)
+{
+    const double s = scale[0];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < Ne * n; i += (long long)gridDim.x * blockDim.x)
+        phi[i] += s * (double)out[i / n];
+}
+
+// p = 1 (nodes at the vertices): phi[e, i] += scale * (mean of out over the
+// elements that share node i's vertex) -- a continuous trilinear correction
+__global__ void k_apply_corr_vertex(int nz, int ny, int nx, const float *__restrict__ out,
+                                    const double *__restrict__ scale, double *__restrict__ phi)
+{
+    const double s = scale[0];
+    const long long Ne = (long long)nz * ny * nx;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < Ne * 8; i += (long long)gridDim.x * blockDim.x) {
+        const long long e = i >> 3;
+        const int loc = (int)(i & 7);
+        const int x = (int)(e % nx), y = (int)((e / nx) % ny), z = (int)(e / ((long long)nx * ny));
+        const int vx = x + (loc & 1), vy = y + ((loc >> 1) & 1), vz = z + (loc >> 2);
+        double sum = 0.0;
+        int cnt = 0;
+        for (int jz = vz - 1; jz <= vz; ++jz)
+            for (int jy = vy - 1; jy <= vy; ++jy)
+                for (int jx = vx - 1; jx <= vx; ++jx)
+                    if (jx >= 0 && jx < nx && jy >= 0 && jy < ny && jz >= 0 && jz < nz) {
+                        sum += (double)out[((long long)jz * ny + jy) * nx + jx];
+                        ++cnt;
+                    }
+        phi[i] += s * sum / (double)cnt;
+    }
+}
+
+inline int grid_for(long long n, int block = 256, int cap = 4096)
+{
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (int)(g > cap ? cap : g);
+}
+
+thread_local std::string g_cnn_err = "no error";
+int cfail(int code, const std::string &m)
+{
+    g_cnn_err = m;
+    allium_cnn_internal::set_last_error(m);
+    return code;
+}
+#define CNN_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) return cfail(-4, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+long layer_offset(int l)
+{
+    long off = 0;
+    for (int k = 0; k < l; ++k) off += 27L * CO_[k] * CI_[k] + CO_[k];
+    return off;
+}
+const long NPARAM = layer_offset(NL);
+
+}  // namespace
+
+struct allium_cnn {
+    float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    bf16 *Wf[NL] = {}, *Wb[NL] = {};
+    float *bias[NL] = {};
+    bool packed = false;
+    long long cap = 0;
+    int cap_planes = 0;
+    bf16 *xin = nullptr, *act[NL - 1] = {}, *ga = nullptr, *gb = nullptr;
+    float *out = nullptr, *part = nullptr, *bpart = nullptr, *xf = nullptr;
+    double *lpart = nullptr, *ravg = nullptr, *rpart = nullptr, *scale = nullptr;
+    long step = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> owned;
+    ~allium_cnn()
+    {
+        for (void *p : owned) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+    template <typename T>
+    hipError_t alloc(T **p, size_t count)
+    {
+        hipError_t e = hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
+        if (e == hipSuccess) owned.push_back(*p);
+        return e;
+    }
+};
+
+namespace {
+
+constexpr int kBiasBlocks = 256;
+constexpr int kLossBlocks = 512;
+
+int ensure(allium_cnn *h, const Geo &g)
+{
+    const int planes = g.B * g.nz;
+    if (g.T <= h->cap && planes <= h->cap_planes) return 0;
+    // grow: free the old work buffers by re-creating the list
+    std::vector<void *> keep;
+    for (void *p : h->owned) {
+        bool work = p == h->xin || p == h->ga || p == h->gb || p == h->out || p == h->part || p == h->xf ||
+                    p == h->ravg;
+        for (int l = 0; l < NL - 1; ++l) work = work || p == h->act[l];
+        if (work) hipFree(p);
+        else keep.push_back(p);
+    }
+    h->owned = keep;
+    const long long T = g.T;
+    CNN_TRY(h->alloc(&h->xin, (size_t)T * 8));
+    for (int l = 0; l < NL - 1; ++l) CNN_TRY(h->alloc(&h->act[l], (size_t)T * HID));
+    CNN_TRY(h->alloc(&h->ga, (size_t)T * HID));
+    CNN_TRY(h->alloc(&h->gb, (size_t)T * HID));
+    CNN_TRY(h->alloc(&h->out, (size_t)T));
+    CNN_TRY(h->alloc(&h->xf, (size_t)T * 3));
+    CNN_TRY(h->alloc(&h->ravg, (size_t)T));
+    CNN_TRY(h->alloc(&h->part, (size_t)planes * 27 * 32 * 32));
+    h->cap = T;
+    h->cap_planes = planes;
+    return 0;
+}
+
+int pack_weights(allium_cnn *h, hipStream_t s)
+{
+    if (h->packed) return 0;
+    for (int l = 0; l < NL; ++l) {
+        const float *Wm = h->params + layer_offset(l);
+        const float *bm = Wm + 27L * CO_[l] * CI_[l];
+        hipLaunchKernelGGL(k_pack_layer, dim3(grid_for(27L * COP_[l] * CIP_[l])), dim3(256), 0, s, Wm, bm, CO_[l],
+                           CI_[l], COP_[l], CIP_[l], h->Wf[l], h->Wb[l], h->bias[l]);
+    }
+    CNN_TRY(hipGetLastError());
+    h->packed = true;
+    return 0;
+}
+
+// forward from the packed bf16 input h->xin; output FP32 h->out
+int forward_packed(allium_cnn *h, const Geo &g, hipStream_t s)
+{
+    const int grid = (int)((g.T + 63) / 64);
+    hipLaunchKernelGGL((k_conv<8, 32, 0>), dim3(grid), dim3(256), 0, s, g, h->xin, h->Wf[0], h->bias[0], h->act[0],
+                       nullptr, nullptr);
+    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, g, h->act[0], h->Wf[1], h->bias[1], h->act[1],
+                       nullptr, nullptr);
+    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, g, h->act[1], h->Wf[2], h->bias[2], h->act[2],
+                       nullptr, nullptr);
+    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, g, h->act[2], h->Wf[3], h->bias[3], h->act[3],
+                       nullptr, nullptr);
+    hipLaunchKernelGGL((k_conv<32, 16, 1>), dim3(grid), dim3(256), 0, s, g, h->act[3], h->Wf[4], h->bias[4], nullptr,
+                       h->out, nullptr);
+    CNN_TRY(hipGetLastError());
+    return 0;
+}
+
+int check_geo(int B, int nz, int ny, int nx, Geo &g)
+{
+    if (B < 1 || nz < 1 || ny < 1 || nx < 1 || nx > (1 << 20)) return cfail(-1, "bad grid size");
+    g.B = B;
+    g.nz = nz;
+    g.ny = ny;
+    g.nx = nx;
+    g.T = (long long)B * nz * ny * nx;
+    return 0;
+}
+
+// forward + backward into h->grad (device, packed); *loss on the host
+int gradient(allium_cnn *h, const Geo &g, const float *d_x, const float *d_t, double *loss, hipStream_t s)
+{
+    int rc = ensure(h, g);
+    if (rc) return rc;
+    rc = pack_weights(h, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pack_input, dim3(grid_for(g.T * 8)), dim3(256), 0, s, g.T, d_x, h->xin);
+    rc = forward_packed(h, g, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, s, g.T, h->out, d_t, h->ga, h->lpart);
+    const int grid = (int)((g.T + 63) / 64);
+    const int planes = g.B * g.nz;
+    // layer 5: gz = ga [T][16]
+    const bf16 *gz = h->ga;
+    bf16 *gnext = h->gb;
+    for (int l = NL - 1; l >= 0; --l) {
+        const bf16 *ain = l == 0 ? h->xin : h->act[l - 1];
+        float *gW = h->grad + layer_offset(l);
+        float *gB = gW + 27L * CO_[l] * CI_[l];
+        const int CIT = ((CIP_[l] + 15) / 16) * 16;
+        if (l == NL - 1) {
+            hipLaunchKernelGGL((k_wgrad<32, 16>), dim3(planes), dim3(256), 0, s, g, gz, ain, h->part);
+            hipLaunchKernelGGL(k_bias_grad<16>, dim3(kBiasBlocks), dim3(256), 0, s, g.T, gz, h->bpart);
+        } else if (l == 0) {
+            hipLaunchKernelGGL((k_wgrad<8, 32>), dim3(planes), dim3(256), 0, s, g, gz, ain, h->part);
+            hipLaunchKernelGGL(k_bias_grad<32>, dim3(kBiasBlocks), dim3(256), 0, s, g.T, gz, h->bpart);
+        } else {
+            hipLaunchKernelGGL((k_wgrad<32, 32>), dim3(planes), dim3(256), 0, s, g, gz, ain, h->part);
+            hipLaunchKernelGGL(k_bias_grad<32>, dim3(kBiasBlocks), dim3(256), 0, s, g.T, gz, h->bpart);
+        }
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(27L * CO_[l] * CI_[l])), dim3(256), 0, s, h->part, planes,
+                           COP_[l], CIT, CO_[l], CI_[l], gW);
+        hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(64), 0, s, h->bpart, kBiasBlocks, COP_[l], CO_[l], gB);
+        if (l > 0) {
+            // gradient of the previous pre-activation: conv with flipped weights, ReLU mask of its output
+            if (l == NL - 1)
+                hipLaunchKernelGGL((k_conv<16, 32, 2>), dim3(grid), dim3(256), 0, s, g, gz, h->Wb[l], nullptr, gnext,
+                                   nullptr, h->act[l - 1]);
+            else
+                hipLaunchKernelGGL((k_conv<32, 32, 2>), dim3(grid), dim3(256), 0, s, g, gz, h->Wb[l], nullptr, gnext,
+                                   nullptr, h->act[l - 1]);
+            const bf16 *t = gnext;
+            gnext = const_cast<bf16 *>(gz);
+            gz = t;
+        }
+    }
+    CNN_TRY(hipGetLastError());
+    std::vector<double> lp(kLossBlocks);
+    CNN_TRY(hipMemcpyAsync(lp.data(), h->lpart, sizeof(double) * kLossBlocks, hipMemcpyDeviceToHost, s));
+    CNN_TRY(hipStreamSynchronize(s));
+    double sum = 0.0;
+    for (double v : lp) sum += v;
+    if (loss) *loss = sum / (double)g.T;
+    return 0;
+}
+
+}  // namespace
+
+namespace allium_cnn_internal {
+
+hipError_t elem_avg(long long Ne, int n, const double *d_w, const double *d_x, double *d_out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_elem_avg, dim3(grid_for(Ne)), dim3(256), 0, s, Ne, n, d_w, d_x, d_out, nullptr);
+    return hipGetLastError();
+}
+
+int correct(allium_cnn *h, int nz, int ny, int nx, int n, const double *d_w, const double *d_r,
+            const double *d_sigt, const double *d_sigs, double *d_phi, hipStream_t s, int prolong)
+{
+    Geo g;
+    int rc = check_geo(1, nz, ny, nx, g);
+    if (rc) return rc;
+    rc = ensure(h, g);
+    if (rc) return rc;
+    rc = pack_weights(h, s);
+    if (rc) return rc;
+    const int nb = 256;
+    hipLaunchKernelGGL(k_elem_avg, dim3(nb), dim3(256), 0, s, g.T, n, d_w, d_r, h->ravg, h->rpart);
+    hipLaunchKernelGGL(k_scale, dim3(1), dim3(256), 0, s, h->rpart, nb, g.T, h->scale);
+    hipLaunchKernelGGL(k_make_input, dim3(grid_for(g.T)), dim3(256), 0, s, g.T, h->ravg, h->scale, d_sigt, d_sigs,
+                       h->xin);
+    rc = forward_packed(h, g, s);
+    if (rc) return rc;
+    if (prolong == 1 && n == 8)
+        hipLaunchKernelGGL(k_apply_corr_vertex, dim3(grid_for(g.T * 8)), dim3(256), 0, s, nz, ny, nx, h->out, h->scale,
+                           d_phi);
+    else
+        hipLaunchKernelGGL(k_apply_corr, dim3(grid_for(g.T * n)), dim3(256), 0, s, g.T, n, h->out, h->scale, d_phi);
+    CNN_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace allium_cnn_internal
+
+extern "C" {
+
+long allium_cnn_num_params(void) { return NPARAM; }
+
+int allium_cnn_create(unsigned long long seed, allium_cnn_t *out)
+{
+    if (!out) return cfail(-1, "null argument");
+    *out = nullptr;
+    auto *h = new allium_cnn();
+    auto bad = [&](hipError_t e) {
+        delete h;
+        return cfail(-4, std::string("allium_cnn_create: ") + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = h->alloc(&h->params, NPARAM)) != hipSuccess) return bad(e);
+    if ((e = h->alloc(&h->m, NPARAM)) != hipSuccess) return bad(e);
+    if ((e = h->alloc(&h->v, NPARAM)) != hipSuccess) return bad(e);
+    if ((e = h->alloc(&h->grad, NPARAM)) != hipSuccess) return bad(e);
+    for (int l = 0; l < NL; ++l) {
+        if ((e = h->alloc(&h->Wf[l], 27L * COP_[l] * CIP_[l])) != hipSuccess) return bad(e);
+        if ((e = h->alloc(&h->Wb[l], 27L * COP_[l] * CIP_[l])) != hipSuccess) return bad(e);
+        if ((e = h->alloc(&h->bias[l], COP_[l])) != hipSuccess) return bad(e);
+    }
+    if ((e = h->alloc(&h->bpart, (size_t)kBiasBlocks * 32)) != hipSuccess) return bad(e);
+    if ((e = h->alloc(&h->lpart, kLossBlocks)) != hipSuccess) return bad(e);
+    if ((e = h->alloc(&h->rpart, 256)) != hipSuccess) return bad(e);
+    if ((e = h->alloc(&h->scale, 1)) != hipSuccess) return bad(e);
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e);
+    // He-normal weights, zero biases
+    std::vector<float> p(NPARAM, 0.f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < NL; ++l) {
+        std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / (27.0 * CI_[l])));
+        float *W = p.data() + layer_offset(l);
+        for (long i = 0; i < 27L * CO_[l] * CI_[l]; ++i) W[i] = (float)nd(rng);
+    }
+    if ((e = hipMemcpy(h->params, p.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice)) != hipSuccess) return bad(e);
+    if ((e = hipMemset(h->m, 0, sizeof(float) * NPARAM)) != hipSuccess) return bad(e);
+    if ((e = hipMemset(h->v, 0, sizeof(float) * NPARAM)) != hipSuccess) return bad(e);
+    *out = h;
+    return 0;
+}
+
+void allium_cnn_destroy(allium_cnn_t h) { delete h; }
+
+int allium_cnn_get_params(allium_cnn_t h, float *params)
+{
+    if (!h || !params) return cfail(-1, "null argument");
+    CNN_TRY(hipMemcpy(params, h->params, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int allium_cnn_set_params(allium_cnn_t h, const float *params)
+{
+    if (!h || !params) return cfail(-1, "null argument");
+    for (long i = 0; i < NPARAM; ++i)
+        if (!std::isfinite(params[i])) return cfail(-6, "non-finite network parameter");
+    CNN_TRY(hipMemcpy(h->params, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+    CNN_TRY(hipMemset(h->m, 0, sizeof(float) * NPARAM));
+    CNN_TRY(hipMemset(h->v, 0, sizeof(float) * NPARAM));
+    h->step = 0;
+    h->packed = false;
+    return 0;
+}
+
+int allium_cnn_forward_device(allium_cnn_t h, int batch, int nz, int ny, int nx, const float *d_x, float *d_out,
+                              void *stream)
+{
+    if (!h || !d_x || !d_out) return cfail(-1, "null argument");
+    Geo g;
+    int rc = check_geo(batch, nz, ny, nx, g);
+    if (rc) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    rc = ensure(h, g);
+    if (rc) return rc;
+    rc = pack_weights(h, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pack_input, dim3(grid_for(g.T * 8)), dim3(256), 0, s, g.T, d_x, h->xin);
+    rc = forward_packed(h, g, s);
+    if (rc) return rc;
+    CNN_TRY(hipMemcpyAsync(d_out, h->out, sizeof(float) * g.T, hipMemcpyDeviceToDevice, s));
+    CNN_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int allium_cnn_gradient_device(allium_cnn_t h, int batch, int nz, int ny, int nx, const float *d_x,
+                               const float *d_target, float *grad, double *loss)
+{
+    if (!h || !d_x || !d_target || !grad) return cfail(-1, "null argument");
+    Geo g;
+    int rc = check_geo(batch, nz, ny, nx, g);
+    if (rc) return rc;
+    rc = gradient(h, g, d_x, d_target, loss, h->stream);
+    if (rc) return rc;
+    CNN_TRY(hipMemcpy(grad, h->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int allium_cnn_train_step_device(allium_cnn_t h, int batch, int nz, int ny, int nx, const float *d_x,
+                                 const float *d_target, float lr, double *loss)
+{
+    if (!h || !d_x || !d_target) return cfail(-1, "null argument");
+    if (!(lr > 0.f) || !std::isfinite(lr)) return cfail(-1, "learning rate must be positive");
+    Geo g;
+    int rc = check_geo(batch, nz, ny, nx, g);
+    if (rc) return rc;
+    double l = 0.0;
+    rc = gradient(h, g, d_x, d_target, &l, h->stream);
+    if (rc) return rc;
+    if (!std::isfinite(l)) return cfail(-6, "non-finite training loss");
+    h->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)h->step), c2 = 1.f - std::pow(b2, (float)h->step);
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(NPARAM)), dim3(256), 0, h->stream, NPARAM, h->params, h->grad, h->m,
+                       h->v, lr, b1, b2, 1e-8f, c1, c2);
+    CNN_TRY(hipGetLastError());
+    h->packed = false;
+    CNN_TRY(hipStreamSynchronize(h->stream));
+    if (loss) *loss = l;
+    return 0;
+}
+
+}  // extern "C"

@@ -161,10 +161,11 @@ hipError_t launch_moments_T(const SweepParams &prm, double inv_wsum, const doubl
                             hipStream_t s);
 hipError_t launch_mass(const SweepParams &prm, const double *Mm, const double *coef, double scale,
                        const double *x, double *y, hipStream_t s);
-// SI update: r = b - (x - y); x += r; partial[blk] = sum r^2 ; returns blocks used
+// SI update: r = b - (x - y) (y null: r = b - x); x += r; partial[blk] = sum r^2;
+// rout (may be null) receives r
 int si_update_blocks(long n);
 hipError_t launch_si_update(long n, const double *b, const double *y, double *x, double *partial,
-                            hipStream_t s);
+                            hipStream_t s, double *rout = nullptr);
 // plain vector ops
 hipError_t launch_axpby(long n, double a, const double *x, double b, const double *y, double *z,
                         hipStream_t s);

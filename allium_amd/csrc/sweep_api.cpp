@@ -13,6 +13,7 @@
 
 #include "../../include/allium_device.h"
 #include "../../include/allium_sweep.h"
+#include "cnn_internal.hpp"
 #include "device.hpp"
 #include "discretization.hpp"
 
@@ -62,6 +63,8 @@ struct DevBuf {
 };
 }  // namespace
 
+void allium_cnn_internal::set_last_error(const std::string &msg) { g_last_error = msg; }
+
 struct allium_sweep {
     allium::Mesh mesh;
     allium::Quadrature quad;
@@ -75,7 +78,7 @@ struct allium_sweep {
     // matrices kept for host-side rebuilds
     std::vector<double> Kst;
     // scratch for host-pointer entry points and the source iteration
-    DevBuf a, b, c, x, y, iso, coef, part, q;
+    DevBuf a, b, c, x, y, iso, coef, part, q, r, wavg, havg;
     hipStream_t stream = nullptr;
     ~allium_sweep()
     {
@@ -834,6 +837,69 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
     std::vector<double> hp(nb);
     double r0 = 0.0;
     int it = 0;
+    // learned correction and the per-iteration history both use element averages
+    allium_cnn *cnn = nullptr;
+    if (opt->accel == 1) {
+        if (!opt->accel_handle) return fail(ALLIUM_ERR_ARG, "accel 1 needs an allium_cnn_t in accel_handle");
+        if (h->mesh.dim != 3 || !h->mesh.uniform || h->G != 1)
+            return fail(ALLIUM_ERR_UNSUPPORTED, "the learned correction needs a uniform 3D mesh and one group");
+        cnn = static_cast<allium_cnn *>(opt->accel_handle);
+    } else if (opt->accel != 0) {
+        return fail(ALLIUM_ERR_ARG, "unknown acceleration");
+    }
+    if (opt->history && h->G != 1) return fail(ALLIUM_ERR_UNSUPPORTED, "iteration history needs one group");
+    const int n = h->ed.n;
+    const long Ne = h->mesh.Ne;
+    if (cnn || opt->history) {
+        // element average = sum_i w_i phi_i with w_i = (row sum of the element mass) / volume
+        std::vector<double> w(n, 0.0);
+        double vol = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) { w[i] += h->ed.Mu[(size_t)i * n + j]; vol += h->ed.Mu[(size_t)i * n + j]; }
+        for (double &v : w) v /= vol;
+        HIP_TRY(h->wavg.ensure(sizeof(double) * n));
+        HIP_TRY(hipMemcpy(h->wavg.p, w.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    if (cnn) HIP_TRY(h->r.ensure(sizeof(double) * np));
+    if (cnn) HIP_TRY(hipMemsetAsync(h->r.p, 0, sizeof(double) * np, s));
+    if (opt->history) HIP_TRY(h->havg.ensure(sizeof(double) * Ne));
+    if (cnn && (opt->accel_prolong < 0 || opt->accel_prolong > 1 || (opt->accel_prolong == 1 && n != 8)))
+        return fail(ALLIUM_ERR_ARG, "accel_prolong must be 0, or 1 with p = 1");
+    bool cnn_on = cnn != nullptr;
+    int fails = 0;
+    const int patience = std::max(1, opt->accel_patience);
+    double prev_nrm = 0.0;
+    // after x += r: apply the learned correction (until the residual has
+    // failed to drop `patience` times in a row) and record the history
+    auto after_update = [&](double nrm) -> int {
+        if (cnn_on) {
+            fails = (it > 1 && !(nrm < prev_nrm)) ? fails + 1 : 0;
+            if (fails >= patience) {
+                cnn_on = false;
+            } else {
+                int rc2 = allium_cnn_internal::correct(cnn, h->mesh.nel[2], h->mesh.nel[1], h->mesh.nel[0], n,
+                                                       h->wavg.d(), h->r.d(), h->d_sigma, h->coef.d(), h->x.d(), s,
+                                                       opt->accel_prolong);
+                if (rc2) return rc2;
+            }
+        }
+        prev_nrm = nrm;
+        if (opt->history) {
+            double *hrow = opt->history + (size_t)(it - 1) * 2 * Ne;
+            HIP_TRY(allium_cnn_internal::elem_avg(Ne, n, h->wavg.d(), h->x.d(), h->havg.d(), s));
+            HIP_TRY(hipMemcpyAsync(hrow, h->havg.p, sizeof(double) * Ne, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (cnn) {
+                HIP_TRY(allium_cnn_internal::elem_avg(Ne, n, h->wavg.d(), h->r.d(), h->havg.d(), s));
+                HIP_TRY(hipMemcpyAsync(hrow + Ne, h->havg.p, sizeof(double) * Ne, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+            } else {
+                const double *prev = it > 1 ? hrow - 2 * Ne : nullptr;
+                for (long e = 0; e < Ne; e++) hrow[Ne + e] = hrow[e] - (prev ? prev[e] : 0.0);
+            }
+        }
+        return ALLIUM_OK;
+    };
     if (opt->precision == ALLIUM_FP32_REFINED) {
         // defect correction: psi += L32^{-1} (q + D^T Ms phi - L psi), residuals in FP64
         HIP_TRY(h->c.ensure(sizeof(double) * nps));
@@ -849,7 +915,8 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
             HIP_TRY(allium::launch_axpby(nps, 1.0, h->a.d(), 1.0, h->c.d(), h->a.d(), s));
             HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->y.d(), s));
             // r = D psi_{k+1} - phi_k ; phi_{k+1} = D psi_{k+1}
-            HIP_TRY(allium::launch_si_update(np, h->y.d(), nullptr, h->x.d(), h->part.d(), s));
+            HIP_TRY(allium::launch_si_update(np, h->y.d(), nullptr, h->x.d(), h->part.d(), s,
+                                             cnn ? h->r.d() : nullptr));
             HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             double nrm = 0.0;
@@ -859,6 +926,8 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
             if (!std::isfinite(nrm)) { rc = fail(ALLIUM_ERR_NUMERIC, "non-finite residual"); break; }
             if (it == 1) r0 = std::max(nrm * opt->reltol, opt->abstol);
             if (nrm < r0 || it >= opt->maxit) break;
+            rc = after_update(nrm);
+            if (rc) break;
         }
         h->ds.precision = prec_save;
         if (rc) return rc;
@@ -875,7 +944,8 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
             rc = do_sweep(h, nullptr, h->iso.d(), h->a.d(), s);
             if (rc) break;
             HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->y.d(), s));
-            HIP_TRY(allium::launch_si_update(np, h->b.d(), h->y.d(), h->x.d(), h->part.d(), s));
+            HIP_TRY(allium::launch_si_update(np, h->b.d(), h->y.d(), h->x.d(), h->part.d(), s,
+                                             cnn ? h->r.d() : nullptr));
             HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             double nrm = 0.0;
@@ -885,6 +955,8 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
             if (!std::isfinite(nrm)) { rc = fail(ALLIUM_ERR_NUMERIC, "non-finite residual"); break; }
             if (it == 1) r0 = std::max(nrm * opt->reltol, opt->abstol);
             if (nrm < r0 || it >= opt->maxit) break;
+            rc = after_update(nrm);
+            if (rc) break;
         }
         if (rc) { h->ds.precision = prec_save; return rc; }
         // final sweep: psi = L^{-1}(D^T Ms phi + q)

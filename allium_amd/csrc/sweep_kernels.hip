@@ -1040,13 +1040,14 @@ int si_update_blocks(long n) { return (int)std::min<long>(2048, (n + kUpdBlock -
 
 __global__ void __launch_bounds__(kUpdBlock) k_si_update(long long n, const double *__restrict__ b,
                                                          const double *__restrict__ y, double *__restrict__ x,
-                                                         double *__restrict__ partial)
+                                                         double *__restrict__ partial, double *__restrict__ rout)
 {
     __shared__ double sh[kUpdBlock];
     double acc = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const double r = y ? b[i] - (x[i] - y[i]) : b[i] - x[i];
         x[i] += r;
+        if (rout) rout[i] = r;
         acc = fma(r, r, acc);
     }
     sh[threadIdx.x] = acc;
@@ -1058,9 +1059,11 @@ __global__ void __launch_bounds__(kUpdBlock) k_si_update(long long n, const doub
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-hipError_t launch_si_update(long n, const double *b, const double *y, double *x, double *partial, hipStream_t s)
+hipError_t launch_si_update(long n, const double *b, const double *y, double *x, double *partial, hipStream_t s,
+                            double *rout)
 {
-    hipLaunchKernelGGL(k_si_update, dim3(si_update_blocks(n)), dim3(kUpdBlock), 0, s, (long long)n, b, y, x, partial);
+    hipLaunchKernelGGL(k_si_update, dim3(si_update_blocks(n)), dim3(kUpdBlock), 0, s, (long long)n, b, y, x, partial,
+                       rout);
     return hipGetLastError();
 }
 

@@ -69,4 +69,4 @@ def problem_3d_hetero(n=64, seed=0, log_sd=1.0, c_range=(0.5, 0.99), c_const=Non
     sig = lognormal_sigma(ne, log_sd, rng)
     c = np.full(ne, c_const) if c_const is not None else rng.uniform(c_range[0], c_range[1], ne)
     return dict(dim=3, nel=nel, vertices=cartesian_vertices(nel), p=p, sn=sn,
-                sigma_t=sig.reshape(1, ne), sigma_s=c * sig, Q=1.0, seed=seed)
+                sigma_t=sig.reshape(1, ne), sigma_s=c * sig, c=c, Q=1.0, seed=seed)

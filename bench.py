@@ -111,9 +111,14 @@ def measure_traffic(args, tuning):
     return total, detail
 
 
-def si_run(n, seed, maxit):
-    """Plain GPU source iteration to a 1e-8 FP64 residual (c = 0.9999, log-sd 1.5)."""
+def si_run(n, seed, maxit, learned=True):
+    """Wall time to a 1e-8 FP64 scalar-flux residual on the 64^3 p=1 S8 c=0.9999
+    medium (lognormal sigma_t, log-sd 1.5, Q=1): plain GPU source iteration
+    against source iteration with the learned correction (FP64 and
+    FP32-refined sweeps).  The network is trained first on the GPU from
+    seeded 32^3 media (not part of the timed solves; its time is reported)."""
     import allium_amd as A
+    from allium_amd import cnn
     from allium_amd import problems as P
 
     cfg = P.problem_3d_hetero(n=n, seed=seed, log_sd=1.5, c_const=0.9999)
@@ -121,12 +126,27 @@ def si_run(n, seed, maxit):
     Linv = A.InverseAdvectionOperator(m, 1, 8, cfg["sigma_t"])
     Linv.set_launch(-1)
     q = Linv.form_source(Q=1.0)
-    A.synchronize()
-    t = time.perf_counter()
-    phi, psi, it, norms = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=maxit, want_psi=False)
-    wall = time.perf_counter() - t
-    return {"iterations": it, "final_residual": float(norms[-1]), "wall_s": wall,
-            "converged": bool(norms[-1] < 1e-8)}
+
+    def solve(**kw):
+        A.synchronize()
+        t = time.perf_counter()
+        phi, _, it, norms = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=maxit, want_psi=False, **kw)
+        wall = time.perf_counter() - t
+        return phi, {"iterations": it, "final_residual": float(norms[-1]), "wall_s": wall,
+                     "converged": bool(norms[-1] < 1e-8)}
+
+    phi0, plain = solve()
+    out = {"plain": plain}
+    if not learned:
+        return out
+    net, info = cnn.fit(n=32, media=60, rounds=2, steps=1500, seed0=1000, sn=8, log=log)
+    out["training"] = info
+    for name, prec in (("learned_fp64", A.FP64), ("learned_fp32_refined", A.FP32_REFINED)):
+        phi1, res = solve(accel=net, precision=prec)
+        res["rel_diff_vs_plain"] = float(np.linalg.norm(phi1 - phi0) / np.linalg.norm(phi0))
+        res["speedup_vs_plain"] = plain["wall_s"] / res["wall_s"]
+        out[name] = res
+    return out
 
 
 def main():
