@@ -56,6 +56,7 @@ SIGNATURES = {
     "allium_sweep_get_box_tiling": (ctypes.c_int, [c_vp, c_ip, c_ip, c_ip, c_ip, c_ip, c_ip]),
     "allium_sweep_mult": (ctypes.c_int, [c_vp, c_dp, c_dp]),
     "allium_sweep_mult_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "allium_sweep_mult_iso_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "allium_sweep_levels": (ctypes.c_int, [c_vp, c_ip]),
     "allium_apply_L_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "allium_apply_L": (ctypes.c_int, [c_vp, c_dp, c_dp]),
@@ -87,6 +88,16 @@ SIGNATURES = {
                                                   c_vp, ctypes.POINTER(ctypes.c_float), c_dp]),
     "allium_cnn_train_step_device": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                     c_vp, c_vp, ctypes.c_float, c_dp]),
+    # allium_trt.h (the descriptor/info structs are declared in allium_amd/trt.py)
+    "allium_trt_create": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc), c_vp,
+                                         ctypes.POINTER(c_vp)]),
+    "allium_trt_destroy": (None, [c_vp]),
+    "allium_trt_step": (ctypes.c_int, [c_vp, c_vp]),
+    "allium_trt_temperature": (ctypes.c_int, [c_vp, c_dp]),
+    "allium_trt_scalar_flux": (ctypes.c_int, [c_vp, c_dp]),
+    "allium_trt_sweep": (c_vp, [c_vp]),
+    "allium_planck_integrals": (ctypes.c_int, [ctypes.c_double, c_dp, c_dp]),
+    "allium_log_groups": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int, c_dp, c_dp]),
     # allium_device.h
     "allium_device_count": (ctypes.c_int, [c_ip]),
     "allium_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),

@@ -392,6 +392,27 @@ bool build_mesh(int dim, const int *nel, const double *vertices, const double *e
 // MGMassIntegrator src/mg_form.cpp:4-65; FaceMassMatricesIntegrator
 // src/sweep.cpp:1187-1244; time mass src/sweep.cpp:1003-1022)
 // ---------------------------------------------------------------------------
+void uniform_element_rule(const Mesh &m, int p, int order, std::vector<double> &phi, std::vector<double> &w)
+{
+    const int dim = m.dim;
+    Basis B;
+    B.dim = dim;
+    B.p = p;
+    B.nodes = gll_nodes(p);
+    B.n = 1;
+    for (int d = 0; d < dim; d++) B.n *= p + 1;
+    const TensorRule r = tensor_rule(dim, gauss(npts(order)));
+    double vol = 1.0;
+    for (int d = 0; d < dim; d++) vol *= m.h[d];
+    phi.assign(r.w.size() * B.n, 0.0);
+    w.assign(r.w.size(), 0.0);
+    std::vector<double> dphi(3 * B.n);
+    for (size_t q = 0; q < r.w.size(); q++) {
+        B.eval(r.x[q].data(), &phi[q * B.n], dphi.data());
+        w[q] = r.w[q] * vol;
+    }
+}
+
 bool assemble(const Mesh &m, int p, int lump, ElementData &ed, std::string &err)
 {
     if (p < 0 || p > 2) { err = "fe_order must be 0, 1 or 2"; return false; }

@@ -60,6 +60,9 @@ struct ElementData {
     std::vector<int> side_nodes;  // 2*dim x nf local node ids of each side, face-lexicographic
 };
 bool assemble(const Mesh &m, int p, int lump, ElementData &ed, std::string &err);
+// Gauss points of mfem::IntRules.Get(cube/square, order) on one uniform element:
+// phi[q * n + i] = basis function i at point q, w[q] = weight times element volume
+void uniform_element_rule(const Mesh &m, int p, int order, std::vector<double> &phi, std::vector<double> &w);
 
 // Sweep plan: level of every node e*Nomega+a and the cells grouped by level.
 struct SweepPlan {

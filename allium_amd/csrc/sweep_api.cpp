@@ -574,6 +574,13 @@ int allium_sweep_mult_device(allium_sweep_t h, const double *d_source, double *d
     return do_sweep(h, d_source, nullptr, d_psi, (hipStream_t)stream);
 }
 
+int allium_sweep_mult_iso_device(allium_sweep_t h, const double *d_source, const double *d_iso, double *d_psi,
+                                 void *stream)
+{
+    if (!h || !d_iso || !d_psi) return fail(ALLIUM_ERR_ARG, "null argument");
+    return do_sweep(h, d_source, d_iso, d_psi, (hipStream_t)stream);
+}
+
 int allium_sweep_mult(allium_sweep_t h, const double *source, double *psi)
 {
     if (!h || !source || !psi) return fail(ALLIUM_ERR_ARG, "null argument");

@@ -1,0 +1,605 @@
+// Multigroup linearized backward-Euler thermal radiative transfer on the
+// MI355X sweep: element-local emission/energy-balance kernels and the
+// time-step driver behind include/allium_trt.h.
+//
+// One step restates LinearizedTRTOperator::Mult (src/trt_linearized.cpp:230-312):
+//   resid = (c_v/dt) M T^n - Bt(T);  Bt' = (c_v/dt) M + dGray(T);  t1 = Bt'^{-1} resid
+//   em = B(T) + dB t1;  phi_src = D Linv(D^T em + q_psi)
+//   (I - D Linv D^T Ms) phi = phi_src,  Ms = dB Bt'^{-1} M_sigma   (source iteration)
+//   em += Ms phi;  psi = Linv(D^T em + q_psi)
+//   T += Bt'^{-1}(M_sigma phi + resid), floored at 1e-8
+// with q_psi = inflow + (1/dt) Mt psi^n and dimensionless black-body emission T^4.
+// Every element-local operator is one thread per element over the tensor
+// Gauss rules the reference's integrators use (order 2p+1 for the group
+// emission, 2p+2 for the gray energy balance).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/allium_device.h"
+#include "../../include/allium_trt.h"
+#include "cnn_internal.hpp"
+#include "device.hpp"
+#include "discretization.hpp"
+#include "planck.hpp"
+
+namespace {
+
+constexpr int N = 8;       // p = 1 hexahedra
+constexpr int GMAX = 64;   // groups a thread keeps spectra for
+
+struct TrtParams {
+    int Ne, G, nq3, nq4;
+    double cvdt;               // c_v / dt
+    const double *phi3, *w3;   // emission rule: basis values nq3 x N, weights
+    const double *phi4, *w4;   // energy-balance rule
+    const double *Mm;          // MassIntegrator-rule element mass (c_v, time mass)
+    const double *Mu;          // MG mass (absorption)
+    const double *bounds;      // G + 1
+    const double *sigma;       // G x Ne
+};
+
+__device__ inline double dot_basis(const double *phi_q, const double *x)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) v = fma(phi_q[j], x[j], v);
+    return v;
+}
+
+// Gauss-Jordan inverse with partial pivoting of an N x N matrix (in place into inv)
+__device__ inline bool invert(double (&A)[N][N], double (&inv)[N][N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) inv[i][j] = i == j ? 1.0 : 0.0;
+    for (int k = 0; k < N; ++k) {
+        int piv = k;
+        double best = fabs(A[k][k]);
+        for (int i = k + 1; i < N; ++i)
+            if (fabs(A[i][k]) > best) { best = fabs(A[i][k]); piv = i; }
+        if (!(best > 0.0)) return false;
+        if (piv != k)
+            for (int j = 0; j < N; ++j) {
+                double t = A[k][j]; A[k][j] = A[piv][j]; A[piv][j] = t;
+                t = inv[k][j]; inv[k][j] = inv[piv][j]; inv[piv][j] = t;
+            }
+        const double r = 1.0 / A[k][k];
+        for (int j = 0; j < N; ++j) { A[k][j] *= r; inv[k][j] *= r; }
+        for (int i = 0; i < N; ++i) {
+            if (i == k) continue;
+            const double f = A[i][k];
+            if (f == 0.0) continue;
+            for (int j = 0; j < N; ++j) { A[i][j] -= f * A[k][j]; inv[i][j] -= f * inv[k][j]; }
+        }
+    }
+    return true;
+}
+
+// sigma[g][e] = coef / (mid_g^3 sqrt(T at the element centre))
+__global__ void k_opacity(int Ne, int G, const double *__restrict__ T, const double *__restrict__ mid, double coef,
+                          double *__restrict__ sigma)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Ne) return;
+    double tc = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) tc += T[(long long)e * N + i];
+    tc *= 1.0 / N;   // trilinear value at the centre
+    const double rs = 1.0 / sqrt(tc);
+    for (int g = 0; g < G; ++g) sigma[(long long)g * Ne + e] = coef * rs / (mid[g] * mid[g] * mid[g]);
+}
+
+// per element: resid, Bt'^{-1}, group emission em (phi-sized) and the
+// emission-gradient weights dpw[e][q][g] = w_q sigma_g 4 T_q^3 r_g(T_q)
+__global__ void __launch_bounds__(64) k_linearize(TrtParams P, const double *__restrict__ T,
+                                                  const double *__restrict__ Tn, double *__restrict__ resid,
+                                                  double *__restrict__ Binv, double *__restrict__ em,
+                                                  double *__restrict__ dpw, int *__restrict__ bad)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P.Ne) return;
+    const double *Te = T + (long long)e * N;
+    double tv[N], A[N][N], spec[GMAX];
+#pragma unroll
+    for (int i = 0; i < N; ++i) tv[i] = Te[i];
+    // (c_v/dt) M and resid = (c_v/dt) M (T^n - T)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double r = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            A[i][j] = P.cvdt * P.Mm[i * N + j];
+            r = fma(A[i][j], Tn[(long long)e * N + j] - tv[j], r);
+        }
+        resid[(long long)e * N + i] = r;
+    }
+    // gray emission and its gradient (Rosseland spectrum), rule 4
+    for (int q = 0; q < P.nq4; ++q) {
+        const double *ph = P.phi4 + q * N;
+        const double Tq = dot_basis(ph, tv);
+        double sp = 0.0, sr = 0.0;
+        allium::planck_spectrum<0>(P.bounds, P.G, Tq, spec);
+        for (int g = 0; g < P.G; ++g) sp = fma(P.sigma[(long long)g * P.Ne + e], spec[g], sp);
+        allium::planck_spectrum<1>(P.bounds, P.G, Tq, spec);
+        for (int g = 0; g < P.G; ++g) sr = fma(P.sigma[(long long)g * P.Ne + e], spec[g], sr);
+        const double T3 = Tq * Tq * Tq;
+        const double we = P.w4[q] * sp * T3 * Tq;      // emission
+        const double wg = P.w4[q] * 4.0 * sr * T3;     // its derivative
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            resid[(long long)e * N + i] -= we * ph[i];
+#pragma unroll
+            for (int j = 0; j < N; ++j) A[i][j] = fma(wg * ph[i], ph[j], A[i][j]);
+        }
+    }
+    double inv[N][N];
+    if (!invert(A, inv)) atomicAdd(bad, 1);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) Binv[((long long)e * N + i) * N + j] = inv[i][j];
+    // group emission and gradient weights, rule 3
+    for (int i = 0; i < N; ++i)
+        for (int g = 0; g < P.G; ++g) em[((long long)g * P.Ne + e) * N + i] = 0.0;
+    for (int q = 0; q < P.nq3; ++q) {
+        const double *ph = P.phi3 + q * N;
+        const double Tq = dot_basis(ph, tv);
+        const double T3 = Tq * Tq * Tq;
+        allium::planck_spectrum<0>(P.bounds, P.G, Tq, spec);
+        for (int g = 0; g < P.G; ++g) {
+            const double s = P.w3[q] * P.sigma[(long long)g * P.Ne + e] * spec[g] * T3 * Tq;
+            double *eg = em + ((long long)g * P.Ne + e) * N;
+            for (int i = 0; i < N; ++i) eg[i] = fma(s, ph[i], eg[i]);
+        }
+        allium::planck_spectrum<1>(P.bounds, P.G, Tq, spec);
+        for (int g = 0; g < P.G; ++g)
+            dpw[((long long)e * P.nq3 + q) * P.G + g] = P.w3[q] * P.sigma[(long long)g * P.Ne + e] * 4.0 * spec[g] * T3;
+    }
+}
+
+// out[g][e] (+)= scale * dB_g u_e with u_e = Binv_e (src_e), src in T space
+// mode 0: src = u directly; mode 1: src = sum_g sigma_g Mu phi_g (Ms phi)
+__global__ void __launch_bounds__(64) k_apply_dB(TrtParams P, int mode, const double *__restrict__ x,
+                                                 const double *__restrict__ Binv, const double *__restrict__ dpw,
+                                                 double scale, int accumulate, double *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P.Ne) return;
+    double t[N], u[N];
+    if (mode == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) t[i] = x[(long long)e * N + i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) t[i] = 0.0;
+        for (int g = 0; g < P.G; ++g) {
+            const double sg = P.sigma[(long long)g * P.Ne + e];
+            const double *xg = x + ((long long)g * P.Ne + e) * N;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < N; ++j) a = fma(P.Mu[i * N + j], xg[j], a);
+                t[i] = fma(sg, a, t[i]);
+            }
+        }
+    }
+    const double *B = Binv + (long long)e * N * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) u[i] = dot_basis(B + i * N, t);
+    double uq[32];
+    for (int q = 0; q < P.nq3; ++q) uq[q] = dot_basis(P.phi3 + q * N, u);
+    for (int g = 0; g < P.G; ++g) {
+        double *og = out + ((long long)g * P.Ne + e) * N;
+        double v[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = 0.0;
+        for (int q = 0; q < P.nq3; ++q) {
+            const double s = dpw[((long long)e * P.nq3 + q) * P.G + g] * uq[q];
+            const double *ph = P.phi3 + q * N;
+#pragma unroll
+            for (int i = 0; i < N; ++i) v[i] = fma(s, ph[i], v[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) og[i] = accumulate ? fma(scale, v[i], og[i]) : scale * v[i];
+    }
+}
+
+// T += Binv (sum_g sigma_g Mu phi_g + resid), floored at 1e-8
+__global__ void k_update_T(TrtParams P, const double *__restrict__ phi, const double *__restrict__ resid,
+                           const double *__restrict__ Binv, double *__restrict__ T, int *__restrict__ floors)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P.Ne) return;
+    double t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = resid[(long long)e * N + i];
+    for (int g = 0; g < P.G; ++g) {
+        const double sg = P.sigma[(long long)g * P.Ne + e];
+        const double *xg = phi + ((long long)g * P.Ne + e) * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) a = fma(P.Mu[i * N + j], xg[j], a);
+            t[i] = fma(sg, a, t[i]);
+        }
+    }
+    const double *B = Binv + (long long)e * N * N;
+    int fl = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double v = T[(long long)e * N + i] + dot_basis(B + i * N, t);
+        if (!(v >= 1e-8)) { v = 1e-8; fl++; }
+        T[(long long)e * N + i] = v;
+    }
+    if (fl) atomicAdd(floors, fl);
+}
+
+// q_psi = inflow + (1/dt) Mt psi^n, one thread per (cell, dof)
+__global__ void k_time_source(long long cells, double tau, const double *__restrict__ Mm,
+                              const double *__restrict__ qin, const double *__restrict__ psin, double *__restrict__ out)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells * N) return;
+    const int i = (int)(t % N);
+    const double *x = psin + (t - i);
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) a = fma(Mm[i * N + j], x[j], a);
+    out[t] = fma(tau, a, qin ? qin[t] : 0.0);
+}
+
+__global__ void k_minmax(long long n, const double *__restrict__ x, double *__restrict__ part)
+{
+    __shared__ double lo[256], hi[256];
+    double a = 1e300, b = -1e300;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        a = fmin(a, x[i]);
+        b = fmax(b, x[i]);
+    }
+    lo[threadIdx.x] = a;
+    hi[threadIdx.x] = b;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + k]);
+            hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + k]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = lo[0]; part[2 * blockIdx.x + 1] = hi[0]; }
+}
+
+thread_local std::string g_err;
+int tfail(int code, const std::string &m)
+{
+    allium_cnn_internal::set_last_error(m);
+    return code;
+}
+#define TRT_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess) return tfail(ALLIUM_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+#define ABI_TRY(expr)                \
+    do {                             \
+        int _rc = (expr);            \
+        if (_rc) return _rc;         \
+    } while (0)
+
+}  // namespace
+
+struct allium_trt {
+    allium_sweep_t L = nullptr;
+    allium_trt_desc d{};
+    int G = 0, No = 0, Ne = 0, nel[3] = {0, 0, 0};
+    long Ndof = 0, psi_size = 0, phi_size = 0;
+    double wsum = 0.0, time = 0.0;
+    int step = 0;
+    std::vector<double> bounds, mid, sigma_h;
+    TrtParams P{};
+    std::vector<void *> owned;
+    double *T = nullptr, *Tn = nullptr, *psi = nullptr, *psin = nullptr, *qin = nullptr, *qpsi = nullptr;
+    double *sigma = nullptr, *em = nullptr, *phi = nullptr, *phisrc = nullptr, *phit = nullptr, *iso = nullptr;
+    double *resid = nullptr, *Binv = nullptr, *dpw = nullptr, *part = nullptr, *dmid = nullptr;
+    int *flags = nullptr;
+    hipStream_t s = nullptr;
+    ~allium_trt()
+    {
+        if (L) allium_sweep_destroy(L);
+        for (void *p : owned) hipFree(p);
+        if (s) hipStreamDestroy(s);
+    }
+    template <typename T>
+    hipError_t alloc(T **p, size_t count, const T *src = nullptr)
+    {
+        hipError_t e = hipMalloc((void **)p, sizeof(T) * std::max<size_t>(count, 1));
+        if (e != hipSuccess) return e;
+        owned.push_back(*p);
+        if (src) return hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyHostToDevice);
+        return hipSuccess;
+    }
+};
+
+namespace {
+
+constexpr int kPart = 1024;
+
+int update_opacity(allium_trt *h)
+{
+    hipLaunchKernelGGL(k_opacity, dim3((h->Ne + 255) / 256), dim3(256), 0, h->s, h->Ne, h->G, h->T, h->dmid,
+                       h->d.opacity_coef, h->sigma);
+    TRT_TRY(hipGetLastError());
+    TRT_TRY(hipMemcpyAsync(h->sigma_h.data(), h->sigma, sizeof(double) * h->sigma_h.size(), hipMemcpyDeviceToHost,
+                           h->s));
+    TRT_TRY(hipStreamSynchronize(h->s));
+    for (double v : h->sigma_h)
+        if (!std::isfinite(v)) return tfail(ALLIUM_ERR_NUMERIC, "non-finite opacity");
+    ABI_TRY(allium_sweep_set_sigma(h->L, h->sigma_h.data()));
+    return ALLIUM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int allium_planck_integrals(double x, double *planck, double *rosseland)
+{
+    if (!(x >= 0.0)) return tfail(ALLIUM_ERR_ARG, "Planck integral needs x >= 0");
+    if (planck) *planck = allium::planck_integral(x);
+    if (rosseland) *rosseland = allium::rosseland_integral(x);
+    return ALLIUM_OK;
+}
+
+int allium_log_groups(double emin, double emax, int G, double *bounds, double *midpoints)
+{
+    if (!(emin > 0.0) || !(emax > emin) || G < 1) return tfail(ALLIUM_ERR_ARG, "bad energy grid");
+    // MultiGroupEnergyGrid::MakeLogSpaced (src/multigroup.cpp:40-59)
+    const double lo = std::log(emin), d = (std::log(emax) - lo) / G;
+    std::vector<double> b(G + 1);
+    b[0] = emin;
+    for (int g = 1; g <= G; g++) b[g] = std::exp(lo + g * d);
+    if (bounds)
+        for (int g = 0; g <= G; g++) bounds[g] = b[g];
+    if (midpoints)
+        for (int g = 0; g < G; g++) midpoints[g] = std::sqrt(b[g] * b[g + 1]);
+    return ALLIUM_OK;
+}
+
+int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
+                      const allium_trt_desc *desc, allium_trt_t *out)
+{
+    if (!mesh || !quad || !desc || !out) return tfail(ALLIUM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (mesh->dim != 3 || fe_order != 1)
+        return tfail(ALLIUM_ERR_UNSUPPORTED, "the TRT solver implements 3D meshes with p = 1");
+    const allium_trt_desc &d = *desc;
+    if (d.groups < 1 || d.groups > GMAX || !(d.dt > 0) || !(d.cv > 0) || !(d.T0 > 0) || !(d.T_inflow >= 0) ||
+        !(d.opacity_coef > 0) || d.si_maxit < 1 || !(d.si_reltol >= 0) || !(d.si_abstol >= 0) ||
+        !(d.si_reltol > 0 || d.si_abstol > 0))
+        return tfail(ALLIUM_ERR_ARG, "bad TRT description");
+    auto h = std::make_unique<allium_trt>();
+    h->d = d;
+    h->G = d.groups;
+    std::string why;
+    allium::Mesh m;
+    if (!allium::build_mesh(3, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, m, why))
+        return tfail(ALLIUM_ERR_ARG, why);
+    if (!m.uniform) return tfail(ALLIUM_ERR_UNSUPPORTED, "the TRT solver needs uniform geometry");
+    for (int s = 0; s < 6; s++)
+        if (mesh->bc[s] != ALLIUM_BC_INFLOW)
+            return tfail(ALLIUM_ERR_UNSUPPORTED, "the TRT solver takes inflow/vacuum boundaries");
+    h->Ne = m.Ne;
+    for (int k = 0; k < 3; k++) h->nel[k] = m.nel[k];
+    h->bounds.resize(h->G + 1);
+    h->mid.resize(h->G);
+    ABI_TRY(allium_log_groups(d.emin, d.emax, h->G, h->bounds.data(), h->mid.data()));
+    // sweep handle with the initial opacities (element-centre T0)
+    std::vector<double> sig0((size_t)h->G * h->Ne);
+    for (int g = 0; g < h->G; g++)
+        for (int e = 0; e < h->Ne; e++)
+            sig0[(size_t)g * h->Ne + e] = d.opacity_coef / std::sqrt(d.T0) / (h->mid[g] * h->mid[g] * h->mid[g]);
+    ABI_TRY(allium_sweep_create(mesh, 1, quad, h->G, sig0.data(), 0, ALLIUM_FP64, &h->L));
+    ABI_TRY(allium_sweep_set_time_absorption(h->L, 1.0 / d.dt));
+    allium_sweep_info_t info;
+    ABI_TRY(allium_sweep_info(h->L, &info));
+    h->No = (int)info.Nomega;
+    h->Ndof = info.Ndof;
+    h->psi_size = info.psi_size;
+    h->phi_size = info.phi_size;
+    std::vector<double> om((size_t)h->No * 3), w(h->No);
+    ABI_TRY(allium_sweep_quadrature(h->L, om.data(), w.data()));
+    for (double v : w) h->wsum += v;
+    // element matrices and rules
+    std::vector<double> Mu(N * N), Gd(3 * N * N), Mm(N * N);
+    ABI_TRY(allium_host_element_matrices(mesh, 1, 0, Mu.data(), Gd.data(), Mm.data()));
+    std::vector<double> phi3, w3, phi4, w4;
+    allium::uniform_element_rule(m, 1, 2 * 1 + 1, phi3, w3);   // PlanckEmissionNFI default order 2p + 1
+    allium::uniform_element_rule(m, 1, 2 * 1 + 2, phi4, w4);   // GrayPlanckEmissionNFI(.., 2, 2 + 0)
+    TRT_TRY(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    TrtParams &P = h->P;
+    P.Ne = h->Ne;
+    P.G = h->G;
+    P.nq3 = (int)w3.size();
+    P.nq4 = (int)w4.size();
+    P.cvdt = d.cv / d.dt;
+    double *dp;
+    TRT_TRY(h->alloc(&dp, phi3.size(), phi3.data())); P.phi3 = dp;
+    TRT_TRY(h->alloc(&dp, w3.size(), w3.data())); P.w3 = dp;
+    TRT_TRY(h->alloc(&dp, phi4.size(), phi4.data())); P.phi4 = dp;
+    TRT_TRY(h->alloc(&dp, w4.size(), w4.data())); P.w4 = dp;
+    TRT_TRY(h->alloc(&dp, Mm.size(), Mm.data())); P.Mm = dp;
+    TRT_TRY(h->alloc(&dp, Mu.size(), Mu.data())); P.Mu = dp;
+    TRT_TRY(h->alloc(&dp, h->bounds.size(), h->bounds.data())); P.bounds = dp;
+    TRT_TRY(h->alloc(&h->dmid, h->mid.size(), h->mid.data()));
+    TRT_TRY(h->alloc(&h->sigma, sig0.size(), sig0.data()));
+    P.sigma = h->sigma;
+    h->sigma_h = sig0;
+    TRT_TRY(h->alloc(&h->T, h->Ndof));
+    TRT_TRY(h->alloc(&h->Tn, h->Ndof));
+    TRT_TRY(h->alloc(&h->resid, h->Ndof));
+    TRT_TRY(h->alloc(&h->Binv, (size_t)h->Ne * N * N));
+    TRT_TRY(h->alloc(&h->dpw, (size_t)h->Ne * P.nq3 * h->G));
+    TRT_TRY(h->alloc(&h->psi, h->psi_size));
+    TRT_TRY(h->alloc(&h->psin, h->psi_size));
+    TRT_TRY(h->alloc(&h->qin, h->psi_size));
+    TRT_TRY(h->alloc(&h->qpsi, h->psi_size));
+    TRT_TRY(h->alloc(&h->em, h->phi_size));
+    TRT_TRY(h->alloc(&h->phi, h->phi_size));
+    TRT_TRY(h->alloc(&h->phisrc, h->phi_size));
+    TRT_TRY(h->alloc(&h->phit, h->phi_size));
+    TRT_TRY(h->alloc(&h->iso, h->phi_size));
+    TRT_TRY(h->alloc(&h->part, 2 * kPart));
+    TRT_TRY(h->alloc(&h->flags, 4));
+    // initial state: T = T0, psi in equilibrium with T0 (PlanckEmissionPSCoefficient,
+    // src/phase_coefficient.cpp:72-76: the Planck fraction between the group bounds)
+    std::vector<double> Th(h->Ndof, d.T0);
+    TRT_TRY(hipMemcpy(h->T, Th.data(), sizeof(double) * h->Ndof, hipMemcpyHostToDevice));
+    std::vector<double> psi0(h->psi_size);
+    const double fourpi = 4.0 * M_PI;
+    for (int g = 0; g < h->G; g++) {
+        const double b = allium::planck_integral(h->bounds[g + 1] / d.T0) - allium::planck_integral(h->bounds[g] / d.T0);
+        const double v = b * std::pow(d.T0, 4) / fourpi;
+        std::fill(psi0.begin() + (size_t)g * h->No * h->Ndof, psi0.begin() + (size_t)(g + 1) * h->No * h->Ndof, v);
+    }
+    TRT_TRY(hipMemcpy(h->psi, psi0.data(), sizeof(double) * h->psi_size, hipMemcpyHostToDevice));
+    // inflow source: Planckian at T_inflow on the flagged sides, vacuum elsewhere
+    std::vector<double> inflow((size_t)h->G * 6, 0.0);
+    for (int g = 0; g < h->G; g++) {
+        const double Ti = d.T_inflow;
+        const double b = Ti > 0 ? allium::planck_integral(h->bounds[g + 1] / Ti) - allium::planck_integral(h->bounds[g] / Ti)
+                                : 0.0;
+        for (int s = 0; s < 6; s++)
+            if ((d.inflow_sides >> s) & 1) inflow[(size_t)g * 6 + s] = b * std::pow(Ti, 4) / fourpi;
+    }
+    ABI_TRY(allium_form_source(h->L, nullptr, inflow.data(), psi0.data()));
+    TRT_TRY(hipMemcpy(h->qin, psi0.data(), sizeof(double) * h->psi_size, hipMemcpyHostToDevice));
+    ABI_TRY(update_opacity(h.get()));
+    *out = h.release();
+    return ALLIUM_OK;
+}
+
+void allium_trt_destroy(allium_trt_t h) { delete h; }
+
+allium_sweep_t allium_trt_sweep(allium_trt_t h) { return h ? h->L : nullptr; }
+
+int allium_trt_step(allium_trt_t h, allium_trt_info *info)
+{
+    if (!h) return tfail(ALLIUM_ERR_ARG, "null handle");
+    hipStream_t s = h->s;
+    const TrtParams &P = h->P;
+    const int eb = (h->Ne + 63) / 64;
+    // previous time level
+    TRT_TRY(hipMemcpyAsync(h->Tn, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToDevice, s));
+    TRT_TRY(hipMemcpyAsync(h->psin, h->psi, sizeof(double) * h->psi_size, hipMemcpyDeviceToDevice, s));
+    const long long cells = h->psi_size / N;
+    hipLaunchKernelGGL(k_time_source, dim3((unsigned)((cells * N + 255) / 256)), dim3(256), 0, s, cells,
+                       1.0 / h->d.dt, P.Mm, h->qin, h->psin, h->qpsi);
+    TRT_TRY(hipMemsetAsync(h->flags, 0, sizeof(int) * 4, s));
+    // linearize at T = T^n
+    hipLaunchKernelGGL(k_linearize, dim3(eb), dim3(64), 0, s, P, h->T, h->Tn, h->resid, h->Binv, h->em, h->dpw,
+                       h->flags);
+    // em += dB Bt'^{-1} resid
+    hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 0, h->resid, h->Binv, h->dpw, 1.0, 1, h->em);
+    TRT_TRY(hipGetLastError());
+    int bad = 0;
+    TRT_TRY(hipMemcpyAsync(&bad, h->flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    TRT_TRY(hipStreamSynchronize(s));
+    if (bad) return tfail(ALLIUM_ERR_NUMERIC, "singular energy-balance Jacobian");
+    const double inv_w = 1.0 / h->wsum;
+    // phi_src = D Linv (D^T em + q_psi)
+    TRT_TRY(allium::launch_axpby(h->phi_size, inv_w, h->em, 0.0, h->em, h->iso, s));
+    ABI_TRY(allium_sweep_mult_iso_device(h->L, h->qpsi, h->iso, h->psi, s));
+    ABI_TRY(allium_moments_device(h->L, h->psi, h->phisrc, s));
+    // source iteration on (I - D Linv D^T Ms) phi = phi_src (SLISolver, src/linalg.cpp:103-153)
+    if (h->d.si_warm_start) ABI_TRY(allium_moments_device(h->L, h->psin, h->phi, s));
+    else TRT_TRY(hipMemsetAsync(h->phi, 0, sizeof(double) * h->phi_size, s));
+    const int nb = allium::si_update_blocks(h->phi_size);
+    std::vector<double> hp(nb);
+    double r0 = 0.0, nrm = 0.0;
+    int it = 0;
+    for (it = 1;; it++) {
+        hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, inv_w, 0, h->iso);
+        TRT_TRY(hipGetLastError());
+        ABI_TRY(allium_sweep_mult_iso_device(h->L, nullptr, h->iso, h->psi, s));
+        ABI_TRY(allium_moments_device(h->L, h->psi, h->phit, s));
+        TRT_TRY(allium::launch_si_update(h->phi_size, h->phisrc, h->phit, h->phi, h->part, s));
+        TRT_TRY(hipMemcpyAsync(hp.data(), h->part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+        TRT_TRY(hipStreamSynchronize(s));
+        nrm = 0.0;
+        for (double v : hp) nrm += v;
+        nrm = std::sqrt(nrm);
+        if (!std::isfinite(nrm)) return tfail(ALLIUM_ERR_NUMERIC, "non-finite transport residual");
+        if (it == 1) r0 = std::max(nrm * h->d.si_reltol, h->d.si_abstol);
+        if (nrm < r0 || it >= h->d.si_maxit) break;
+    }
+    // em += Ms phi ; psi = Linv(D^T em + q_psi)
+    hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, 1.0, 1, h->em);
+    TRT_TRY(allium::launch_axpby(h->phi_size, inv_w, h->em, 0.0, h->em, h->iso, s));
+    ABI_TRY(allium_sweep_mult_iso_device(h->L, h->qpsi, h->iso, h->psi, s));
+    // temperature update
+    hipLaunchKernelGGL(k_update_T, dim3((h->Ne + 255) / 256), dim3(256), 0, s, P, h->phi, h->resid, h->Binv, h->T,
+                       h->flags + 1);
+    TRT_TRY(hipGetLastError());
+    h->step++;
+    h->time += h->d.dt;
+    // explicit opacity for the next step
+    ABI_TRY(update_opacity(h));
+    if (info) {
+        int fl = 0;
+        TRT_TRY(hipMemcpyAsync(&fl, h->flags + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_minmax, dim3(kPart), dim3(256), 0, s, (long long)h->Ndof, h->T, h->part);
+        std::vector<double> mm(2 * kPart);
+        TRT_TRY(hipMemcpyAsync(mm.data(), h->part, sizeof(double) * mm.size(), hipMemcpyDeviceToHost, s));
+        std::vector<double> phih(h->phi_size);
+        TRT_TRY(hipMemcpyAsync(phih.data(), h->phi, sizeof(double) * h->phi_size, hipMemcpyDeviceToHost, s));
+        TRT_TRY(hipStreamSynchronize(s));
+        info->step = h->step;
+        info->time = h->time;
+        info->transport_iterations = it;
+        info->transport_residual = nrm;
+        info->floored = fl;
+        info->T_min = 1e300;
+        info->T_max = -1e300;
+        for (int k = 0; k < kPart; k++) {
+            info->T_min = std::min(info->T_min, mm[2 * k]);
+            info->T_max = std::max(info->T_max, mm[2 * k + 1]);
+        }
+        // sum_g int phi_g = sum over dofs of phi times the element-mass row sums
+        double vol_w[N];
+        std::vector<double> Mu(N * N);
+        TRT_TRY(hipMemcpy(Mu.data(), P.Mu, sizeof(double) * N * N, hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++) {
+            vol_w[i] = 0.0;
+            for (int j = 0; j < N; j++) vol_w[i] += Mu[i * N + j];
+        }
+        double E = 0.0;
+        for (long k = 0; k < h->phi_size; k++) E += phih[k] * vol_w[k % N];
+        info->radiation_energy = E;
+        ABI_TRY(allium_sweep_timing(h->L, &info->sweep_seconds, &info->sweeps));
+    }
+    return ALLIUM_OK;
+}
+
+int allium_trt_temperature(allium_trt_t h, double *T)
+{
+    if (!h || !T) return tfail(ALLIUM_ERR_ARG, "null argument");
+    TRT_TRY(hipMemcpy(T, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToHost));
+    return ALLIUM_OK;
+}
+
+int allium_trt_scalar_flux(allium_trt_t h, double *phi)
+{
+    if (!h || !phi) return tfail(ALLIUM_ERR_ARG, "null argument");
+    TRT_TRY(hipMemcpy(phi, h->phi, sizeof(double) * h->phi_size, hipMemcpyDeviceToHost));
+    return ALLIUM_OK;
+}
+
+}  // extern "C"

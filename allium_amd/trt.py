@@ -1,0 +1,85 @@
+"""Multigroup linearized backward-Euler TRT on the GPU (include/allium_trt.h).
+
+Mirrors scallion's "linearized" solver: one LinearizedTRTOperator::Mult per
+time step (src/trt_linearized.cpp:230-312) in the time loop of
+exe/scallion.cpp:1354-1555, in dimensionless units (c = 1, black-body
+emission T^4)."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+class TRTDesc(ctypes.Structure):
+    _fields_ = [("groups", ctypes.c_int), ("emin", ctypes.c_double), ("emax", ctypes.c_double),
+                ("opacity_coef", ctypes.c_double), ("cv", ctypes.c_double), ("dt", ctypes.c_double),
+                ("T0", ctypes.c_double), ("T_inflow", ctypes.c_double), ("inflow_sides", ctypes.c_int),
+                ("si_reltol", ctypes.c_double), ("si_abstol", ctypes.c_double), ("si_maxit", ctypes.c_int),
+                ("si_warm_start", ctypes.c_int)]
+
+
+class TRTInfo(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_int), ("time", ctypes.c_double), ("transport_iterations", ctypes.c_int),
+                ("transport_residual", ctypes.c_double), ("floored", ctypes.c_long), ("T_min", ctypes.c_double),
+                ("T_max", ctypes.c_double), ("radiation_energy", ctypes.c_double),
+                ("sweep_seconds", ctypes.c_double), ("sweeps", ctypes.c_long)]
+
+
+def planck_integrals(x):
+    """(normalized Planck, Rosseland) integrals over [0, x] from the library."""
+    p, r = ctypes.c_double(), ctypes.c_double()
+    check(_lib.load().allium_planck_integrals(float(x), ctypes.byref(p), ctypes.byref(r)))
+    return p.value, r.value
+
+
+def log_groups(emin, emax, G):
+    b, m = np.zeros(G + 1), np.zeros(G)
+    check(_lib.load().allium_log_groups(emin, emax, G, b.ctypes.data_as(_lib.c_dp), m.ctypes.data_as(_lib.c_dp)))
+    return b, m
+
+
+class TRT:
+    """Linearized TRT solver on a uniform 3D hex mesh (p = 1)."""
+
+    def __init__(self, nel, sn=6, groups=16, emin=1e-2, emax=1e1, opacity_coef=10.0, cv=1.0, dt=1e-3, T0=0.01,
+                 T_inflow=1.0, inflow_sides=1, si_reltol=1e-8, si_abstol=0.0, si_maxit=1000, warm_start=False,
+                 quad=None):
+        from . import Mesh, _quad_desc
+        self._lib = _lib.load()
+        self.mesh = Mesh(nel)
+        q, self._keep = _quad_desc(sn if quad is None else quad, 3)
+        d = TRTDesc(groups, emin, emax, opacity_coef, cv, dt, T0, T_inflow, inflow_sides, si_reltol, si_abstol,
+                    si_maxit, int(warm_start))
+        h = ctypes.c_void_p()
+        m = self.mesh.desc()
+        check(self._lib.allium_trt_create(ctypes.byref(m), 1, ctypes.byref(q), ctypes.addressof(d), ctypes.byref(h)))
+        self._h = h
+        self.G = groups
+        self.Ndof = self.mesh.Ne * 8
+
+    def step(self):
+        info = TRTInfo()
+        check(self._lib.allium_trt_step(self._h, ctypes.addressof(info)))
+        return {k: getattr(info, k) for k, _ in TRTInfo._fields_}
+
+    def temperature(self):
+        T = np.empty(self.Ndof)
+        check(self._lib.allium_trt_temperature(self._h, T.ctypes.data_as(_lib.c_dp)))
+        return T
+
+    def scalar_flux(self):
+        phi = np.empty(self.G * self.Ndof)
+        check(self._lib.allium_trt_scalar_flux(self._h, phi.ctypes.data_as(_lib.c_dp)))
+        return phi
+
+    def set_launch(self, mode):
+        check(self._lib.allium_sweep_set_launch(self._lib.allium_trt_sweep(self._h), mode))
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.allium_trt_destroy(self._h)
+        except Exception:
+            pass

@@ -85,7 +85,7 @@ def measure_traffic(args, tuning):
         d = tempfile.mkdtemp(prefix="allium_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run",
                "--", sys.executable, os.path.abspath(__file__), "--n", str(args.n), "--seed", str(args.seed),
-               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-traffic", "--tuning", tun,
+               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-trt", "--no-traffic", "--tuning", tun,
                "--copy-mib", str(copy_mib), "--precision", args.precision, *box]
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=os.environ.get("TMPDIR", "/tmp"))
         path = os.path.join(d, "run_counter_collection.csv")
@@ -149,6 +149,31 @@ def si_run(n, seed, maxit, learned=True):
     return out
 
 
+def trt_run(steps=20):
+    """Multigroup linearized backward-Euler TRT, dimensionless: unit cube 32^3
+    hex, p=1, S6 (48 dirs), 16 log-spaced groups on [1e-2, 1e1],
+    sigma_g = 10/(e_g^3 sqrt T), c_v = 1, T0 = 0.01, Planckian T = 1 inflow on
+    x = 0, dt = 1e-3, 20 steps."""
+    import allium_amd as A
+    from allium_amd import trt
+
+    g = trt.TRT((32, 32, 32), sn=6, groups=16, emin=1e-2, emax=1e1, opacity_coef=10.0, cv=1.0, dt=1e-3, T0=0.01,
+                T_inflow=1.0, inflow_sides=1, si_reltol=1e-8)
+    g.set_launch(-1)
+    A.synchronize()
+    t = time.perf_counter()
+    its = []
+    for _ in range(steps):
+        info = g.step()
+        its.append(info["transport_iterations"])
+    wall = time.perf_counter() - t
+    unknowns = 16 * 48 * 32 ** 3 * 8
+    return {"steps": steps, "wall_s": wall, "ms_per_step": wall / steps * 1e3, "transport_iterations": its,
+            "sweeps": info["sweeps"], "sweep_seconds": info["sweep_seconds"],
+            "unknowns_per_sweep": unknowns, "sweep_unknowns_per_s": unknowns * info["sweeps"] / info["sweep_seconds"],
+            "T_min": info["T_min"], "T_max": info["T_max"], "radiation_energy": info["radiation_energy"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +187,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-si", action="store_true")
+    ap.add_argument("--no-trt", action="store_true", help="skip the 20-step multigroup TRT run")
     ap.add_argument("--si-maxit", type=int, default=3000)
     ap.add_argument("--tuning", default=None, help="mode,block,lanes (skips the autotuner)")
     ap.add_argument("--box", default=None, help="bx,by,bz,dirs_per_group,threads,prefetch of box sweeps (modes 4/5)")
@@ -260,6 +286,12 @@ def main():
             log("SI:", out["source_iteration"])
         except Exception as exc:  # reported, not hidden
             out["source_iteration"] = {"error": str(exc)}
+    if not args.no_trt:
+        try:
+            out["trt_config4"] = trt_run()
+            log("TRT:", {k: v for k, v in out["trt_config4"].items() if k != "transport_iterations"})
+        except Exception as exc:  # reported, not hidden
+            out["trt_config4"] = {"error": str(exc)}
     if not args.no_cpu:
         cb = cpu_baseline(args.n, args.seed, args.cpu_reps)
         log(f"cpu baseline {cb['value']:.3e} unknowns/s on {cb['cores']} threads")

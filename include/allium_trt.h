@@ -1,0 +1,83 @@
+/*
+ * allium_trt.h -- multigroup linearized backward-Euler thermal radiative
+ * transfer on the MI355X sweep (scallion's "linearized" solver).
+ *
+ * One time step is LinearizedTRTOperator::Mult (src/trt_linearized.cpp:230-312)
+ * driven by scallion's time loop (exe/scallion.cpp:1354-1555):
+ *   - emission B(T) = sum_g int sigma_g b_g(T) T^4 phi_i (PlanckEmissionNFI,
+ *     src/trt_integrators.cpp:56-118) and the energy balance
+ *     Bt(T) = (c_v/dt) M T + sum_g int sigma_g b_g(T) T^4 phi_i
+ *     (GrayPlanckEmissionNFI + MassIntegrator, exe/scallion.cpp:752-760),
+ *     linearized at the current T with the Rosseland spectrum;
+ *   - the transport system (I - D Linv D^T Ms) phi = D Linv(...) with the
+ *     pseudo-scattering Ms = dB (dBt)^{-1} M_sigma, solved by source iteration
+ *     (SLISolver, src/linalg.cpp:103-153);
+ *   - a final sweep for psi, T += (dBt)^{-1}(M_sigma phi + resid), floored at 1e-8;
+ *   - explicit opacities sigma_g = coef / (e_g^3 sqrt(T)) at the element centre
+ *     and group midpoint e_g (recomputed after every step, exe/scallion.cpp:1462).
+ * Units are dimensionless (radiation constant times light speed = 1, c = 1):
+ * the time absorption is 1/dt and black-body emission is T^4.
+ * Groups are log-spaced (MultiGroupEnergyGrid::MakeLogSpaced,
+ * src/multigroup.cpp:40-59).  Uniform 3D meshes, p = 1.
+ */
+#ifndef ALLIUM_TRT_H
+#define ALLIUM_TRT_H
+
+#include "allium_sweep.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct allium_trt *allium_trt_t;
+
+typedef struct {
+    int groups;           /* G */
+    double emin, emax;    /* photon energy range of the log-spaced groups */
+    double opacity_coef;  /* sigma_g = opacity_coef / (e_g^3 sqrt(T)) */
+    double cv;            /* heat capacity */
+    double dt;            /* time step */
+    double T0;            /* initial temperature; radiation starts in equilibrium with it */
+    double T_inflow;      /* Planckian inflow temperature on the inflow sides */
+    int inflow_sides;     /* bit s set: side s (axis*2 + hi) carries the inflow; others vacuum */
+    double si_reltol;     /* source iteration (SLISolver) stops when ||phi_{k+1} - phi_k|| <  */
+    double si_abstol;     /*   max(si_reltol * first norm, si_abstol) or after si_maxit */
+    int si_maxit;
+    int si_warm_start;    /* 0: start from phi = 0 (SLISolver without iterative_mode);
+                             1: from D psi^n */
+} allium_trt_desc;
+
+typedef struct {
+    int step;             /* steps taken */
+    double time;
+    int transport_iterations;   /* source iterations of the last step */
+    double transport_residual;  /* final ||phi_{k+1} - phi_k|| of the last step */
+    long floored;         /* temperature values floored at 1e-8 in the last step */
+    double T_min, T_max;  /* over the temperature dofs after the last step */
+    double radiation_energy;    /* sum_g int phi_g (dimensionless E = phi / c) */
+    double sweep_seconds; /* GPU time in sweeps so far */
+    long sweeps;
+} allium_trt_info;
+
+int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
+                      const allium_trt_desc *desc, allium_trt_t *out);
+void allium_trt_destroy(allium_trt_t h);
+/* advance one linearized backward-Euler step */
+int allium_trt_step(allium_trt_t h, allium_trt_info *info);
+/* temperature dofs (Ne * n) and scalar flux (G * Ne * n, RIGHT layout) */
+int allium_trt_temperature(allium_trt_t h, double *T);
+int allium_trt_scalar_flux(allium_trt_t h, double *phi);
+/* the sweep handle the solver uses (tuning, timing) */
+allium_sweep_t allium_trt_sweep(allium_trt_t h);
+
+/* host helpers: normalized Planck and Rosseland integrals over [0, x]
+ * (src/planck.hpp:158-180) and the log-spaced group bounds (G + 1) and
+ * geometric midpoints (G) */
+int allium_planck_integrals(double x, double *planck, double *rosseland);
+int allium_log_groups(double emin, double emax, int G, double *bounds, double *midpoints);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
