@@ -195,3 +195,56 @@ def train(corr, X, T, steps=1500, batch=8, lr=1e-3, seed=0, log=None):
         if log and (s + 1) % 250 == 0:
             log(f"  step {s + 1}: loss {np.mean(losses[-50:]):.4f}")
     return np.array(losses)
+
+
+def conv_flops(shape, batch=1):
+    """Multiply-add FLOPs (2 per MAC) of one forward pass on a (Z, Y, X) grid."""
+    vox = batch * int(np.prod(shape))
+    ch = [3, 32, 32, 32, 32, 1]
+    return 2 * vox * 27 * sum(ch[l] * ch[l + 1] for l in range(5))
+
+
+def main(argv=None):
+    """Time the network kernels: training steps on batches of 32^3 grids and
+    inference on an n^3 grid, with the MFMA rate they reach."""
+    import argparse
+
+    from . import synchronize
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--n", type=int, default=64, help="inference grid n^3")
+    ap.add_argument("--batch", type=int, default=8, help="training batch of 32^3 grids")
+    ap.add_argument("--train-steps", type=int, default=20)
+    ap.add_argument("--infer", type=int, default=20)
+    a = ap.parse_args(argv)
+    rng = np.random.default_rng(0)
+    net = FluxCorrector(0)
+    tshape = (a.batch, 32, 32, 32)
+    dx = DeviceArray32(rng.normal(size=tshape + (3,)).astype(np.float32))
+    dt = DeviceArray32(rng.normal(size=tshape).astype(np.float32))
+    net.train_step(dx, dt, tshape, 1e-3)
+    synchronize()
+    t = time.perf_counter()
+    for _ in range(a.train_steps):
+        net.train_step(dx, dt, tshape, 1e-3)
+    synchronize()
+    t_train = (time.perf_counter() - t) / a.train_steps
+    ishape = (1, a.n, a.n, a.n)
+    ix = DeviceArray32(rng.normal(size=ishape + (3,)).astype(np.float32))
+    iout = DeviceArray32(np.zeros(ishape, np.float32))
+    lib = net._lib
+    check(lib.allium_cnn_forward_device(net.handle, 1, a.n, a.n, a.n, ix.ptr, iout.ptr, None))
+    synchronize()
+    t = time.perf_counter()
+    for _ in range(a.infer):
+        check(lib.allium_cnn_forward_device(net.handle, 1, a.n, a.n, a.n, ix.ptr, iout.ptr, None))
+    synchronize()
+    t_inf = (time.perf_counter() - t) / a.infer
+    f_inf = conv_flops((a.n,) * 3)
+    f_train = 3 * conv_flops((32, 32, 32), a.batch)   # forward + backward-data + weight gradients
+    print({"train_step_ms": t_train * 1e3, "train_tflops": f_train / t_train / 1e12,
+           "inference_ms": t_inf * 1e3, "inference_tflops": f_inf / t_inf / 1e12,
+           "inference_grid": a.n, "train_batch": a.batch}, flush=True)
+
+
+if __name__ == "__main__":
+    main()

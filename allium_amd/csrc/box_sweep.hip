@@ -135,7 +135,7 @@ __device__ __forceinline__ void box_solve(const SweepParams &p, const BoxPlan &b
                                           const BoxInputs<Sh<DIM, P>::N> &in, int g, Mid &&mid)
 {
     using S = Sh<DIM, P>;
-    constexpr int N = S::N, NF = S::NF;
+    constexpr int N = S::N, NF = S::NF, PS = box_slot_stride(S::NF);
     const int a = in.a, d = in.d;
     double *slot[DIM];
     double u[DIM][NF];
@@ -148,8 +148,8 @@ __device__ __forceinline__ void box_solve(const SweepParams &p, const BoxPlan &b
 #pragma unroll
             for (int o = 0; o < DIM; ++o)
                 if (o != ax) { s += mul * in.lc[o]; mul *= bp.bs[o]; }
-            slot[ax] = pl + ((long long)d * bp.psz[ax] + s) * NF;
-            pl += (long long)bp.nd * bp.psz[ax] * NF;
+            slot[ax] = pl + ((long long)d * bp.psz[ax] + s) * PS;
+            pl += (long long)bp.nd * bp.psz[ax] * PS;
 #pragma unroll
             for (int m = 0; m < NF; ++m) u[ax][m] = slot[ax][m];
         }
@@ -210,7 +210,7 @@ template <int DIM, int P, typename T, int OCT, int PF>
 __device__ __forceinline__ void box_task(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
 {
     using S = Sh<DIM, P>;
-    constexpr int N = S::N, NF = S::NF;
+    constexpr int N = S::N, NF = S::NF, PS = box_slot_stride(S::NF);
     const int task = blockIdx.x / bp.noct;
     const int g = blockIdx.y / bp.nchunk, chunk = blockIdx.y - g * bp.nchunk;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -229,7 +229,7 @@ __device__ __forceinline__ void box_task(const SweepParams &p, const BoxPlan &bp
 #pragma unroll
     for (int ax = 0; ax < DIM; ++ax) nplane += bp.psz[ax];
     // LDS layout (doubles first, then 32-bit tables)
-    double *Kw = lds + (long long)bp.nd * nplane * NF;
+    double *Kw = lds + (long long)bp.nd * nplane * PS;
     double *omw = Kw + (long long)bp.nd * bp.kstride;
     int *dirw = reinterpret_cast<int *>(omw + 3 * bp.nd);
     uint32_t *lcw = reinterpret_cast<uint32_t *>(dirw + bp.nd);
@@ -263,7 +263,7 @@ __device__ __forceinline__ void box_task(const SweepParams &p, const BoxPlan &bp
             const int np = bp.psz[ax];
             for (int i = tid; i < ndir * np; i += nt) {
                 const int d = i / np, s = i - d * np;
-                double *dst = pl + ((long long)d * np + s) * NF;
+                double *dst = pl + ((long long)d * np + s) * PS;
                 int c[3] = {0, 0, 0}, rem = s;
                 const int a = gdirs[d];
                 bool valid = in_dom && a >= 0;
@@ -285,7 +285,7 @@ __device__ __forceinline__ void box_task(const SweepParams &p, const BoxPlan &bp
                     for (int m = 0; m < NF; ++m) dst[m] = 0.0;
                 }
             }
-            pl += (long long)bp.nd * np * NF;
+            pl += (long long)bp.nd * np * PS;
         }
     }
     __syncthreads();

@@ -69,6 +69,10 @@ struct Tile {
 // between the box's cells through LDS planes.  Boxes on one anti-diagonal of
 // the box grid (in the octant's orientation) are independent: one launch per
 // box level.
+// doubles per trace slot in the box sweep's LDS planes: an odd stride so that
+// the slots lanes read at once spread over the LDS banks
+__host__ __device__ constexpr int box_slot_stride(int nf) { return nf % 2 == 0 ? nf + 1 : nf; }
+
 struct BoxPlan {
     int bs[3];               // box extents in elements (1 beyond dim)
     int nb[3];               // boxes per axis

@@ -313,7 +313,8 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
         plane += bp.psz[ax];
     }
     const long vol = (long)bs[0] * bs[1] * bs[2];
-    const long lds = 8L * ((long)nd * nf * plane + (long)nd * bp.kstride + 3L * nd) + 4L * (nd + vol + bp.nl + 1);
+    const long lds = 8L * ((long)nd * allium::box_slot_stride(nf) * plane + (long)nd * bp.kstride + 3L * nd) +
+                     4L * (nd + vol + bp.nl + 1);
     // the box kernel builds Mu and the face masses as tensor products of the
     // exact 1D masses: check that the assembled matrices are those
     {

@@ -111,7 +111,7 @@ def measure_traffic(args, tuning):
     return total, detail
 
 
-def si_run(n, seed, maxit, learned=True):
+def si_run(n, seed, maxit, learned=True, tuning=None):
     """Wall time to a 1e-8 FP64 scalar-flux residual on the 64^3 p=1 S8 c=0.9999
     medium (lognormal sigma_t, log-sd 1.5, Q=1): plain GPU source iteration
     against source iteration with the learned correction (FP64 and
@@ -124,7 +124,14 @@ def si_run(n, seed, maxit, learned=True):
     cfg = P.problem_3d_hetero(n=n, seed=seed, log_sd=1.5, c_const=0.9999)
     m = A.Mesh(cfg["nel"])
     Linv = A.InverseAdvectionOperator(m, 1, 8, cfg["sigma_t"])
-    Linv.set_launch(-1)
+    if tuning is None:
+        Linv.set_launch(-1)
+    else:
+        if "box" in tuning:
+            b = tuning["box"]
+            Linv.set_box_tiling(b["bx"], b["by"], b["bz"], dirs_per_group=b["dirs_per_group"], threads=b["threads"],
+                                prefetch=b["prefetch"])
+        Linv.set_tuning(tuning["mode"], tuning["block"], tuning["lanes"])
     q = Linv.form_source(Q=1.0)
 
     def solve(**kw):
@@ -187,6 +194,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-si", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="plain source iteration only (no network training)")
     ap.add_argument("--no-trt", action="store_true", help="skip the 20-step multigroup TRT run")
     ap.add_argument("--si-maxit", type=int, default=3000)
     ap.add_argument("--tuning", default=None, help="mode,block,lanes (skips the autotuner)")
@@ -282,7 +290,8 @@ def main():
     }
     if not args.no_si:
         try:
-            out["source_iteration"] = si_run(args.n, args.seed, args.si_maxit)
+            out["source_iteration"] = si_run(args.n, args.seed, args.si_maxit, learned=not args.no_learned,
+                                             tuning=tuning if args.tuning else None)
             log("SI:", out["source_iteration"])
         except Exception as exc:  # reported, not hidden
             out["source_iteration"] = {"error": str(exc)}
