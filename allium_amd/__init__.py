@@ -283,6 +283,41 @@ class InverseAdvectionOperator:
         return t.value, n.value
 
 
+class SweepBatch:
+    """Independent instances swept together, one launch per level of all of
+    them (3D p = 2 handles): psi_i = L_i^{-1}(source_i + iso_i)."""
+
+    def __init__(self, operators):
+        self._lib = _lib.load()
+        self.ops = list(operators)
+        arr = (ctypes.c_void_p * len(self.ops))(*[op._h.value for op in self.ops])
+        h = ctypes.c_void_p()
+        check(self._lib.allium_sweep_batch_create(arr, len(self.ops), ctypes.byref(h)))
+        self._h = h
+
+    def Mult(self, sources, psis, isos=None):
+        """Device vectors: sources (entries may be None), psis, optional isos."""
+        n = len(self.ops)
+        ptr = lambda v: None if v is None else v.ptr.value
+        src = (ctypes.c_void_p * n)(*[ptr(v) for v in sources]) if sources is not None else None
+        iso = (ctypes.c_void_p * n)(*[ptr(v) for v in isos]) if isos is not None else None
+        out = (ctypes.c_void_p * n)(*[ptr(v) for v in psis])
+        check(self._lib.allium_sweep_batch_mult_device(self._h, src, iso, out, None))
+        return psis
+
+    def timing(self):
+        t, n = ctypes.c_double(), ctypes.c_long()
+        check(self._lib.allium_sweep_batch_timing(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.allium_sweep_batch_destroy(self._h)
+        except Exception:
+            pass
+
+
 class AdvectionOperator:
     """L psi and particle balance from the sweep's matrices (src/sweep.cpp:1246-1462)."""
 

@@ -69,6 +69,11 @@ SIGNATURES = {
     "allium_source_iteration": (ctypes.c_int, [c_vp, c_dp, c_dp, ctypes.POINTER(SIOptions), c_dp, c_dp, c_dp,
                                                c_ip]),
     "allium_sweep_timing": (ctypes.c_int, [c_vp, c_dp, c_lp]),
+    "allium_sweep_batch_create": (ctypes.c_int, [ctypes.POINTER(c_vp), ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "allium_sweep_batch_destroy": (None, [c_vp]),
+    "allium_sweep_batch_mult_device": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp),
+                                                      ctypes.POINTER(c_vp), c_vp]),
+    "allium_sweep_batch_timing": (ctypes.c_int, [c_vp, c_dp, c_lp]),
     "allium_host_quadrature": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_ip]),
     "allium_host_levels": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(QuadDesc), c_ip, c_lp]),
     "allium_host_form_source": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc),

@@ -1,0 +1,178 @@
+// Cooperative cell solves for local DG systems that do not fit one lane's
+// registers (3D p = 2: 27 x 27), restating src/sweep.cpp:366-475.
+//
+// 32 lanes solve one cell (e, a, g): lane r < n assembles row r of
+//   A = sum_d Omega_d G_d + sigma M + tau Mt + sum_{outflow} (Omega.n) F
+// and of the right-hand side (source, isotropic source, upwind traces), then
+// the cell's lanes run an LU without pivoting (A + A^T is positive definite)
+// in which the owner of pivot row k publishes it through LDS, and a
+// column-oriented back substitution.  A 256-thread workgroup holds 8 cells.
+// The launch covers one level of up to kMaxBatch independent instances, so
+// a batch of uneven meshes sweeps in max(levels) launches instead of
+// sum(levels).
+#include "device.hpp"
+#include "kernel_common.hpp"
+
+namespace allium {
+
+bool coop_supported(int dim, int p) { return dim == 3 && p == 2; }
+
+template <int DIM, int P, bool UNI, typename T>
+__global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepParams *__restrict__ prms,
+                                                    CoopLevel cl)
+{
+    using S = Sh<DIM, P>;
+    constexpr int N = S::N, NF = S::NF, NS = S::NS;
+    static_assert(N <= 32, "one row per lane of a 32-lane group");
+    __shared__ T piv[8][N + 1];
+    const int slot = threadIdx.x >> 5, r = threadIdx.x & 31;
+    const long long item = (long long)blockIdx.x * 8 + slot;
+    int inst = 0;
+    while (inst + 1 < cl.ninst && item >= cl.start[inst + 1]) ++inst;
+    const bool valid = item < cl.start[cl.ninst];
+    const SweepParams &p = prms ? prms[inst] : p0;
+    int e = 0, a = 0, g = 0;
+    if (valid) {
+        const int local = (int)(item - cl.start[inst]);
+        const int nc = cl.ncells[inst];
+        g = local / nc;
+        const uint32_t cell = p.cells[cl.cbeg[inst] + (local - g * nc)];
+        e = (int)(cell >> 10);
+        a = (int)(cell & 1023u);
+    }
+    const bool row = valid && r < N;
+    const int rr = r < N ? r : 0;
+
+    // ---- row rr of the local system -----------------------------------------
+    double Ar[N], b = 0.0;
+    const double om[3] = {p.omega[4 * a], p.omega[4 * a + 1], p.omega[4 * a + 2]};
+    const double sig = p.sigma[(long long)g * p.Ne + e];
+    const long long base = ((long long)g * p.Nomega + a) * p.Ndof;
+    const long long off = base + (long long)e * N;
+    if constexpr (UNI) {
+        const double *K = p.K + ((size_t)a * N + rr) * N;   // includes tau * Mt
+        const double *Mu = p.Mu + (size_t)rr * N;
+#pragma unroll
+        for (int j = 0; j < N; ++j) Ar[j] = fma(sig, Mu[j], K[j]);
+    } else {
+        const double *Gd = p.Gd + (size_t)e * DIM * N * N + (size_t)rr * N;
+        const double *Mu = p.Mu + (size_t)e * N * N + (size_t)rr * N;
+        const double *Mt = p.Mt + (size_t)e * N * N + (size_t)rr * N;
+        const double tau = p.timedep ? p.tau : 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double v = om[0] * Gd[j];
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) v = fma(om[d], Gd[(size_t)d * N * N + j], v);
+            v = fma(sig, Mu[j], v);
+            Ar[j] = fma(tau, Mt[j], v);
+        }
+    }
+    if (p.src) b = p.src[off + rr];
+    if (p.iso) b += p.iso[(long long)g * p.Ndof + (long long)e * N + rr];
+    int c[3] = {e % p.nel[0], (e / p.nel[0]) % p.nel[1], e / (p.nel[0] * p.nel[1])};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int ax = s >> 1, hi = s & 1;
+        const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
+        double dot;
+        const double *F;
+        if constexpr (UNI) {
+            dot = hi ? om[ax] : -om[ax];
+            F = p.Fface + s * NF * NF;
+        } else {
+            int cf[3] = {c[0], c[1], c[2]};
+            cf[ax] += hi;
+            const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
+            dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
+            if (!hi && interior) dot = -dot;
+            F = p.Fface + (size_t)f * NF * NF;
+        }
+        int k = 0;
+        if (!on_side<DIM, P>(rr, s, k)) continue;
+        if constexpr (!UNI) {
+            if (dot > 0) {   // outflow: own face block
+#pragma unroll
+                for (int l = 0; l < NF; ++l) Ar[side_node<DIM, P>(s, l)] = fma(dot, F[k * NF + l], Ar[side_node<DIM, P>(s, l)]);
+            }
+        }
+        const bool refl = !interior && ((p.refl_mask >> s) & 1);
+        if (dot < 0 && (interior || refl)) {
+            double acc = 0.0;
+            if (interior) {
+                const double *un = p.psi + base + (long long)(hi ? e + p.stride[ax] : e - p.stride[ax]) * N;
+#pragma unroll
+                for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], un[side_node<DIM, P>(s ^ 1, l)], acc);
+            } else {
+                const double *ur = p.psi + ((long long)g * p.Nomega + p.refl[a * NS + s]) * p.Ndof + (long long)e * N;
+#pragma unroll
+                for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], ur[side_node<DIM, P>(s, l)], acc);
+            }
+            b = fma(-dot, acc, b);
+        }
+    }
+    // lanes without a row carry an identity row so that nothing is singular
+    T A[N], bb = (T)b;
+#pragma unroll
+    for (int j = 0; j < N; ++j) A[j] = row ? (T)Ar[j] : (T)(j == r ? 1.0 : 0.0);
+    if (!row) bb = (T)0;
+
+    // ---- LU without pivoting across the cell's lanes ------------------------
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (r == k) {
+#pragma unroll
+            for (int j = k; j < N; ++j) piv[slot][j] = A[j];
+            piv[slot][N] = bb;
+        }
+        __syncthreads();
+        if (r > k && r < N) {
+            const T l = A[k] / piv[slot][k];
+#pragma unroll
+            for (int j = k + 1; j < N; ++j) A[j] = fma(-l, piv[slot][j], A[j]);
+            bb = fma(-l, piv[slot][N], bb);
+        }
+        __syncthreads();
+    }
+    // ---- back substitution, one unknown per step ----------------------------
+    T x = (T)0;
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k) {
+        if (r == k) {
+            x = bb / A[k];
+            piv[slot][0] = x;
+        }
+        __syncthreads();
+        const T xk = piv[slot][0];
+        if (r < k) bb = fma(-A[k], xk, bb);
+        __syncthreads();
+    }
+    if (row) p.psi[off + rr] = (double)x;
+}
+
+template <int DIM, int P, bool UNI>
+static hipError_t launch_dp(const SweepParams &p0, const SweepParams *d_prms, const CoopLevel &cl, int precision,
+                            hipStream_t s)
+{
+    const long long items = cl.start[cl.ninst];
+    if (items <= 0) return hipSuccess;
+    const int grid = (int)((items + 7) / 8);
+    if (precision == 1)
+        hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, float>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
+    else
+        hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, double>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
+    return hipGetLastError();
+}
+
+hipError_t launch_coop_level(const SweepParams &p0, const SweepParams *d_prms, const CoopLevel &cl, int dim, int p,
+                             bool uniform, int precision, hipStream_t s)
+{
+    if (dim == 3 && p == 2)
+        return uniform ? launch_dp<3, 2, true>(p0, d_prms, cl, precision, s)
+                       : launch_dp<3, 2, false>(p0, d_prms, cl, precision, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace allium

@@ -144,6 +144,21 @@ struct DeviceSweep {
 
 hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, double *psi,
                         hipStream_t stream);
+
+// Cooperative cell solves (local systems too large for one lane: 3D p = 2):
+// 32 lanes per cell, lane r owns row r.  One launch runs one level of up to
+// kMaxBatch independent instances (batched sweeps of uneven meshes).
+constexpr int kMaxBatch = 16;
+struct CoopLevel {
+    int ninst;
+    int start[kMaxBatch + 1];    // prefix over instances of (cells x groups) in this launch
+    long long cbeg[kMaxBatch];   // first cell of the level in each instance's cells array
+    int ncells[kMaxBatch];       // cells of the level in each instance
+};
+bool coop_supported(int dim, int p);
+// d_prms: device array of the instances' parameters, or null with ninst == 1 to use p0
+hipError_t launch_coop_level(const SweepParams &p0, const SweepParams *d_prms, const CoopLevel &cl, int dim,
+                             int p, bool uniform, int precision, hipStream_t s);
 hipError_t reset_persistent(DeviceSweep &ds, hipStream_t stream);
 // one sweep as box-level launches (modes 4/5); the box plan must be built
 hipError_t launch_box_sweep(const DeviceSweep &ds, const SweepParams &prm, hipStream_t stream);

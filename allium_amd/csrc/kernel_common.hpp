@@ -99,6 +99,19 @@ __device__ __forceinline__ void lu_resolve(const T (&A)[N][N], T (&x)[N])
     }
 }
 
+// is local node r on side s, and its face-lexicographic index there
+template <int DIM, int P>
+__device__ __forceinline__ bool on_side(int r, int s, int &k)
+{
+    constexpr int Q = P + 1;
+    const int i0 = r % Q, i1 = (r / Q) % Q, i2 = r / (Q * Q);
+    const int ax = s >> 1, want = (s & 1) ? P : 0;
+    const int ia = ax == 0 ? i0 : (ax == 1 ? i1 : i2);
+    if (DIM == 2) k = ax == 0 ? i1 : i0;
+    else k = ax == 0 ? i1 + Q * i2 : (ax == 1 ? i0 + Q * i2 : i0 + Q * i1);
+    return ia == want;
+}
+
 // 1D mass matrix of the Gauss-Lobatto Lagrange basis of degree P on [0, 1]
 // (exact integrals: P=1 h/6 [[2, 1], [1, 2]]; P=2 nodes 0, 1/2, 1:
 // h/30 [[4, 2, -1], [2, 16, 2], [-1, 2, 4]]).

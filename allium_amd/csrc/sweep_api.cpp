@@ -215,7 +215,17 @@ int upload_all(allium_sweep *h)
     prm.Gd = dGd; prm.Fface = dF; prm.Ffacef = fF; prm.fnor = dN;
     HIP_TRY(upload(ds.owned, &h->d_sigma, h->sigma.data(), h->sigma.size()));
     prm.sigma = h->d_sigma;
-    HIP_TRY(upload(ds.owned, &ds.d_cells, h->plan.cells.data(), h->plan.cells.size()));
+    if (allium::coop_supported(dim, ed.p)) {
+        // cooperative solves read their element's matrices row by row: order
+        // each level by element, then direction, so that the cells one
+        // workgroup solves mostly share an element (cache reuse)
+        std::vector<uint32_t> cells = h->plan.cells;
+        for (int L = 0; L < h->plan.nlevels; L++)
+            std::sort(cells.begin() + h->plan.level_ptr[L], cells.begin() + h->plan.level_ptr[L + 1]);
+        HIP_TRY(upload(ds.owned, &ds.d_cells, cells.data(), cells.size()));
+    } else {
+        HIP_TRY(upload(ds.owned, &ds.d_cells, h->plan.cells.data(), h->plan.cells.size()));
+    }
     prm.cells = ds.d_cells;
     prm.refl_mask = 0;
     for (int s = 0; s < 2 * dim; s++)
@@ -600,6 +610,8 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
     if (!h || mode < -1 || mode > 5) return fail(ALLIUM_ERR_ARG, "bad launch mode");
     if (mode == 3 && !h->ds.angle_independent)
         return fail(ALLIUM_ERR_UNSUPPORTED, "direction sweeps need directions that do not couple (no reflective side)");
+    if (allium::coop_supported(h->mesh.dim, h->ed.p) && mode != 0 && mode != 1 && mode != -1)
+        return fail(ALLIUM_ERR_UNSUPPORTED, "3D p=2 sweeps run cooperative level launches (modes 0 and 1)");
     if ((mode == 4 || mode == 5) && !h->ds.box_buf) {
         int rc = default_box_plan(h);
         if (rc) return rc;
@@ -616,16 +628,24 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
     HIP_TRY(hipMemsetAsync(h->b.p, 0, bytes, h->stream));
     struct Cand { int mode, block, lanes, bx, by, bz, nd, pf; };
     std::vector<Cand> cands;
-    if (h->ds.angle_independent)
+    const bool coop = allium::coop_supported(h->mesh.dim, h->ed.p);
+    if (coop) {
+        cands.push_back({0, 256, 1, 0, 0, 0, 0, 0});
+        cands.push_back({1, 256, 1, 0, 0, 0, 0, 0});
+    }
+    if (coop) {
+    } else if (h->ds.angle_independent)
         for (int bl : {256, 512}) cands.push_back({3, bl, 1, 0, 0, 0, 0, 0});
     for (int md : {0, 1})
         for (int bl : {64, 128, 256})
             for (int ln : {0, 1, 2, 4}) {
+                if (coop) continue;
                 if (ln == 0 && !h->ds.d_wcells) continue;
                 if (ln > 0 && h->ed.n % ln) continue;
                 cands.push_back({md, bl, ln, 0, 0, 0, 0, 0});
             }
-    const bool boxes = h->mesh.uniform && h->ds.angle_independent && allium::box_sweep_supported(h->mesh.dim, h->ed.p);
+    const bool boxes = !coop && h->mesh.uniform && h->ds.angle_independent &&
+                       allium::box_sweep_supported(h->mesh.dim, h->ed.p);
     if (boxes) {
         const bool d3 = h->mesh.dim == 3;
         for (int b : d3 ? std::vector<int>{4, 8, 16} : std::vector<int>{8, 16, 32})
@@ -979,6 +999,105 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
     if (iterations) *iterations = it;
     const bool converged = norms ? (norms[it - 1] < r0) : true;
     if (!converged) return fail(ALLIUM_ERR_NOT_CONVERGED, "source iteration reached maxit");
+    return ALLIUM_OK;
+}
+
+}  // extern "C"
+
+struct allium_sweep_batch {
+    std::vector<allium_sweep *> h;
+    std::vector<allium::CoopLevel> levels;
+    std::vector<allium::SweepParams> prms;
+    allium::SweepParams *d_prms = nullptr;
+    bool uniform = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double seconds = 0.0;
+    long sweeps = 0;
+    ~allium_sweep_batch()
+    {
+        if (d_prms) hipFree(d_prms);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+    }
+};
+
+extern "C" {
+
+int allium_sweep_batch_create(const allium_sweep_t *handles, int count, allium_sweep_batch_t *out)
+{
+    if (!handles || !out || count < 1 || count > allium::kMaxBatch)
+        return fail(ALLIUM_ERR_ARG, "batch needs 1..16 handles");
+    *out = nullptr;
+    auto b = std::make_unique<allium_sweep_batch>();
+    for (int i = 0; i < count; i++) {
+        allium_sweep *h = handles[i];
+        if (!h) return fail(ALLIUM_ERR_ARG, "null handle in batch");
+        if (!allium::coop_supported(h->mesh.dim, h->ed.p))
+            return fail(ALLIUM_ERR_UNSUPPORTED, "batched sweeps are implemented for 3D p=2 handles");
+        if (i > 0 && h->mesh.uniform != handles[0]->mesh.uniform)
+            return fail(ALLIUM_ERR_ARG, "batched handles must all be uniform or all general geometry");
+        b->h.push_back(h);
+    }
+    b->uniform = handles[0]->mesh.uniform;
+    int maxl = 0;
+    for (auto *h : b->h) maxl = std::max(maxl, h->ds.nlevels);
+    for (int L = 0; L < maxl; L++) {
+        allium::CoopLevel cl{};
+        cl.ninst = count;
+        cl.start[0] = 0;
+        for (int i = 0; i < count; i++) {
+            const auto &ds = b->h[i]->ds;
+            const bool has = L < ds.nlevels;
+            cl.cbeg[i] = has ? ds.level_ptr[L] : 0;
+            cl.ncells[i] = has ? (int)(ds.level_ptr[L + 1] - ds.level_ptr[L]) : 0;
+            cl.start[i + 1] = cl.start[i] + cl.ncells[i] * b->h[i]->G;
+        }
+        b->levels.push_back(cl);
+    }
+    b->prms.resize(count);
+    HIP_TRY(hipMalloc(&b->d_prms, sizeof(allium::SweepParams) * count));
+    HIP_TRY(hipEventCreate(&b->e0));
+    HIP_TRY(hipEventCreate(&b->e1));
+    *out = b.release();
+    return ALLIUM_OK;
+}
+
+void allium_sweep_batch_destroy(allium_sweep_batch_t b) { delete b; }
+
+int allium_sweep_batch_mult_device(allium_sweep_batch_t b, const double *const *d_source, const double *const *d_iso,
+                                   double *const *d_psi, void *stream)
+{
+    if (!b || !d_psi) return fail(ALLIUM_ERR_ARG, "null argument");
+    const int n = (int)b->h.size();
+    const int prec = b->h[0]->ds.precision;
+    for (int i = 0; i < n; i++) {
+        if (!d_psi[i]) return fail(ALLIUM_ERR_ARG, "null psi in batch");
+        if (b->h[i]->ds.precision != prec) return fail(ALLIUM_ERR_ARG, "batched handles must share precision");
+        b->prms[i] = b->h[i]->ds.prm;
+        b->prms[i].src = d_source ? d_source[i] : nullptr;
+        b->prms[i].iso = d_iso ? d_iso[i] : nullptr;
+        b->prms[i].psi = d_psi[i];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(b->d_prms, b->prms.data(), sizeof(allium::SweepParams) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipEventRecord(b->e0, s));
+    for (const auto &cl : b->levels)
+        HIP_TRY(allium::launch_coop_level(b->prms[0], b->d_prms, cl, 3, 2, b->uniform, prec, s));
+    HIP_TRY(hipEventRecord(b->e1, s));
+    HIP_TRY(hipEventSynchronize(b->e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+    b->seconds += ms * 1e-3;
+    b->sweeps++;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_batch_timing(allium_sweep_batch_t b, double *seconds, long *sweeps)
+{
+    if (!b) return fail(ALLIUM_ERR_ARG, "null batch");
+    if (seconds) *seconds = b->seconds;
+    if (sweeps) *sweeps = b->sweeps;
     return ALLIUM_OK;
 }
 

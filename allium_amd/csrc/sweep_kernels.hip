@@ -243,19 +243,6 @@ __device__ __forceinline__ float group_bcast(float v)
     }
 }
 
-// is local node r on side s, and its face-lexicographic index there
-template <int DIM, int P>
-__device__ __forceinline__ bool on_side(int r, int s, int &k)
-{
-    constexpr int Q = P + 1;
-    const int i0 = r % Q, i1 = (r / Q) % Q, i2 = r / (Q * Q);
-    const int ax = s >> 1, want = (s & 1) ? P : 0;
-    const int ia = ax == 0 ? i0 : (ax == 1 ? i1 : i2);
-    if (DIM == 2) k = ax == 0 ? i1 : i0;
-    else k = ax == 0 ? i1 + Q * i2 : (ax == 1 ? i0 + Q * i2 : i0 + Q * i1);
-    return ia == want;
-}
-
 template <int DIM, int P, bool UNI, typename T, int LPC>
 __device__ __forceinline__ void group_cell_solve(const SweepParams &p, const int e, const int a, const int g,
                                                  const int l)
@@ -586,9 +573,27 @@ __global__ void k_fill_int(int *p, int n, int v)
 bool sweep_supported(int dim, int p, std::string &why)
 {
     if (dim == 2 && p >= 0 && p <= 2) return true;
-    if (dim == 3 && p >= 0 && p <= 1) return true;
-    why = "GPU sweep implements 2D p=0..2 and 3D p=0..1";
+    if (dim == 3 && p >= 0 && p <= 2) return true;
+    why = "GPU sweep implements 2D p=0..2 and 3D p=0..2";
     return false;
+}
+
+// levels of one instance through the cooperative kernel (3D p = 2)
+static hipError_t run_coop(DeviceSweep &ds, const SweepParams &prm, hipStream_t s, int mode)
+{
+    if (mode != 0) return hipErrorInvalidValue;
+    for (int L = 0; L < ds.nlevels; ++L) {
+        CoopLevel cl{};
+        cl.ninst = 1;
+        cl.cbeg[0] = ds.level_ptr[L];
+        cl.ncells[0] = (int)(ds.level_ptr[L + 1] - ds.level_ptr[L]);
+        cl.start[0] = 0;
+        cl.start[1] = cl.ncells[0] * prm.G;
+        if (cl.start[1] == 0) continue;
+        hipError_t e = launch_coop_level(prm, nullptr, cl, prm.dim, prm.p, prm.uniform != 0, ds.precision, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <int DIM, int P, bool UNI, typename T, int LPC>
@@ -701,6 +706,7 @@ static hipError_t dispatch(DeviceSweep &ds, const SweepParams &prm, hipStream_t 
         switch (prm.p) {
         case 0: return dispatch_dp<3, 0>(ds, prm, s, mode);
         case 1: return dispatch_dp<3, 1>(ds, prm, s, mode);
+        case 2: return run_coop(ds, prm, s, mode);
         }
     }
     return hipErrorInvalidValue;
@@ -862,6 +868,7 @@ hipError_t launch_apply_L(const SweepParams &prm, const double *x, double *y, hi
     } else {
         if (prm.p == 0) return apply_L_dp<3, 0>(prm, x, y, s);
         if (prm.p == 1) return apply_L_dp<3, 1>(prm, x, y, s);
+        if (prm.p == 2) return apply_L_dp<3, 2>(prm, x, y, s);
     }
     return hipErrorInvalidValue;
 }
@@ -962,6 +969,7 @@ hipError_t launch_balance(const SweepParams &prm, const double *psi, double *par
     } else {
         if (prm.p == 0) BAL(3, 0);
         else if (prm.p == 1) BAL(3, 1);
+        else if (prm.p == 2) BAL(3, 2);
     }
 #undef BAL
     return hipGetLastError();

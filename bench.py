@@ -85,7 +85,7 @@ def measure_traffic(args, tuning):
         d = tempfile.mkdtemp(prefix="allium_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run",
                "--", sys.executable, os.path.abspath(__file__), "--n", str(args.n), "--seed", str(args.seed),
-               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-trt", "--no-traffic", "--tuning", tun,
+               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-trt", "--no-config3", "--no-traffic", "--tuning", tun,
                "--copy-mib", str(copy_mib), "--precision", args.precision, *box]
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=os.environ.get("TMPDIR", "/tmp"))
         path = os.path.join(d, "run_counter_collection.csv")
@@ -181,6 +181,50 @@ def trt_run(steps=20):
             "T_min": info["T_min"], "T_max": info["T_max"], "radiation_energy": info["radiation_energy"]}
 
 
+def config3_run(seed=0, reps=3, instances=12):
+    """Uneven batch in one launch: 12 independent 3D problems, hex meshes n^3
+    with n drawn from {16..48} by seed, interior nodes jittered by +-0.05h
+    (with S12, 0.2h and some 0.1h meshes give cyclic sweep graphs, where the
+    reference aborts), DG p=2
+    (27 dofs), S12 (168 dirs), lognormal sigma_t (log-sd 1), FP64 and FP32 LU.
+    Throughput of batched sweeps with an isotropic unit source."""
+    import allium_amd as A
+    from allium_amd import problems as P
+
+    rng = np.random.default_rng(seed)
+    ns = [int(v) for v in rng.integers(16, 49, instances)]
+    t0 = time.perf_counter()
+    ops, isos, psis = [], [], []
+    for i, n in enumerate(ns):
+        nel = (n, n, n)
+        r = np.random.default_rng(seed * 1000 + i)
+        verts = P.jittered_vertices(nel, 0.05, r)
+        sig = P.lognormal_sigma(n ** 3, 1.0, r).reshape(1, -1)
+        op = A.InverseAdvectionOperator(A.Mesh(nel, vertices=verts), 2, 12, sig)
+        ops.append(op)
+        isos.append(A.DeviceVector.from_host(np.ones(op.phi_size)))
+        psis.append(A.DeviceVector(op.size))
+    setup = time.perf_counter() - t0
+    batch = A.SweepBatch(ops)
+    unknowns = sum(op.size for op in ops)
+    out = {"meshes": ns, "unknowns_per_batch_sweep": unknowns, "setup_s": setup,
+           "levels_max": max(op.nlevels for op in ops), "levels_sum": sum(op.nlevels for op in ops)}
+    for name, prec in (("fp64", A.FP64), ("fp32_lu", A.FP32_REFINED)):
+        for op in ops:
+            op.set_precision(prec)
+        batch.Mult(None, psis, isos)
+        t_before, n_before = batch.timing()
+        for _ in range(reps):
+            batch.Mult(None, psis, isos)
+        t_after, n_after = batch.timing()
+        dt = (t_after - t_before) / (n_after - n_before)
+        out[name] = {"ms_per_batch_sweep": dt * 1e3, "unknowns_per_s": unknowns / dt}
+    for d in psis[:1]:
+        if not np.isfinite(d.to_host()).all():
+            out["error"] = "non-finite angular flux"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +240,7 @@ def main():
     ap.add_argument("--no-si", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="plain source iteration only (no network training)")
     ap.add_argument("--no-trt", action="store_true", help="skip the 20-step multigroup TRT run")
+    ap.add_argument("--no-config3", action="store_true", help="skip the batched p=2 jittered instances")
     ap.add_argument("--si-maxit", type=int, default=3000)
     ap.add_argument("--tuning", default=None, help="mode,block,lanes (skips the autotuner)")
     ap.add_argument("--box", default=None, help="bx,by,bz,dirs_per_group,threads,prefetch of box sweeps (modes 4/5)")
@@ -295,6 +340,12 @@ def main():
             log("SI:", out["source_iteration"])
         except Exception as exc:  # reported, not hidden
             out["source_iteration"] = {"error": str(exc)}
+    if not args.no_config3:
+        try:
+            out["batch_p2_config3"] = config3_run(args.seed)
+            log("config 3 batch:", out["batch_p2_config3"])
+        except Exception as exc:  # reported, not hidden
+            out["batch_p2_config3"] = {"error": str(exc)}
     if not args.no_trt:
         try:
             out["trt_config4"] = trt_run()

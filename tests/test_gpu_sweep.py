@@ -355,3 +355,74 @@ def test_errors_are_reported_not_aborted():
         Linv.set_launch(2)
     with pytest.raises(A.AlliumError, match="bad tuning"):
         Linv.set_tuning(0, 512, 1)
+
+
+P2_CASES = [
+    # (mesh, jitter, sn)
+    ((5, 4, 4), 0.1, 4),
+    ((4, 4, 3), 0.0, 4),
+    ((3, 3, 3), 0.1, 6),
+]
+
+
+@pytest.mark.parametrize("nel,jitter,sn", P2_CASES, ids=[f"{'x'.join(map(str, c[0]))}_j{c[1]}_S{c[2]}" for c in P2_CASES])
+def test_p2_cooperative_sweep_matches_oracle(nel, jitter, sn):
+    """3D p=2 (27 dofs): 32 lanes per cell; FP64 within 1e-12 of the oracle,
+    FP32 LU within 1e-4; level launches and their hipGraph."""
+    print(f"p2 case {nel} jitter {jitter} S{sn}", flush=True)
+    ne = int(np.prod(nel))
+    rng = np.random.default_rng(61)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+    v = P.jittered_vertices(nel, jitter, rng) if jitter else None
+    Linv, pr = _pair(nel, 2, sn, sigma, vertices=v)
+    q = pr.form_source(Q=1.0, inflow=np.ones((1, 6))) + rng.uniform(0, 0.1, pr.size)
+    ref = pr.sweep(q)
+    for mode in (0, 1):
+        Linv.set_launch(mode)
+        assert rel_err(Linv.Mult(q, np.empty_like(q)), ref) < FP64_TOL
+    with pytest.raises(A.AlliumError, match="cooperative"):
+        Linv.set_launch(3)
+    Linv.set_precision(A.FP32_REFINED)
+    e32 = rel_err(Linv.Mult(q, np.empty_like(q)), ref)
+    print(f"  fp32 LU rel err {e32:.2e}")
+    assert e32 < 1e-4
+
+
+def test_p2_batch_of_uneven_instances():
+    """Three uneven jittered p=2 meshes swept in one launch per level: equal to
+    sweeping each alone (same kernel) and to the oracle."""
+    print("p2 batch", flush=True)
+    rng = np.random.default_rng(71)
+    ops, prs, qs = [], [], []
+    for nel in ((4, 4, 4), (6, 3, 2), (3, 5, 4)):
+        ne = int(np.prod(nel))
+        sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+        Linv, pr = _pair(nel, 2, 4, sigma, vertices=P.jittered_vertices(nel, 0.1, rng))
+        ops.append(Linv)
+        prs.append(pr)
+        qs.append(pr.form_source(Q=1.0) + rng.uniform(0, 0.1, pr.size))
+    batch = A.SweepBatch(ops)
+    dq = [A.DeviceVector.from_host(q) for q in qs]
+    dpsi = [A.DeviceVector(q.size) for q in qs]
+    batch.Mult(dq, dpsi)
+    for Linv, pr, q, d in zip(ops, prs, qs, dpsi):
+        got = d.to_host()
+        assert rel_err(got, pr.sweep(q)) < FP64_TOL
+        assert np.array_equal(got, Linv.Mult(q, np.empty_like(q)))
+    t, n = batch.timing()
+    assert n == 1 and t > 0
+
+
+def test_p2_fp32_source_iteration_reaches_fp64_solution():
+    nel = (4, 4, 4)
+    ne = 64
+    rng = np.random.default_rng(81)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+    c = rng.uniform(0.5, 0.99, ne)
+    Linv, pr = _pair(nel, 2, 4, sigma, vertices=P.jittered_vertices(nel, 0.1, rng))
+    q = pr.form_source(Q=1.0)
+    phi64, _, it64, _ = A.source_iteration(Linv, c * sigma[0], q, abstol=1e-11)
+    phi32, _, it32, _ = A.source_iteration(Linv, c * sigma[0], q, abstol=1e-11, precision=A.FP32_REFINED)
+    ref_phi, _, _, _ = pr.source_iteration(c * sigma[0], q, abstol=1e-11)
+    assert rel_err(phi64, ref_phi) < 1e-10
+    assert rel_err(phi32, ref_phi) < 1e-9
