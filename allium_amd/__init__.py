@@ -357,25 +357,101 @@ class DiscreteToMoment:
         return psi
 
 
+def _dsa_desc(kappa=-1.0, bound_below=True, alpha=0.0, sym=-1.0, cg_reltol=1e-6, cg_abstol=0.0, cg_maxit=100,
+              smooth=2):
+    return _lib.DSADesc(float(kappa), int(bool(bound_below)), float(alpha), float(sym), float(cg_reltol),
+                        float(cg_abstol), int(cg_maxit), int(smooth))
+
+
+def host_mip_matrix(mesh, fe_order, quad, sigma_t, sigma_s, **desc):
+    """The scalar MIP diffusion matrix of DSA as CSR arrays (row_ptr, col, val), host only."""
+    lib = _lib.load()
+    qd, keep = _quad_desc(quad, mesh.dim)
+    dd = _dsa_desc(**desc)
+    st = _f64(np.broadcast_to(sigma_t, (mesh.Ne,)))
+    ss = _f64(np.broadcast_to(sigma_s, (mesh.Ne,)))
+    nnz = ctypes.c_longlong(0)
+    md = mesh.desc()
+    check(lib.allium_host_mip_matrix(ctypes.byref(md), fe_order, ctypes.byref(qd), _d(st), _d(ss), ctypes.byref(dd),
+                                     ctypes.byref(nnz), None, None, None))
+    n = (fe_order + 1) ** mesh.dim
+    rp = np.zeros(mesh.Ne * n + 1, dtype=np.int64)
+    col = np.zeros(nnz.value, dtype=np.int32)
+    val = np.zeros(nnz.value)
+    check(lib.allium_host_mip_matrix(ctypes.byref(md), fe_order, ctypes.byref(qd), _d(st), _d(ss), ctypes.byref(dd),
+                                     ctypes.byref(nnz), rp.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                     col.ctypes.data_as(_lib.c_ip), _d(val)))
+    return rp, col, val
+
+
+class DSA:
+    """Diffusion synthetic acceleration with the scalar MIP operator on the GPU
+    (DiffusionSyntheticAccelerationOperator, src/transport_op.hpp:38-59, as
+    chive's "scalar mip" preconditioner, exe/chive.cpp:663-682).
+
+    ``Mult(x, y)``: y = x + K^{-1} Ms x on device vectors; ``solve(b)``: K^{-1} b
+    for host arrays.  Options: kappa (< 0: |kappa| (p+1)^2), bound_below,
+    alpha (<= 0: from the quadrature), sym, cg_reltol, cg_abstol, cg_maxit,
+    smooth (damped Jacobi sweeps per multigrid level and direction)."""
+
+    def __init__(self, Linv, sigma_s, **desc):
+        self.Linv = Linv
+        self._lib = Linv._lib
+        self._desc = _dsa_desc(**desc)
+        h = ctypes.c_void_p()
+        check(self._lib.allium_dsa_create(Linv._h, _d(_f64(np.broadcast_to(sigma_s, (Linv.mesh.Ne,)))),
+                                          ctypes.byref(self._desc), ctypes.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self._lib.allium_dsa_destroy(h)
+            self.handle = None
+
+    def Mult(self, x, y):
+        check(self._lib.allium_dsa_mult_device(self.handle, x.ptr, y.ptr, None))
+        return y
+
+    def solve(self, b):
+        x = np.empty(self.Linv.phi_size)
+        check(self._lib.allium_dsa_solve(self.handle, _d(_f64(b)), _d(x)))
+        return x
+
+    def info(self):
+        inf = _lib.DSAInfo()
+        check(self._lib.allium_dsa_get_info(self.handle, ctypes.byref(inf)))
+        nl = inf.levels
+        return dict(levels=nl, level_size=list(inf.level_size[:nl]), omega=list(inf.omega[:nl]),
+                    dense_coarsest=bool(inf.dense_coarsest), last_cg_iterations=inf.last_cg_iterations,
+                    last_cg_norm=inf.last_cg_norm, applications=inf.applications, cg_iterations=inf.cg_iterations,
+                    seconds=inf.seconds)
+
+
 def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000, precision=FP64,
-                     want_psi=True, accel=None, history=None, prolong=1, patience=2):
+                     want_psi=True, accel=None, history=None, prolong=1, patience=2, fallback=None):
     """chive's SI: returns (phi, psi, iterations, norms) (exe/chive.cpp:700-735).
 
-    ``accel``: a cnn.FluxCorrector applied after every iteration (switched off
-    once the residual has failed to drop ``patience`` times in a row);
-    ``prolong`` 1 spreads its element corrections continuously over the
-    vertices (p = 1), 0 keeps them element-constant.  ``history``: a
-    (maxit, 2, Ne) float64 array that receives, for every iteration but the
-    last, the element-averaged scalar flux after it and the element-averaged
-    change it made before the correction."""
+    ``accel``: a DSA object (the SLISolver preconditioner: every iteration is
+    corrected and the norm is that of the corrected change), or a
+    cnn.FluxCorrector applied after every iteration (switched off once the
+    residual has failed to drop ``patience`` times in a row, after which
+    ``fallback``, a DSA object or None, takes over); ``prolong`` 1 spreads the
+    network's element corrections continuously over the vertices (p = 1), 0
+    keeps them element-constant.  ``history``: a (maxit, 2, Ne) float64 array
+    that receives, for every iteration but the last, the element-averaged
+    scalar flux after it and the element-averaged change it made before any
+    correction."""
     lib = Linv._lib
     hist_p = None
     if history is not None:
         if history.dtype != np.float64 or not history.flags.c_contiguous or history.size < 2 * maxit * Linv.mesh.Ne:
             raise ValueError("history must be a contiguous float64 array of maxit x 2 x Ne")
         hist_p = _d(history)
-    opt = _lib.SIOptions(abstol, reltol, int(maxit), int(precision), 0 if accel is None else 1,
-                         None if accel is None else accel.handle, int(prolong), int(patience), hist_p)
+    kind = 0 if accel is None else (2 if isinstance(accel, DSA) else 1)
+    opt = _lib.SIOptions(abstol, reltol, int(maxit), int(precision), kind,
+                         None if accel is None else accel.handle, None if fallback is None else fallback.handle,
+                         int(prolong), int(patience), hist_p)
     phi = np.empty(Linv.phi_size)
     psi = np.empty(Linv.size) if want_psi else None
     norms = np.zeros(maxit + 1)

@@ -1,4 +1,4 @@
-"""ctypes declarations of liballium_amd.so (include/allium_sweep.h, include/allium_device.h).
+"""ctypes declarations of liballium_amd.so (include/allium_*.h).
 
 The library is built in-tree by ``allium_amd/build.py``.  There is no fallback:
 if the shared library is missing every call raises.
@@ -34,7 +34,21 @@ class SweepInfo(ctypes.Structure):
 class SIOptions(ctypes.Structure):
     _fields_ = [("abstol", ctypes.c_double), ("reltol", ctypes.c_double), ("maxit", ctypes.c_int),
                 ("precision", ctypes.c_int), ("accel", ctypes.c_int), ("accel_handle", c_vp),
-                ("accel_prolong", ctypes.c_int), ("accel_patience", ctypes.c_int), ("history", c_dp)]
+                ("accel_fallback", c_vp), ("accel_prolong", ctypes.c_int), ("accel_patience", ctypes.c_int),
+                ("history", c_dp)]
+
+
+class DSADesc(ctypes.Structure):
+    _fields_ = [("kappa", ctypes.c_double), ("bound_below", ctypes.c_int), ("alpha", ctypes.c_double),
+                ("sym", ctypes.c_double), ("cg_reltol", ctypes.c_double), ("cg_abstol", ctypes.c_double),
+                ("cg_maxit", ctypes.c_int), ("smooth", ctypes.c_int)]
+
+
+class DSAInfo(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int), ("level_size", ctypes.c_longlong * 16), ("omega", ctypes.c_double * 16),
+                ("dense_coarsest", ctypes.c_int), ("last_cg_iterations", ctypes.c_int),
+                ("last_cg_norm", ctypes.c_double), ("applications", ctypes.c_long), ("cg_iterations", ctypes.c_long),
+                ("seconds", ctypes.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -93,6 +107,17 @@ SIGNATURES = {
                                                   c_vp, ctypes.POINTER(ctypes.c_float), c_dp]),
     "allium_cnn_train_step_device": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                     c_vp, c_vp, ctypes.c_float, c_dp]),
+    # allium_dsa.h
+    "allium_dsa_default_desc": (None, [ctypes.POINTER(DSADesc)]),
+    "allium_dsa_create": (ctypes.c_int, [c_vp, c_dp, ctypes.POINTER(DSADesc), ctypes.POINTER(c_vp)]),
+    "allium_dsa_destroy": (None, [c_vp]),
+    "allium_dsa_mult_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "allium_dsa_solve_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "allium_dsa_solve": (ctypes.c_int, [c_vp, c_dp, c_dp]),
+    "allium_dsa_get_info": (ctypes.c_int, [c_vp, ctypes.POINTER(DSAInfo)]),
+    "allium_host_mip_matrix": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc), c_dp,
+                                              c_dp, ctypes.POINTER(DSADesc), ctypes.POINTER(ctypes.c_longlong),
+                                              ctypes.POINTER(ctypes.c_longlong), c_ip, c_dp]),
     # allium_trt.h (the descriptor/info structs are declared in allium_amd/trt.py)
     "allium_trt_create": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc), c_vp,
                                          ctypes.POINTER(c_vp)]),

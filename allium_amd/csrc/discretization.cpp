@@ -512,6 +512,178 @@ bool assemble(const Mesh &m, int p, int lump, ElementData &ed, std::string &err)
 }
 
 // ---------------------------------------------------------------------------
+// scalar MIP diffusion for DSA (src/moment_discretization.cpp:73-109;
+// MIPDiffusionIntegrator::AssembleFaceMatrix, src/moment_integrators.cpp:3-195)
+// ---------------------------------------------------------------------------
+bool assemble_mip(const Mesh &m, int p, const double *sigma_t, const double *sigma_a, const MipParams &prm,
+                  MipMatrix &K, std::string &err)
+{
+    if (p < 0 || p > 2) { err = "fe_order must be 0, 1 or 2"; return false; }
+    const int dim = m.dim;
+    int n = 1;
+    for (int d = 0; d < dim; d++) n *= p + 1;
+    const int ns = 2 * dim, nc = 1 << dim;
+    for (int e = 0; e < m.Ne; e++)
+        if (!(sigma_t[e] > 0) || !std::isfinite(sigma_a[e])) { err = "MIP needs sigma_t > 0 and finite sigma_a"; return false; }
+    K.n = n; K.nside = ns; K.nc = nc;
+    K.Kd.assign((size_t)m.Ne * n * n, 0.0);
+    K.Ko.assign((size_t)m.Ne * ns * n * n, 0.0);
+    K.nb.assign((size_t)m.Ne * ns, -1);
+    Basis B{dim, p, n, gll_nodes(p)};
+    // vertex -> node interpolation: node coordinates are GLL points on [0,1]
+    K.P.assign((size_t)n * nc, 0.0);
+    for (int i = 0; i < n; i++) {
+        const int idx[3] = {i % (p + 1), (i / (p + 1)) % (p + 1), i / ((p + 1) * (p + 1))};
+        for (int c = 0; c < nc; c++) {
+            double w = 1.0;
+            for (int d = 0; d < dim; d++) {
+                const double x = B.nodes[idx[d]];
+                w *= ((c >> d) & 1) ? x : 1.0 - x;
+            }
+            K.P[(size_t)i * nc + c] = w;
+        }
+    }
+    for (int e = 0; e < m.Ne; e++)
+        for (int s = 0; s < ns; s++) {
+            const int f = m.face_of(e, s);
+            const int o = (m.f_e1[f] == e) ? m.f_e2[f] : m.f_e1[f];
+            K.nb[(size_t)e * ns + s] = (m.f_e2[f] >= 0) ? o : -1;
+        }
+    // DiffusionIntegrator and MassIntegrator rules: order 2p + dim - 1
+    const TensorRule r_df = tensor_rule(dim, (prm.lump & 2) ? corners() : gauss(npts(2 * p + dim - 1)));
+    const TensorRule r_ms = tensor_rule(dim, (prm.lump & 1) ? corners() : gauss(npts(2 * p + dim - 1)));
+    // face integrators: order 2p
+    const TensorRule r_fc = tensor_rule(dim - 1, (prm.lump & 4) ? corners() : gauss(npts(2 * p)));
+    parallel_for(m.Ne, [&](long b, long eend) {
+        std::vector<double> phi(n), dphi(3 * n), g(3 * n);
+        double J[9], adj[9];
+        for (long e = b; e < eend; e++) {
+            double *Kd = &K.Kd[(size_t)e * n * n];
+            const double D = 1.0 / (3.0 * sigma_t[e]);
+            for (size_t q = 0; q < r_df.w.size(); q++) {
+                elem_jacobian(m, (int)e, r_df.x[q].data(), J);
+                const double det = det_adj(dim, J, adj);
+                B.eval(r_df.x[q].data(), phi.data(), dphi.data());
+                // det * grad phi_i = dphi_i^T adj
+                for (int i = 0; i < n; i++)
+                    for (int d = 0; d < dim; d++) {
+                        double v = 0.0;
+                        for (int r = 0; r < dim; r++) v += dphi[3 * i + r] * adj[r * 3 + d];
+                        g[3 * i + d] = v;
+                    }
+                const double wq = r_df.w[q] * D / det;
+                for (int i = 0; i < n; i++)
+                    for (int j = 0; j < n; j++) {
+                        double v = 0.0;
+                        for (int d = 0; d < dim; d++) v += g[3 * i + d] * g[3 * j + d];
+                        Kd[i * n + j] += wq * v;
+                    }
+            }
+            for (size_t q = 0; q < r_ms.w.size(); q++) {
+                elem_jacobian(m, (int)e, r_ms.x[q].data(), J);
+                const double wq = det_adj(dim, J, adj) * r_ms.w[q] * sigma_a[e];
+                B.eval(r_ms.x[q].data(), phi.data(), dphi.data());
+                for (int i = 0; i < n; i++)
+                    for (int j = 0; j < n; j++) Kd[i * n + j] += wq * phi[i] * phi[j];
+            }
+        }
+    });
+    // faces, one axis at a time so that no two faces of a pass touch the same
+    // element block from different threads: faces are split by parity of
+    // their position along the axis
+    for (int ax = 0; ax < dim; ax++)
+        for (int par = 0; par < 2; par++) {
+            const int f0 = m.foff[ax], nfa = m.foff[ax + 1] - m.foff[ax];
+            parallel_for(nfa, [&](long b, long fend) {
+                std::vector<double> phi1(n), phi2(n), dphi1(3 * n), dphi2(3 * n), dn1(n), dn2(n);
+                std::vector<double> E((size_t)4 * n * n), Jm((size_t)4 * n * n);
+                double J1[9], J2[9], adj1[9], adj2[9], xi1[3], xi2[3], N[3];
+                const int n2 = 2 * n;
+                for (long fl = b; fl < fend; fl++) {
+                    const int f = f0 + (int)fl;
+                    const int *ext = m.fext[ax];
+                    const int c[3] = {(int)(fl % ext[0]), (int)((fl / ext[0]) % ext[1]), (int)(fl / ((long)ext[0] * ext[1]))};
+                    if ((c[ax] & 1) != par) continue;
+                    const int e1 = m.f_e1[f], e2 = m.f_e2[f];
+                    if (e2 < 0) {
+                        // BoundaryMassIntegrator(alpha) on inflow (Marshak) sides
+                        const int side = m.f_bside[f];
+                        if (m.bc[side] != 0) continue;
+                        double *Kd = &K.Kd[(size_t)e1 * n * n];
+                        for (size_t q = 0; q < r_fc.w.size(); q++) {
+                            side_point(dim, ax, side & 1, r_fc.x[q].data(), xi1);
+                            const double ws = r_fc.w[q] * side_ortho(m, e1, ax, side & 1, xi1, N) * prm.alpha;
+                            B.eval(xi1, phi1.data(), dphi1.data());
+                            for (int i = 0; i < n; i++)
+                                for (int j = 0; j < n; j++) Kd[i * n + j] += ws * phi1[i] * phi1[j];
+                        }
+                        continue;
+                    }
+                    std::fill(E.begin(), E.end(), 0.0);
+                    std::fill(Jm.begin(), Jm.end(), 0.0);
+                    const double D1 = 1.0 / (3.0 * sigma_t[e1]), D2 = 1.0 / (3.0 * sigma_t[e2]);
+                    for (size_t q = 0; q < r_fc.w.size(); q++) {
+                        side_point(dim, ax, 1, r_fc.x[q].data(), xi1);
+                        side_point(dim, ax, 0, r_fc.x[q].data(), xi2);
+                        const double area = side_ortho(m, e1, ax, 1, xi1, N);   // |N|, N out of e1
+                        elem_jacobian(m, e1, xi1, J1);
+                        elem_jacobian(m, e2, xi2, J2);
+                        const double det1 = det_adj(dim, J1, adj1), det2 = det_adj(dim, J2, adj2);
+                        B.eval(xi1, phi1.data(), dphi1.data());
+                        B.eval(xi2, phi2.data(), dphi2.data());
+                        const double w = r_fc.w[q];
+                        const double w1 = w * D1 / (2.0 * det1), w2 = w * D2 / (2.0 * det2);
+                        // adj * N (J^{-1} N times det)
+                        double a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
+                        for (int r = 0; r < dim; r++)
+                            for (int mm = 0; mm < dim; mm++) { a1[r] += adj1[r * 3 + mm] * N[mm]; a2[r] += adj2[r * 3 + mm] * N[mm]; }
+                        for (int j = 0; j < n; j++) {
+                            double v1 = 0.0, v2 = 0.0;
+                            for (int r = 0; r < dim; r++) { v1 += dphi1[3 * j + r] * a1[r]; v2 += dphi2[3 * j + r] * a2[r]; }
+                            dn1[j] = w1 * v1;
+                            dn2[j] = w2 * v2;
+                        }
+                        for (int i = 0; i < n; i++)
+                            for (int j = 0; j < n; j++) {
+                                E[(size_t)i * n2 + j] += phi1[i] * dn1[j];
+                                E[(size_t)i * n2 + n + j] += phi1[i] * dn2[j];
+                                E[(size_t)(n + i) * n2 + j] -= phi2[i] * dn1[j];
+                                E[(size_t)(n + i) * n2 + n + j] -= phi2[i] * dn2[j];
+                            }
+                        if (prm.kappa != 0.0) {
+                            double wq = (w1 + w2) * area * area * prm.kappa;
+                            wq = std::max(wq, prm.lower * w * area);
+                            for (int i = 0; i < n; i++)
+                                for (int j = 0; j < n; j++) {
+                                    Jm[(size_t)i * n2 + j] += wq * phi1[i] * phi1[j];
+                                    Jm[(size_t)i * n2 + n + j] -= wq * phi1[i] * phi2[j];
+                                    Jm[(size_t)(n + i) * n2 + j] -= wq * phi2[i] * phi1[j];
+                                    Jm[(size_t)(n + i) * n2 + n + j] += wq * phi2[i] * phi2[j];
+                                }
+                        }
+                    }
+                    // face matrix = sym E^T - E + J, scattered into the element blocks
+                    double *K11 = &K.Kd[(size_t)e1 * n * n], *K22 = &K.Kd[(size_t)e2 * n * n];
+                    double *K12 = &K.Ko[((size_t)e1 * ns + 2 * ax + 1) * n * n];
+                    double *K21 = &K.Ko[((size_t)e2 * ns + 2 * ax) * n * n];
+                    for (int i = 0; i < n2; i++)
+                        for (int j = 0; j < n2; j++) {
+                            const double v = prm.sym * E[(size_t)j * n2 + i] - E[(size_t)i * n2 + j] + Jm[(size_t)i * n2 + j];
+                            const int ii = i % n, jj = j % n;
+                            if (i < n && j < n) K11[ii * n + jj] += v;
+                            else if (i < n) K12[ii * n + jj] += v;
+                            else if (j < n) K21[ii * n + jj] += v;
+                            else K22[ii * n + jj] += v;
+                        }
+                }
+            });
+        }
+    for (double v : K.Kd)
+        if (!std::isfinite(v)) { err = "non-finite MIP matrix"; return false; }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // sweep graph levels (src/sweep.cpp:155-208 edges; 308-322, 477-512 Kahn)
 // ---------------------------------------------------------------------------
 namespace {

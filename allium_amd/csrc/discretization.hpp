@@ -64,6 +64,31 @@ bool assemble(const Mesh &m, int p, int lump, ElementData &ed, std::string &err)
 // phi[q * n + i] = basis function i at point q, w[q] = weight times element volume
 void uniform_element_rule(const Mesh &m, int p, int order, std::vector<double> &phi, std::vector<double> &w);
 
+// Scalar interior-penalty (MIP) diffusion matrix of diffusion synthetic
+// acceleration in element-block form (InteriorPenaltyDiscretization::GetOperator,
+// src/moment_discretization.cpp:73-109, with MIPDiffusionIntegrator,
+// src/moment_integrators.cpp:3-195):
+//   K = DiffusionIntegrator(1/(3 sigma_t)) + MassIntegrator(sigma_a)
+//     + interior faces: -<{D grad u.n},[v]> + sym <{D grad v.n},[u]> + <pen [u],[v]>,
+//       pen = max(kappa {D/h}, lower) per face point
+//     + inflow (Marshak) boundary faces: alpha <u, v>.
+struct MipParams {
+    double kappa = 4.0;   // penalty factor; 0 turns the penalty off (lower bound too)
+    double lower = 0.0;   // penalty lower bound (chive: alpha / 2)
+    double sym = -1.0;    // sigma of the integrator: -1 symmetric interior penalty
+    double alpha = 0.5;   // Marshak boundary coefficient
+    int lump = 0;         // bit 0 mass, bit 1 diffusion, bit 2 face terms on the corner rule
+};
+struct MipMatrix {
+    int n = 0, nside = 0, nc = 0;
+    std::vector<double> Kd;   // Ne x n x n, row-major
+    std::vector<double> Ko;   // Ne x nside x n x n: rows of e, columns of nb[e][s]
+    std::vector<int> nb;      // Ne x nside, -1 on the boundary
+    std::vector<double> P;    // n x nc (nc = 2^dim): DG node values of the d-linear vertex functions
+};
+bool assemble_mip(const Mesh &m, int p, const double *sigma_t, const double *sigma_a, const MipParams &prm,
+                  MipMatrix &K, std::string &err);
+
 // Sweep plan: level of every node e*Nomega+a and the cells grouped by level.
 struct SweepPlan {
     int nlevels = 0;

@@ -14,8 +14,10 @@
 #include "../../include/allium_device.h"
 #include "../../include/allium_sweep.h"
 #include "cnn_internal.hpp"
+#include "dsa_internal.hpp"
 #include "device.hpp"
 #include "discretization.hpp"
+#include "sweep_handle.hpp"
 
 namespace {
 thread_local std::string g_last_error = "no error";
@@ -45,52 +47,10 @@ hipError_t upload(std::vector<void *> &owned, T **dst, const T *src, size_t coun
     return hipSuccess;
 }
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) hipFree(p); }
-    hipError_t ensure(size_t b)
-    {
-        if (bytes >= b) return hipSuccess;
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, b);
-        if (e == hipSuccess) bytes = b;
-        return e;
-    }
-    double *d() const { return (double *)p; }
-};
 }  // namespace
 
 void allium_cnn_internal::set_last_error(const std::string &msg) { g_last_error = msg; }
 
-struct allium_sweep {
-    allium::Mesh mesh;
-    allium::Quadrature quad;
-    allium::ElementData ed;
-    allium::SweepPlan plan;
-    allium::DeviceSweep ds;
-    int G = 1;
-    long psi_size = 0, phi_size = 0;
-    std::vector<double> sigma;
-    double *d_sigma = nullptr;
-    // matrices kept for host-side rebuilds
-    std::vector<double> Kst;
-    // scratch for host-pointer entry points and the source iteration
-    DevBuf a, b, c, x, y, iso, coef, part, q, r, wavg, havg;
-    hipStream_t stream = nullptr;
-    ~allium_sweep()
-    {
-        for (void *p : ds.owned) hipFree(p);
-        if (ds.box_buf) hipFree(ds.box_buf);
-        if (ds.graph_exec) hipGraphExecDestroy(ds.graph_exec);
-        if (ds.graph_stream) hipStreamDestroy(ds.graph_stream);
-        if (ds.ev0) hipEventDestroy(ds.ev0);
-        if (ds.ev1) hipEventDestroy(ds.ev1);
-        if (stream) hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 
@@ -867,14 +827,23 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
     int it = 0;
     // learned correction and the per-iteration history both use element averages
     allium_cnn *cnn = nullptr;
+    allium_dsa *dsa = nullptr;
     if (opt->accel == 1) {
         if (!opt->accel_handle) return fail(ALLIUM_ERR_ARG, "accel 1 needs an allium_cnn_t in accel_handle");
         if (h->mesh.dim != 3 || !h->mesh.uniform || h->G != 1)
             return fail(ALLIUM_ERR_UNSUPPORTED, "the learned correction needs a uniform 3D mesh and one group");
         cnn = static_cast<allium_cnn *>(opt->accel_handle);
+        if (opt->accel_fallback) dsa = static_cast<allium_dsa *>(opt->accel_fallback);
+    } else if (opt->accel == 2) {
+        if (!opt->accel_handle) return fail(ALLIUM_ERR_ARG, "accel 2 needs an allium_dsa_t in accel_handle");
+        dsa = static_cast<allium_dsa *>(opt->accel_handle);
     } else if (opt->accel != 0) {
         return fail(ALLIUM_ERR_ARG, "unknown acceleration");
     }
+    if (dsa && allium_dsa_internal::handle_of(dsa) != h)
+        return fail(ALLIUM_ERR_ARG, "the DSA object was built on another sweep handle");
+    // DSA runs from the first iteration (accel 2) or once the learned correction is off
+    bool dsa_on = dsa && !cnn;
     if (opt->history && h->G != 1) return fail(ALLIUM_ERR_UNSUPPORTED, "iteration history needs one group");
     const int n = h->ed.n;
     const long Ne = h->mesh.Ne;
@@ -888,8 +857,9 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         HIP_TRY(h->wavg.ensure(sizeof(double) * n));
         HIP_TRY(hipMemcpy(h->wavg.p, w.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     }
-    if (cnn) HIP_TRY(h->r.ensure(sizeof(double) * np));
-    if (cnn) HIP_TRY(hipMemsetAsync(h->r.p, 0, sizeof(double) * np, s));
+    const bool want_r = cnn || dsa;
+    if (want_r) HIP_TRY(h->r.ensure(sizeof(double) * np));
+    if (want_r) HIP_TRY(hipMemsetAsync(h->r.p, 0, sizeof(double) * np, s));
     if (opt->history) HIP_TRY(h->havg.ensure(sizeof(double) * Ne));
     if (cnn && (opt->accel_prolong < 0 || opt->accel_prolong > 1 || (opt->accel_prolong == 1 && n != 8)))
         return fail(ALLIUM_ERR_ARG, "accel_prolong must be 0, or 1 with p = 1");
@@ -904,6 +874,7 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
             fails = (it > 1 && !(nrm < prev_nrm)) ? fails + 1 : 0;
             if (fails >= patience) {
                 cnn_on = false;
+                dsa_on = dsa != nullptr;
             } else {
                 int rc2 = allium_cnn_internal::correct(cnn, h->mesh.nel[2], h->mesh.nel[1], h->mesh.nel[0], n,
                                                        h->wavg.d(), h->r.d(), h->d_sigma, h->coef.d(), h->x.d(), s,
@@ -928,6 +899,17 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         }
         return ALLIUM_OK;
     };
+    // ||phi_{k+1} - phi_k|| from the update's partial sums, or with DSA on the
+    // preconditioned change z = r + K^{-1} Ms r (SLISolver with a preconditioner)
+    auto residual_norm = [&](double &nrm) -> int {
+        if (dsa_on) return allium_dsa_internal::correct(dsa, h->r.d(), h->x.d(), s, &nrm);
+        HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        nrm = 0.0;
+        for (int i = 0; i < nb; i++) nrm += hp[i];
+        nrm = std::sqrt(nrm);
+        return ALLIUM_OK;
+    };
     if (opt->precision == ALLIUM_FP32_REFINED) {
         // defect correction: psi += L32^{-1} (q + D^T Ms phi - L psi), residuals in FP64
         HIP_TRY(h->c.ensure(sizeof(double) * nps));
@@ -944,12 +926,10 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
             HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->y.d(), s));
             // r = D psi_{k+1} - phi_k ; phi_{k+1} = D psi_{k+1}
             HIP_TRY(allium::launch_si_update(np, h->y.d(), nullptr, h->x.d(), h->part.d(), s,
-                                             cnn ? h->r.d() : nullptr));
-            HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
+                                             want_r ? h->r.d() : nullptr));
             double nrm = 0.0;
-            for (int i = 0; i < nb; i++) nrm += hp[i];
-            nrm = std::sqrt(nrm);
+            rc = residual_norm(nrm);
+            if (rc) break;
             if (norms) norms[it - 1] = nrm;
             if (!std::isfinite(nrm)) { rc = fail(ALLIUM_ERR_NUMERIC, "non-finite residual"); break; }
             if (it == 1) r0 = std::max(nrm * opt->reltol, opt->abstol);
@@ -973,12 +953,10 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
             if (rc) break;
             HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->y.d(), s));
             HIP_TRY(allium::launch_si_update(np, h->b.d(), h->y.d(), h->x.d(), h->part.d(), s,
-                                             cnn ? h->r.d() : nullptr));
-            HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
+                                             want_r ? h->r.d() : nullptr));
             double nrm = 0.0;
-            for (int i = 0; i < nb; i++) nrm += hp[i];
-            nrm = std::sqrt(nrm);
+            rc = residual_norm(nrm);
+            if (rc) break;
             if (norms) norms[it - 1] = nrm;
             if (!std::isfinite(nrm)) { rc = fail(ALLIUM_ERR_NUMERIC, "non-finite residual"); break; }
             if (it == 1) r0 = std::max(nrm * opt->reltol, opt->abstol);
@@ -1110,7 +1088,9 @@ int allium_sweep_timing(allium_sweep_t h, double *sweep_seconds, long *sweeps)
 }
 
 // ---- host-side setup queries ------------------------------------------------
-static int host_quad(const allium_quad_desc *quad, int dim, allium::Quadrature &q)
+}  // extern "C"
+
+int allium_internal_quad(const allium_quad_desc *quad, int dim, allium::Quadrature &q)
 {
     std::string why;
     if (quad->nomega > 0) {
@@ -1125,6 +1105,8 @@ static int host_quad(const allium_quad_desc *quad, int dim, allium::Quadrature &
     if (!allium::level_symmetric(quad->order, dim, q, why)) return fail(ALLIUM_ERR_ARG, why);
     return ALLIUM_OK;
 }
+
+extern "C" {
 
 int allium_host_quadrature(int order, int dim, double *omega, double *weights, int *nomega)
 {
@@ -1150,7 +1132,7 @@ int allium_host_levels(const allium_mesh_desc *mesh, const allium_quad_desc *qua
     std::string why;
     if (!allium::build_mesh(mesh->dim, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, m, why))
         return fail(ALLIUM_ERR_ARG, why);
-    int rc = host_quad(quad, mesh->dim, q);
+    int rc = allium_internal_quad(quad, mesh->dim, q);
     if (rc) return rc;
     int st = allium::build_plan(m, q, plan, why);
     if (st == -2) return fail(ALLIUM_ERR_NOT_DAG, why);
@@ -1171,7 +1153,7 @@ int allium_host_form_source(const allium_mesh_desc *mesh, int fe_order, const al
     std::string why;
     if (!allium::build_mesh(mesh->dim, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, m, why))
         return fail(ALLIUM_ERR_ARG, why);
-    int rc = host_quad(quad, mesh->dim, q);
+    int rc = allium_internal_quad(quad, mesh->dim, q);
     if (rc) return rc;
     if (!allium::assemble(m, fe_order, 0, ed, why)) return fail(ALLIUM_ERR_ARG, why);
     allium::form_source(m, ed, q, G, Q, inflow, source);

@@ -1,0 +1,227 @@
+"""CPU oracle of diffusion synthetic acceleration (test infrastructure only).
+
+Restates, in FP64 numpy/scipy:
+  * the scalar MIP diffusion matrix of chive's "scalar mip" preconditioner
+    (InteriorPenaltyDiscretization::GetOperator, src/moment_discretization.cpp:73-109;
+    MIPDiffusionIntegrator::AssembleFaceMatrix, src/moment_integrators.cpp:3-195;
+    penalty factor and lower bound as exe/chive.cpp:663-682 sets them;
+    ComputeAlpha, src/angular_quadrature.cpp:266-276) on structured quad/hex
+    meshes, returned as a scipy CSR matrix in the phi layout;
+  * DiffusionSyntheticAccelerationOperator (src/transport_op.hpp:38-59) inside
+    SLISolver (src/linalg.cpp:103-153): z = r + K^{-1} Ms r, x += z, norm ||z||,
+    with K solved directly (scipy's sparse LU), as chive's "direct" inner solver.
+Pinned in tests/test_dsa_oracle.py by closed forms (row sums: absorption plus
+Marshak boundary mass; the product with a linear function: the boundary flux
+D du/dn) and by the reference's thick-diffusion-limit test deck
+(test/chive/tdl.lua, test/chive/test_tdl.py: fewer than 12 iterations).
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from . import gauss_rule, gll_nodes
+
+
+def compute_alpha(omega, weights):
+    """sum_a |Omega_a . e_x| w_a / sum_a w_a."""
+    omega = np.asarray(omega, float)
+    w = np.asarray(weights, float)
+    return float(np.sum(np.abs(omega[:, 0]) * w) / np.sum(w))
+
+
+def _lagrange(nodes, x):
+    """1D Lagrange basis on `nodes` at points x: values and derivatives (len(x), len(nodes))."""
+    k = len(nodes)
+    v = np.ones((len(x), k))
+    d = np.zeros((len(x), k))
+    for i in range(k):
+        for m in range(k):
+            if m == i:
+                continue
+            r = 1.0 / (nodes[i] - nodes[m])
+            d[:, i] = d[:, i] * (x - nodes[m]) * r + v[:, i] * r
+            v[:, i] *= (x - nodes[m]) * r
+    return v, d
+
+
+def _basis(dim, p, xi):
+    """Tensor GLL basis at reference points xi (nq, dim): phi (nq, n), grad (nq, n, dim)."""
+    nodes = gll_nodes(p) if p > 0 else np.array([0.5])
+    vals, ders = zip(*[_lagrange(nodes, xi[:, d]) for d in range(dim)])
+    nq, k = xi.shape[0], p + 1
+    idx = np.array(np.unravel_index(np.arange(k ** dim), (k,) * dim)[::-1]).T   # x fastest
+    phi = np.ones((nq, k ** dim))
+    grad = np.ones((nq, k ** dim, dim))
+    for d in range(dim):
+        phi *= vals[d][:, idx[:, d]]
+        for g in range(dim):
+            grad[:, :, g] *= (ders[d] if g == d else vals[d])[:, idx[:, d]]
+    return phi, grad
+
+
+def _rule(dim, order):
+    x, w = gauss_rule(order // 2 + 1)
+    if dim == 0:
+        return np.zeros((1, 0)), np.ones(1)
+    grids = np.meshgrid(*([x] * dim), indexing="ij")
+    wg = np.meshgrid(*([w] * dim), indexing="ij")
+    pts = np.stack([g.ravel() for g in grids[::-1]], -1)           # x fastest
+    wts = np.prod(np.stack([g.ravel() for g in wg]), axis=0)
+    return pts, wts
+
+
+class _Geometry:
+    def __init__(self, dim, nel, vertices):
+        self.dim = dim
+        self.nel = list(nel)[:dim]
+        self.nv = [n + 1 for n in self.nel]
+        self.vert = np.asarray(vertices, float).reshape(-1, dim)
+        self.Ne = int(np.prod(self.nel))
+
+    def coords(self, e):
+        c, r = [], e
+        for n in self.nel:
+            c.append(r % n)
+            r //= n
+        return c
+
+    def corners(self, e):
+        c = self.coords(e)
+        out = []
+        for k in range(2 ** self.dim):
+            b = [(k >> d) & 1 for d in range(self.dim)]
+            vi, s = 0, 1
+            for d in range(self.dim):
+                vi += (c[d] + b[d]) * s
+                s *= self.nv[d]
+            out.append(self.vert[vi])
+        return np.array(out)
+
+    def jac(self, e, xi):
+        """J[q, m, d] = dX_m / dxi_d of the d-linear map at points xi (nq, dim)."""
+        X = self.corners(e)
+        _, g = _basis(self.dim, 1, xi)                               # d-linear shape gradients
+        return np.einsum("km,qkd->qmd", X, g)
+
+
+def _face_normal(J, ax, hi):
+    """Area-weighted outward normal of the side (ax, hi) from the element Jacobians (nq, dim, dim)."""
+    dim = J.shape[1]
+    t = [d for d in range(dim) if d != ax]
+    if dim == 2:
+        N = np.stack([J[:, 1, t[0]], -J[:, 0, t[0]]], -1)
+    else:
+        N = np.cross(J[:, :, t[0]], J[:, :, t[1]])
+    s = np.einsum("qm,qm->q", N, J[:, :, ax])
+    sign = np.where((s > 0) == bool(hi), 1.0, -1.0)
+    return N * sign[:, None]
+
+
+def mip_matrix(dim, nel, vertices, p, sigma_t, sigma_s, omega, weights, bc=None, kappa=None, bound_below=True,
+               alpha=None, sym=-1.0):
+    """Scalar MIP diffusion matrix K (CSR, Ndof x Ndof in the phi layout)."""
+    geo = _Geometry(dim, nel, vertices)
+    sigma_t = np.asarray(sigma_t, float).reshape(-1)
+    sigma_a = sigma_t - np.asarray(sigma_s, float).reshape(-1)
+    D = 1.0 / (3.0 * sigma_t)
+    if kappa is None:
+        kappa = float((p + 1) ** 2)
+    if alpha is None:
+        alpha = compute_alpha(omega, weights)
+    lower = alpha / 2 if bound_below else 0.0
+    bc = list(bc) if bc is not None else [0] * (2 * dim)
+    n = (p + 1) ** dim
+    rows, cols, vals = [], [], []
+
+    def add(e1, e2, B):
+        r = np.repeat(np.arange(n) + e1 * n, n)
+        c = np.tile(np.arange(n) + e2 * n, n)
+        rows.append(r)
+        cols.append(c)
+        vals.append(B.ravel())
+
+    xv, wv = _rule(dim, 2 * p + dim - 1)
+    phi_v, g_v = _basis(dim, p, xv)
+    for e in range(geo.Ne):
+        J = geo.jac(e, xv)
+        det = np.linalg.det(J)
+        Jinv = np.linalg.inv(J)
+        grad = np.einsum("qnr,qrd->qnd", g_v, Jinv)                 # physical gradients
+        w = wv * np.abs(det)
+        B = D[e] * np.einsum("q,qid,qjd->ij", w, grad, grad) + sigma_a[e] * np.einsum("q,qi,qj->ij", w, phi_v, phi_v)
+        add(e, e, B)
+    uf, wf = _rule(dim - 1, 2 * p)
+    stride = [1, geo.nel[0], geo.nel[0] * (geo.nel[1] if dim > 1 else 1)]
+    for e1 in range(geo.Ne):
+        c = geo.coords(e1)
+        for ax in range(dim):
+            def side_pts(hi):
+                xi = np.zeros((len(wf), dim))
+                others = [d for d in range(dim) if d != ax]
+                for k, d in enumerate(others):
+                    xi[:, d] = uf[:, k]
+                xi[:, ax] = hi
+                return xi
+            for hi in (0, 1):
+                at_bdr = (c[ax] == 0 and hi == 0) or (c[ax] == geo.nel[ax] - 1 and hi == 1)
+                if at_bdr:
+                    if bc[2 * ax + hi] != 0:
+                        continue
+                    xi = side_pts(hi)
+                    N = _face_normal(geo.jac(e1, xi), ax, hi)
+                    ph, _ = _basis(dim, p, xi)
+                    add(e1, e1, alpha * np.einsum("q,qi,qj->ij", wf * np.linalg.norm(N, axis=1), ph, ph))
+                    continue
+                if hi == 0:
+                    continue                                          # each interior face once, from e1 = lower
+                e2 = e1 + stride[ax]
+                xi1, xi2 = side_pts(1), side_pts(0)
+                J1, J2 = geo.jac(e1, xi1), geo.jac(e2, xi2)
+                N = _face_normal(J1, ax, 1)
+                area = np.linalg.norm(N, axis=1)
+                ph1, g1 = _basis(dim, p, xi1)
+                ph2, g2 = _basis(dim, p, xi2)
+                gr1 = np.einsum("qnr,qrd->qnd", g1, np.linalg.inv(J1))
+                gr2 = np.einsum("qnr,qrd->qnd", g2, np.linalg.inv(J2))
+                # {D grad u . N} with the 1/2 average, per face point
+                dn1 = 0.5 * D[e1] * np.einsum("qnd,qd->qn", gr1, N) * wf[:, None]
+                dn2 = 0.5 * D[e2] * np.einsum("qnd,qd->qn", gr2, N) * wf[:, None]
+                v = np.concatenate([ph1, -ph2], axis=1)                  # [v] = v1 - v2
+                du = np.concatenate([dn1, dn2], axis=1)
+                E = np.einsum("qi,qj->ij", v, du)
+                pen = kappa * wf * area ** 2 * 0.5 * (D[e1] / np.abs(np.linalg.det(J1)) + D[e2] / np.abs(np.linalg.det(J2)))
+                if kappa != 0.0:
+                    pen = np.maximum(pen, lower * wf * area)
+                else:
+                    pen = np.zeros_like(pen)
+                Jm = np.einsum("q,qi,qj->ij", pen, v, v)
+                F = sym * E.T - E + Jm
+                add(e1, e1, F[:n, :n])
+                add(e1, e2, F[:n, n:])
+                add(e2, e1, F[n:, :n])
+                add(e2, e2, F[n:, n:])
+    N0 = geo.Ne * n
+    K = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N0, N0))
+    return K.tocsr()
+
+
+def dsa_source_iteration(pr, sigma_s, source, K, abstol=1e-8, reltol=0.0, maxit=1000):
+    """SLISolver on T = I - D L^{-1} D^T Ms with the DSA preconditioner
+    z = r + K^{-1} Ms r; returns (phi, iterations, norms)."""
+    solve = spla.factorized(sp.csc_matrix(K))
+    ms = lambda x: pr.mass(sigma_s, x)
+    b = pr.D(pr.sweep(source))
+    x = np.zeros_like(b)
+    norms = []
+    r0 = None
+    for it in range(1, maxit + 1):
+        r = b - x + pr.D(pr.sweep(pr.DT(ms(x))))
+        z = r + solve(ms(r))
+        x = x + z
+        nrm = float(np.sqrt(z @ z))
+        norms.append(nrm)
+        if it == 1:
+            r0 = max(nrm * reltol, abstol)
+        if nrm < r0:
+            break
+    return x, it, np.array(norms)

@@ -1,0 +1,177 @@
+"""CPU checks of diffusion synthetic acceleration: the MIP oracle against closed
+forms and the reference's thick-diffusion-limit deck, and the library's host
+assembly of the MIP matrix against the oracle.  No GPU compute is called."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import allium_amd as A
+import oracle
+from allium_amd import problems as P
+from oracle import mip_oracle as M
+from tests.helpers import ls_fixture
+
+
+def _uniform(dim, n):
+    nel = (n,) * dim
+    return nel, P.cartesian_vertices(nel)
+
+
+def test_alpha_is_half_for_level_symmetric_sets():
+    # sum |mu| w / sum w -> 1/2 for a set that integrates |mu| exactly
+    for order, dim in [(4, 2), (8, 3), (12, 3)]:
+        om, w = ls_fixture(order, dim)
+        a = M.compute_alpha(om, w)
+        assert 0.45 < a < 0.55
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (3, 1), (2, 2)])
+def test_row_sums_are_absorption_plus_marshak(dim, p):
+    """K 1 = sigma_a int phi_i + alpha int_{inflow sides} phi_i (gradients and
+    jumps of a constant vanish).  For p = 1 the closed forms are vol / 2^dim per
+    node and area / 2^(dim-1) per boundary node."""
+    n = 4
+    nel, vert = _uniform(dim, n)
+    rng = np.random.default_rng(1)
+    st = rng.uniform(0.5, 2.0, n ** dim)
+    ss = st * rng.uniform(0.1, 0.9, n ** dim)
+    om, w = ls_fixture(4, dim)
+    bc = [0, 1] + [0] * (2 * dim - 2)           # x-high side reflective: no Marshak term
+    K = M.mip_matrix(dim, nel, vert, p, st, ss, om, w, bc=bc)
+    alpha = M.compute_alpha(om, w)
+    rs = np.asarray(K.sum(axis=1)).ravel()
+    h = 1.0 / n
+    nn = (p + 1) ** dim
+    if p == 1:
+        expect = np.zeros_like(rs)
+        for e in range(n ** dim):
+            c = [(e // n ** d) % n for d in range(dim)]
+            for i in range(nn):
+                b = [(i >> d) & 1 for d in range(dim)]
+                v = (st[e] - ss[e]) * h ** dim / 2 ** dim
+                for ax in range(dim):
+                    for hi in (0, 1):
+                        if bc[2 * ax + hi] == 0 and b[ax] == hi and c[ax] == (n - 1) * hi:
+                            v += alpha * h ** (dim - 1) / 2 ** (dim - 1)
+                expect[e * nn + i] = v
+        np.testing.assert_allclose(rs, expect, rtol=1e-12, atol=1e-15)
+    else:
+        # totals: sigma_a-weighted volume plus alpha times the inflow boundary measure
+        tot = np.sum(st - ss) * h ** dim + alpha * (2 * dim - 1)
+        assert abs(rs.sum() - tot) < 1e-12 * tot
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (3, 1), (2, 2), (3, 2)])
+def test_linear_function_gives_boundary_flux(dim, p):
+    """With constant sigma_t, sigma_a = 0 and no Marshak term, K x_nodal is the
+    boundary flux int_{dD} D (du/dn) phi_i of u = x: +D int phi_i on x = 1,
+    -D int phi_i on x = 0 (the consistency terms cancel the interior flux)."""
+    n = 3
+    nel, vert = _uniform(dim, n)
+    st = np.full(n ** dim, 2.0)
+    om, w = ls_fixture(4, dim)
+    K = M.mip_matrix(dim, nel, vert, p, st, st, om, w, alpha=1e-300)   # alpha ~ 0
+    nodes = oracle.gll_nodes(p)
+    nn = (p + 1) ** dim
+    x = np.zeros(n ** dim * nn)
+    for e in range(n ** dim):
+        for i in range(nn):
+            x[e * nn + i] = ((e % n) + nodes[i % (p + 1)]) / n
+    y = K @ x
+    D = 1.0 / 6.0
+    # boundary integrals of the GLL basis on a face: tensor of 1D integrals
+    xg, wg = oracle.gauss_rule(p + 2)
+    lag = M._lagrange(nodes, xg)[0]
+    w1 = (wg[:, None] * lag).sum(0) / n            # int over one face edge of each 1D node function
+    expect = np.zeros_like(y)
+    for e in range(n ** dim):
+        cx = e % n
+        for i in range(nn):
+            ix = i % (p + 1)
+            rest = [(i // (p + 1) ** d) % (p + 1) for d in range(1, dim)]
+            face = np.prod([w1[r] for r in rest])
+            if cx == n - 1 and ix == p:
+                expect[e * nn + i] += D * face
+            if cx == 0 and ix == 0:
+                expect[e * nn + i] -= D * face
+    np.testing.assert_allclose(y, expect, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim,p,jitter", [(2, 1, 0.0), (3, 1, 0.0), (3, 1, 0.15), (2, 2, 0.2), (3, 2, 0.1)])
+def test_oracle_matrix_is_spd(dim, p, jitter):
+    n = 3
+    nel = (n,) * dim
+    rng = np.random.default_rng(3)
+    vert = P.jittered_vertices(nel, jitter, rng) if jitter else P.cartesian_vertices(nel)
+    st = P.lognormal_sigma(n ** dim, 1.0, rng)
+    ss = st * 0.99
+    om, w = ls_fixture(4, dim)
+    K = M.mip_matrix(dim, nel, vert, p, st, ss, om, w).toarray()
+    assert np.abs(K - K.T).max() < 1e-13 * np.abs(K).max()
+    assert np.linalg.eigvalsh(K).min() > 0
+
+
+@pytest.mark.parametrize("dim,p,jitter,sig_scale", [(2, 1, 0.0, 1.0), (3, 1, 0.0, 1.0), (3, 1, 0.15, 1.0),
+                                                    (2, 2, 0.2, 1.0), (3, 2, 0.1, 1.0), (2, 1, 0.0, 1e4),
+                                                    (3, 0, 0.0, 1.0)])
+def test_host_assembly_matches_oracle(dim, p, jitter, sig_scale):
+    """allium_host_mip_matrix (C++) against the numpy restatement; sigma_t =
+    1e4 drives every face onto the alpha/2 lower bound of the penalty."""
+    n = 4 if dim == 2 else 3
+    nel = (n,) * dim
+    rng = np.random.default_rng(7)
+    vert = P.jittered_vertices(nel, jitter, rng) if jitter else P.cartesian_vertices(nel)
+    st = P.lognormal_sigma(n ** dim, 1.0, rng) * sig_scale
+    ss = st * rng.uniform(0.5, 0.999, n ** dim)
+    bc = [0, 1] + [0] * (2 * dim - 2)
+    om, w = ls_fixture(4, dim)
+    mesh = A.Mesh(nel, vertices=vert, bc=bc)
+    rp, col, val = A.host_mip_matrix(mesh, p, (om, w), st, ss)
+    Kh = sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2).toarray()
+    Ko = M.mip_matrix(dim, nel, vert, p, st, ss, om, w, bc=bc).toarray()
+    assert np.abs(Kh - Ko).max() < 1e-12 * np.abs(Ko).max()
+
+
+def test_host_assembly_options():
+    """kappa, the lower bound, alpha and sym reach the assembly as the oracle reads them."""
+    nel, vert = _uniform(2, 4)
+    rng = np.random.default_rng(2)
+    st = P.lognormal_sigma(16, 1.0, rng)
+    om, w = ls_fixture(4, 2)
+    mesh = A.Mesh(nel, vertices=vert)
+    for opts, oopts in [(dict(kappa=9.0), dict(kappa=9.0)), (dict(kappa=-2.0), dict(kappa=8.0)),
+                        (dict(bound_below=False), dict(bound_below=False)), (dict(alpha=0.3), dict(alpha=0.3)),
+                        (dict(sym=1.0), dict(sym=1.0)), (dict(kappa=0.0), dict(kappa=0.0))]:
+        rp, col, val = A.host_mip_matrix(mesh, 1, (om, w), st, st * 0.5, **opts)
+        Kh = sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2).toarray()
+        Ko = M.mip_matrix(2, nel, vert, 1, st, st * 0.5, om, w, **oopts).toarray()
+        assert np.abs(Kh - Ko).max() < 1e-12 * np.abs(Ko).max(), opts
+
+
+def _tdl(eps=1e-4):
+    """test/chive/tdl.lua: 8x8 unit square, sigma_t = 1/eps, sigma_s = 1/eps - eps,
+    source eps (per steradian), vacuum, S4 level-symmetric, p = 1."""
+    nel = (8, 8)
+    vert = P.cartesian_vertices(nel)
+    om, w = ls_fixture(4, 2)
+    st = np.full(64, 1.0 / eps)
+    ss = np.full(64, 1.0 / eps - eps)
+    return nel, vert, om, w, st, ss, eps
+
+
+def test_tdl_deck_converges_in_under_12_iterations():
+    """The reference's chive.tdl.mip acceptance test (test/chive/test_tdl.py:
+    `assert it < 12` with abstol 1e-6, max_iter 50), with the scalar MIP DSA and
+    a direct inner solve.  Plain SI does not converge in 50 iterations there."""
+    nel, vert, om, w, st, ss, eps = _tdl()
+    pr = oracle.Problem(2, nel, vert, 1, om, w, st)
+    q = pr.form_source(Q=eps)
+    K = M.mip_matrix(2, nel, vert, 1, st, ss, om, w)
+    phi, it, norms = M.dsa_source_iteration(pr, ss, q, K, abstol=1e-6, maxit=50)
+    assert norms[-1] < 1e-6 and it < 12
+    _, _, it_plain, n_plain = pr.source_iteration(ss, q, abstol=1e-6, maxit=50)
+    assert it_plain == 50 and n_plain[-1] > 1e-6
+    # the DSA fixed point is the transport solution: plain SI from it stays put
+    b = pr.D(pr.sweep(q))
+    r = b - phi + pr.D(pr.sweep(pr.DT(pr.mass(ss, phi))))
+    assert np.linalg.norm(r) < 1e-5 * np.linalg.norm(phi)
