@@ -123,16 +123,160 @@ __device__ __forceinline__ void box_fetch(const SweepParams &p, const BoxPlan &b
     }
 }
 
-// Solve one cell whose inputs are loaded: traces and K[a] from LDS, the mass
-// and face-mass matrices as compile-time tensor products (uniform axis-aligned
-// boxes: Mu = vol * unit_mass, F_side = area * unit_face_mass); store
-// psi to HBM and the outflow traces to the planes (only when in.ok).  mid()
-// runs between the assembly of the local system and its LU and may overwrite
-// `in` (the next level's loads go there: issued after this cell's inputs were
-// consumed and before its psi stores, so their latency overlaps the LU).
+// Uniform p = 1 cells by fast diagonalisation.  On an axis-aligned box every
+// operator of the cell system is a Kronecker product of 1D Gauss-Lobatto
+// matrices: with the exact 1D mass m = [[1/3,1/6],[1/6,1/3]] and upwind
+// convection C+ = [[1/2,1/2],[-1/2,1/2]] (C- its mirror image),
+//   M^{-1} A = sum_d t_d (B_d on axis d) + sigma I,  t_d = |Omega_d| / h_d,
+// with B+ = m^{-1} C+ = [[3,1],[-3,1]] = W R W^{-1}, R = [[2,sqrt2],[-sqrt2,2]]
+// = 2 I + sqrt2 J, W+ = [[1,0],[-1,sqrt2]], W- = P W+ (P the swap).  So
+//   A psi = b  <=>  (alpha I + sum_d beta_d J_d) y = W^{-1} M^{-1} b,
+//   alpha = sigma + 2 sum_d t_d,  beta_d = sqrt2 t_d,  psi = W y,
+// and the commuting J_d (J^2 = -I) are diagonal after complexifying along x
+// (z = y0 - i y1 makes J_x multiplication by i) and splitting y and z into
+// (z0 -+ i z1): 2^(dim-1) complex divisions replace the N x N LU, and
+// M^{-1} maps an inflow face term to t_d (m^{-1} e_in) (x) u_d.
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void fd_solve_p1(const double (&b)[1 << DIM], const double (&u)[DIM][1 << (DIM - 1)],
+                                            const double (&t)[3], double sig, double inv_vol, T (&x)[1 << DIM])
+{
+    constexpr int N = 1 << DIM;
+    constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
+    T r[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] = (T)(b[i] * inv_vol);
+    // pairs along axis ax: (i, i + (1 << ax)) for i with bit ax clear
+    auto pass = [&](int ax, T a00, T a01, T a10, T a11) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            if (i & (1 << ax)) continue;
+            const T v0 = r[i], v1 = r[i + (1 << ax)];
+            r[i] = a00 * v0 + a01 * v1;
+            r[i + (1 << ax)] = a10 * v0 + a11 * v1;
+        }
+    };
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) pass(ax, (T)4, (T)-2, (T)-2, (T)4);
+    // inflow faces: r += t_ax (m^{-1} e_in) (x) u_ax; face index m runs over
+    // the other axes, the lower one fastest
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+        const bool neg = (OCT >> ax) & 1;
+        const T c0 = neg ? (T)-2 : (T)4, c1 = neg ? (T)4 : (T)-2;
+        const T ta = (T)t[ax];
+#pragma unroll
+        for (int m = 0; m < (1 << (DIM - 1)); ++m) {
+            int i = 0, k = 0;
+#pragma unroll
+            for (int o = 0; o < DIM; ++o)
+                if (o != ax) { i |= ((m >> k) & 1) << o; ++k; }
+            const T v = ta * (T)u[ax][m];
+            r[i] += c0 * v;
+            r[i | (1 << ax)] += c1 * v;
+        }
+    }
+    // y = W^{-1} r: W+^{-1} = [[1,0],[1/sqrt2,1/sqrt2]], W-^{-1} = [[0,1],[1/sqrt2,1/sqrt2]]
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+        if ((OCT >> ax) & 1) pass(ax, (T)0, (T)1, ir2, ir2);
+        else pass(ax, (T)1, (T)0, ir2, ir2);
+    }
+    T alpha = (T)sig, beta[3];
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+        alpha += (T)2 * (T)t[ax];
+        beta[ax] = r2 * (T)t[ax];
+    }
+    // complex z[j] = y[2j] - i y[2j+1] over the remaining axes (j = i1 + 2 i2)
+    constexpr int NZ = N / 2;
+    T zr[NZ], zi[NZ];
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) { zr[j] = r[2 * j]; zi[j] = -r[2 * j + 1]; }
+    // split axis k (bit k of j): (z0, z1) -> (p, m) = (z0 - i z1, z0 + i z1)
+    auto split = [&](int k) {
+#pragma unroll
+        for (int j = 0; j < NZ; ++j) {
+            if (j & (1 << k)) continue;
+            const int j1 = j | (1 << k);
+            const T ar = zr[j], ai = zi[j], br = zr[j1], bi = zi[j1];
+            zr[j] = ar + bi; zi[j] = ai - br;
+            zr[j1] = ar - bi; zi[j1] = ai + br;
+        }
+    };
+    // inverse: z0 = (p + m) / 2, z1 = i (p - m) / 2
+    auto merge = [&](int k) {
+#pragma unroll
+        for (int j = 0; j < NZ; ++j) {
+            if (j & (1 << k)) continue;
+            const int j1 = j | (1 << k);
+            const T pr = zr[j], pi = zi[j], mr = zr[j1], mi = zi[j1];
+            zr[j] = (T)0.5 * (pr + mr); zi[j] = (T)0.5 * (pi + mi);
+            zr[j1] = (T)0.5 * (mi - pi); zi[j1] = (T)0.5 * (pr - mr);
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < DIM - 1; ++k) split(k);
+    // eigenvalues alpha + i (beta_x +- beta_y +- beta_z): bit k of j set -> minus on axis k+1
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        T li = beta[0];
+#pragma unroll
+        for (int k = 0; k < DIM - 1; ++k) li += ((j >> k) & 1) ? -beta[k + 1] : beta[k + 1];
+        const T inv = (T)1 / (alpha * alpha + li * li);
+        const T ar = zr[j], ai = zi[j];
+        zr[j] = (ar * alpha + ai * li) * inv;
+        zi[j] = (ai * alpha - ar * li) * inv;
+    }
+#pragma unroll
+    for (int k = DIM - 2; k >= 0; --k) merge(k);
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) { r[2 * j] = zr[j]; r[2 * j + 1] = -zi[j]; }
+    // psi = W y: W+ = [[1,0],[-1,sqrt2]], W- = [[-1,sqrt2],[1,0]]
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+        if ((OCT >> ax) & 1) pass(ax, (T)-1, r2, (T)1, (T)0);
+        else pass(ax, (T)1, (T)0, (T)-1, r2);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = r[i];
+}
+
+// psi of a solved cell whose store is deferred.  gfx950 counts vector memory
+// loads and stores in one in-order counter, so a store issued before the next
+// cell's loads would make waiting for those loads wait for the store too;
+// storing after the next loads are in flight keeps stores off the critical path.
+template <int N>
+struct PendingPsi {
+    double v[N];
+    double *dst = nullptr;
+    __device__ __forceinline__ void flush()
+    {
+        if (!dst) return;
+        if constexpr (N % 2 == 0) {
+            d2v *o2 = reinterpret_cast<d2v *>(dst);
+#pragma unroll
+            for (int i = 0; i < N / 2; ++i) { d2v w; w.x = v[2 * i]; w.y = v[2 * i + 1]; o2[i] = w; }
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) dst[i] = v[i];
+        }
+        dst = nullptr;
+    }
+};
+
+// Solve one cell whose inputs are loaded: p = 1 by fast diagonalisation;
+// otherwise traces and K[a] from LDS, the mass and face-mass matrices as
+// compile-time tensor products (uniform axis-aligned boxes: Mu = vol *
+// unit_mass, F_side = area * unit_face_mass), and an LU; store
+// the outflow traces to the planes and leave psi in `pend` (only when in.ok);
+// the previous cell's pending psi is stored once this cell's inputs are in.
+// mid() runs between the assembly of the local system and its solve (the next
+// level's loads go there: issued after this cell's inputs were consumed and
+// before the pending psi store, so their latency overlaps the solve).
 template <int DIM, int P, typename T, int OCT, typename Mid>
 __device__ __forceinline__ void box_solve(const SweepParams &p, const BoxPlan &bp, const BoxLds &L,
-                                          const BoxInputs<Sh<DIM, P>::N> &in, int g, Mid &&mid)
+                                          const BoxInputs<Sh<DIM, P>::N> &in, int g, Mid &&mid,
+                                          PendingPsi<Sh<DIM, P>::N> &pend)
 {
     using S = Sh<DIM, P>;
     constexpr int N = S::N, NF = S::NF, PS = box_slot_stride(S::NF);
@@ -155,6 +299,17 @@ __device__ __forceinline__ void box_solve(const SweepParams &p, const BoxPlan &b
         }
     }
 
+    T x[N];
+    if constexpr (P == 1) {
+        double b[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) b[i] = in.q[i] + in.is[i];
+        const double inv_vol = 1.0 / bp.vol;
+        const double t[3] = {L.om[3 * d] * inv_vol, L.om[3 * d + 1] * inv_vol, L.om[3 * d + 2] * inv_vol};
+        mid();
+        pend.flush();
+        fd_solve_p1<DIM, T, OCT>(b, u, t, in.sig + (p.timedep ? p.tau : 0.0), inv_vol, x);
+    } else {
     // ---- right-hand side: q + iso + |Omega_ax| F_in u -----------------------
     double b[N];
 #pragma unroll
@@ -180,22 +335,18 @@ __device__ __forceinline__ void box_solve(const SweepParams &p, const BoxPlan &b
     for (int i = 0; i < N; ++i)
 #pragma unroll
         for (int j = 0; j < N; ++j) A[i][j] = (T)fma(sv, unit_mass<DIM, P>(i, j), Kl[i * N + j]);
-    T x[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) x[i] = (T)b[i];
     mid();
+    pend.flush();
     lu_factor_solve<N, T>(A, x);
+    }
 
     // ---- store psi and the outflow traces ----------------------------------
     const long long off = ((long long)g * p.Nomega + a) * p.Ndof + (long long)in.e * N;
-    if constexpr (N % 2 == 0) {
-        d2v *o2 = reinterpret_cast<d2v *>(p.psi + off);
 #pragma unroll
-        for (int i = 0; i < N / 2; ++i) { d2v v; v.x = (double)x[2 * i]; v.y = (double)x[2 * i + 1]; o2[i] = v; }
-    } else {
-#pragma unroll
-        for (int i = 0; i < N; ++i) p.psi[off + i] = (double)x[i];
-    }
+    for (int i = 0; i < N; ++i) pend.v[i] = (double)x[i];
+    pend.dst = p.psi + off;
 #pragma unroll
     for (int ax = 0; ax < DIM; ++ax) {
         const int s_out = ((OCT >> ax) & 1) ? 2 * ax : 2 * ax + 1;
@@ -204,10 +355,30 @@ __device__ __forceinline__ void box_solve(const SweepParams &p, const BoxPlan &b
     }
 }
 
+// Workgroup barrier that orders LDS only.  Inside a box the cells exchange
+// traces through LDS alone: psi goes to HBM for the next box level (a later
+// launch) and q, the isotropic source and sigma are read-only, so the barrier
+// need not wait for outstanding global loads (the prefetched next level) or
+// stores (this level's psi), which __syncthreads() would.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // PF: 0 loads each cell's inputs when it is solved; 2 loads the next level's
 // first-pass inputs during the current solve (between assembly and LU).
-template <int DIM, int P, typename T, int OCT, int PF>
-__device__ __forceinline__ void box_task(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
+// Everything a box workgroup sets up before its levels: the box's extent,
+// its directions, the LDS tables and the inflow trace planes.
+struct BoxCtx {
+    BoxLds L;
+    int lo[3], ext[3];
+    int g, ndir;
+};
+
+template <int DIM, int P, int OCT>
+__device__ __forceinline__ BoxCtx box_stage(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
 {
     using S = Sh<DIM, P>;
     constexpr int N = S::N, NF = S::NF, PS = box_slot_stride(S::NF);
@@ -238,11 +409,12 @@ __device__ __forceinline__ void box_task(const SweepParams &p, const BoxPlan &bp
     const BoxLds L{lds, Kw, omw, dirw, lcw, lpw};
 
     // ---- stage the workgroup tables ----------------------------------------
-    for (int i = tid; i < ndir * N * N; i += nt) {
-        const int d = i / (N * N), r = i - d * N * N;
-        const int a = gdirs[d];
-        Kw[d * bp.kstride + r] = a >= 0 ? p.K[(long long)a * N * N + r] : 0.0;
-    }
+    if constexpr (P != 1)
+        for (int i = tid; i < ndir * N * N; i += nt) {
+            const int d = i / (N * N), r = i - d * N * N;
+            const int a = gdirs[d];
+            Kw[d * bp.kstride + r] = a >= 0 ? p.K[(long long)a * N * N + r] : 0.0;
+        }
     for (int i = tid; i < bp.nd; i += nt) {
         const int a = i < ndir ? gdirs[i] : -1;
         dirw[i] = a;
@@ -288,37 +460,347 @@ __device__ __forceinline__ void box_task(const SweepParams &p, const BoxPlan &bp
             pl += (long long)bp.nd * np * PS;
         }
     }
-    __syncthreads();
+    lds_barrier();
+    return BoxCtx{L, {lo[0], lo[1], lo[2]}, {ext[0], ext[1], ext[2]}, g, ndir};
+}
+
+template <int DIM, int P, typename T, int OCT, int PF>
+__device__ __forceinline__ void box_task(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
+{
+    using S = Sh<DIM, P>;
+    constexpr int N = S::N;
+    const BoxCtx cx = box_stage<DIM, P, OCT>(p, bp, box_base, lds);
+    const BoxLds &L = cx.L;
+    const int g = cx.g, ndir = cx.ndir;
+    const int(&lo)[3] = cx.lo;
+    const int(&ext)[3] = cx.ext;
+    const int tid = threadIdx.x, nt = blockDim.x;
 
     // ---- the box's levels ---------------------------------------------------
     auto nothing = [] {};
+    PendingPsi<N> pend;
     if constexpr (PF == 0) {
         for (int l = 0; l < bp.nl; ++l) {
             const int items = (L.lpl[l + 1] - L.lpl[l]) * ndir;
             for (int it = tid; it < items; it += nt) {
                 BoxInputs<N> in;
                 box_fetch<DIM, P, OCT>(p, bp, L, ndir, lo, ext, g, l, it, in);
-                if (in.ok) box_solve<DIM, P, T, OCT>(p, bp, L, in, g, nothing);
+                if (in.ok) box_solve<DIM, P, T, OCT>(p, bp, L, in, g, nothing, pend);
             }
-            __syncthreads();
+            lds_barrier();
         }
     } else {
-        BoxInputs<N> cur, nxt;
-        box_fetch<DIM, P, OCT>(p, bp, L, ndir, lo, ext, g, 0, tid, cur);
-        for (int l = 0; l < bp.nl; ++l) {
+        // two input buffers with alternating roles (no register copy between
+        // levels, so the prefetched loads stay in flight across the barrier)
+        auto level = [&](int l, BoxInputs<N> &cur, BoxInputs<N> &nxt) {
             const int items = (L.lpl[l + 1] - L.lpl[l]) * ndir;
             auto pre = [&] { box_fetch<DIM, P, OCT>(p, bp, L, ndir, lo, ext, g, l + 1, tid, nxt); };
-            if (cur.ok) box_solve<DIM, P, T, OCT>(p, bp, L, cur, g, pre);
+            if (cur.ok) box_solve<DIM, P, T, OCT>(p, bp, L, cur, g, pre, pend);
             else pre();
             for (int it = tid + nt; it < items; it += nt) {
                 BoxInputs<N> in;
                 box_fetch<DIM, P, OCT>(p, bp, L, ndir, lo, ext, g, l, it, in);
-                if (in.ok) box_solve<DIM, P, T, OCT>(p, bp, L, in, g, nothing);
+                if (in.ok) box_solve<DIM, P, T, OCT>(p, bp, L, in, g, nothing, pend);
             }
-            cur = nxt;
-            __syncthreads();
+            lds_barrier();
+        };
+        BoxInputs<N> b0, b1;
+        box_fetch<DIM, P, OCT>(p, bp, L, ndir, lo, ext, g, 0, tid, b0);
+        for (int l = 0; l < bp.nl; l += 2) {
+            level(l, b0, b1);
+            if (l + 1 < bp.nl) level(l + 1, b1, b0);
         }
     }
+    pend.flush();
+}
+
+// ---------------------------------------------------------------------------
+// p = 1 with 2^(dim-1) lanes per cell.  Lane r of a cell holds its two nodes
+// along x, (0, r) and (1, r) with r = i_y + 2 i_z, which are adjacent in
+// memory: a cell's q, isotropic source and psi are one contiguous 64-byte
+// (3D) access shared by its lanes instead of 64 cache lines per wave-wide
+// access.  The fast diagonalisation above runs on the lanes: the x passes in
+// registers, the y and z passes and the complex splits as swaps inside
+// quads (DPP quad_perm), the division lane by lane.
+// ---------------------------------------------------------------------------
+template <typename T, int CTRL>
+__device__ __forceinline__ T quad_swap(T x)
+{
+    if constexpr (sizeof(T) == 8) {
+        const long long v = __builtin_bit_cast(long long, x);
+        const int lo = __builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), CTRL, 0xF, 0xF, false);
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (unsigned int)lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+    }
+}
+constexpr int kSwapY = 0xB1;   // quad_perm [1,0,3,2]: partner along y (lane bit 0)
+constexpr int kSwapZ = 0x4E;   // quad_perm [2,3,0,1]: partner along z (lane bit 1)
+
+template <typename T, int CTRL>
+__device__ __forceinline__ void lane_pass(T &v0, T &v1, int bit, T a00, T a01, T a10, T a11)
+{
+    const T p0 = quad_swap<T, CTRL>(v0), p1 = quad_swap<T, CTRL>(v1);
+    const T x0 = bit ? p0 : v0, x1 = bit ? v0 : p0;   // the pair (entry 0, entry 1) for node i0 = 0
+    const T y0 = bit ? p1 : v1, y1 = bit ? v1 : p1;   // and for i0 = 1
+    v0 = bit ? a10 * x0 + a11 * x1 : a00 * x0 + a01 * x1;
+    v1 = bit ? a10 * y0 + a11 * y1 : a00 * y0 + a01 * y1;
+}
+
+// (zr, zi) -> p = z0 - i z1 on the bit-0 lane, m = z0 + i z1 on the bit-1 lane
+template <typename T, int CTRL>
+__device__ __forceinline__ void lane_split(T &zr, T &zi, int bit)
+{
+    const T pr = quad_swap<T, CTRL>(zr), pi = quad_swap<T, CTRL>(zi);
+    const T nr = bit ? pr - zi : zr + pi;
+    const T ni = bit ? pi + zr : zi - pr;
+    zr = nr;
+    zi = ni;
+}
+
+// inverse: z0 = (p + m) / 2 on the bit-0 lane, z1 = i (p - m) / 2 on the bit-1 lane
+template <typename T, int CTRL>
+__device__ __forceinline__ void lane_merge(T &zr, T &zi, int bit)
+{
+    const T pr = quad_swap<T, CTRL>(zr), pi = quad_swap<T, CTRL>(zi);
+    const T nr = bit ? (T)0.5 * (zi - pi) : (T)0.5 * (zr + pr);
+    const T ni = bit ? (T)0.5 * (pr - zr) : (T)0.5 * (zi + pi);
+    zr = nr;
+    zi = ni;
+}
+
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig, T ux, const T (&uy)[2],
+                                            const T (&uz)[2], int by, int bz)
+{
+    constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
+    auto passx = [&](T a00, T a01, T a10, T a11) {
+        const T w0 = v0, w1 = v1;
+        v0 = a00 * w0 + a01 * w1;
+        v1 = a10 * w0 + a11 * w1;
+    };
+    // r = M^{-1} b (the caller scaled b by 1 / vol)
+    passx((T)4, (T)-2, (T)-2, (T)4);
+    lane_pass<T, kSwapY>(v0, v1, by, (T)4, (T)-2, (T)-2, (T)4);
+    if constexpr (DIM == 3) lane_pass<T, kSwapZ>(v0, v1, bz, (T)4, (T)-2, (T)-2, (T)4);
+    // inflow faces: t_ax (m^{-1} e_in) (x) u_ax
+    {
+        constexpr bool nx = OCT & 1, ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
+        const T cx0 = nx ? (T)-2 : (T)4, cx1 = nx ? (T)4 : (T)-2;
+        v0 += t[0] * cx0 * ux;
+        v1 += t[0] * cx1 * ux;
+        const T cy = (by == (ny ? 1 : 0)) ? (T)4 : (T)-2;
+        v0 += t[1] * cy * uy[0];
+        v1 += t[1] * cy * uy[1];
+        if constexpr (DIM == 3) {
+            const T cz = (bz == (nz ? 1 : 0)) ? (T)4 : (T)-2;
+            v0 += t[2] * cz * uz[0];
+            v1 += t[2] * cz * uz[1];
+        }
+    }
+    // y = W^{-1} r
+    if constexpr (OCT & 1) passx((T)0, (T)1, ir2, ir2);
+    else passx((T)1, (T)0, ir2, ir2);
+    if constexpr ((OCT >> 1) & 1) lane_pass<T, kSwapY>(v0, v1, by, (T)0, (T)1, ir2, ir2);
+    else lane_pass<T, kSwapY>(v0, v1, by, (T)1, (T)0, ir2, ir2);
+    if constexpr (DIM == 3) {
+        if constexpr ((OCT >> 2) & 1) lane_pass<T, kSwapZ>(v0, v1, bz, (T)0, (T)1, ir2, ir2);
+        else lane_pass<T, kSwapZ>(v0, v1, bz, (T)1, (T)0, ir2, ir2);
+    }
+    // complexify along x, split y (and z), divide by alpha + i(beta_x +- beta_y +- beta_z)
+    T zr = v0, zi = -v1;
+    lane_split<T, kSwapY>(zr, zi, by);
+    if constexpr (DIM == 3) lane_split<T, kSwapZ>(zr, zi, bz);
+    T alpha = sig, li = r2 * t[0];
+    alpha += (T)2 * (t[0] + t[1]);
+    li += by ? -r2 * t[1] : r2 * t[1];
+    if constexpr (DIM == 3) {
+        alpha += (T)2 * t[2];
+        li += bz ? -r2 * t[2] : r2 * t[2];
+    }
+    {
+        const T inv = (T)1 / (alpha * alpha + li * li);
+        const T ar = zr, ai = zi;
+        zr = (ar * alpha + ai * li) * inv;
+        zi = (ai * alpha - ar * li) * inv;
+    }
+    if constexpr (DIM == 3) lane_merge<T, kSwapZ>(zr, zi, bz);
+    lane_merge<T, kSwapY>(zr, zi, by);
+    v0 = zr;
+    v1 = -zi;
+    // psi = W y
+    if constexpr (OCT & 1) passx((T)-1, r2, (T)1, (T)0);
+    else passx((T)1, (T)0, (T)-1, r2);
+    if constexpr ((OCT >> 1) & 1) lane_pass<T, kSwapY>(v0, v1, by, (T)-1, r2, (T)1, (T)0);
+    else lane_pass<T, kSwapY>(v0, v1, by, (T)1, (T)0, (T)-1, r2);
+    if constexpr (DIM == 3) {
+        if constexpr ((OCT >> 2) & 1) lane_pass<T, kSwapZ>(v0, v1, bz, (T)-1, r2, (T)1, (T)0);
+        else lane_pass<T, kSwapZ>(v0, v1, bz, (T)1, (T)0, (T)-1, r2);
+    }
+}
+
+// one lane's inputs of one cell, loaded ahead of its solve
+struct SplitIn {
+    double q0, q1, i0, i1, sig;
+    long long off;
+    int e, d;
+    int lc[3];
+    bool ok;
+};
+
+template <int DIM, int OCT>
+__device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int l, int it,
+                                            int r, SplitIn &in)
+{
+    constexpr int N = 2 << (DIM - 1);
+    in.ok = false;
+    in.q0 = in.q1 = in.i0 = in.i1 = in.sig = 0.0;
+    in.off = 0;
+    in.e = in.d = 0;
+    in.lc[0] = in.lc[1] = in.lc[2] = 0;
+    if (l >= bp.nl) return;
+    const BoxLds &L = cx.L;
+    const int c0 = L.lpl[l], cnt = L.lpl[l + 1] - c0;
+    if (it >= cnt * cx.ndir) return;
+    const int d = it / cnt, k = it - d * cnt;
+    const uint32_t cw = L.lcl[c0 + k];
+    in.lc[0] = (int)(cw & 1023u);
+    in.lc[1] = (int)((cw >> 10) & 1023u);
+    in.lc[2] = (int)(cw >> 20);
+    const int a = L.dirs[d];
+    if (a < 0 || in.lc[0] >= cx.ext[0] || in.lc[1] >= cx.ext[1] || in.lc[2] >= cx.ext[2]) return;
+    int c[3] = {0, 0, 0};
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax)
+        c[ax] = ((OCT >> ax) & 1) ? cx.lo[ax] + cx.ext[ax] - 1 - in.lc[ax] : cx.lo[ax] + in.lc[ax];
+    in.e = c[0] + p.nel[0] * (c[1] + p.nel[1] * c[2]);
+    in.d = d;
+    in.off = ((long long)cx.g * p.Nomega + a) * p.Ndof + (long long)in.e * N + 2 * r;
+    in.ok = true;
+    if (p.src) {
+        const d2v v = *reinterpret_cast<const d2v *>(p.src + in.off);
+        in.q0 = v.x;
+        in.q1 = v.y;
+    }
+    if (p.iso) {
+        const d2v v = *reinterpret_cast<const d2v *>(p.iso + (long long)cx.g * p.Ndof + (long long)in.e * N + 2 * r);
+        in.i0 = v.x;
+        in.i1 = v.y;
+    }
+    in.sig = p.sigma[(long long)cx.g * p.Ne + in.e];
+}
+
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int r,
+                                            const SplitIn &in)
+{
+    constexpr int LPC = 1 << (DIM - 1), NF = LPC, PS = box_slot_stride(NF);
+    const BoxLds &L = cx.L;
+    const int by = r & 1, bz = (r >> 1) & 1, d = in.d;
+    const double inv_vol = 1.0 / bp.vol;
+    // trace slots of this cell (the other axes' local coordinates)
+    double *slot[DIM];
+    {
+        double *pl = L.planes;
+#pragma unroll
+        for (int ax = 0; ax < DIM; ++ax) {
+            int sidx = 0, mul = 1;
+#pragma unroll
+            for (int o = 0; o < DIM; ++o)
+                if (o != ax) { sidx += mul * in.lc[o]; mul *= bp.bs[o]; }
+            slot[ax] = pl + ((long long)d * bp.psz[ax] + sidx) * PS;
+            pl += (long long)bp.nd * bp.psz[ax] * PS;
+        }
+    }
+    // x face: node (., r); y face: (i0, i_z) -> i0 + 2 i_z; z face: (i0, i_y) -> i0 + 2 i_y
+    const int my0 = DIM == 3 ? 2 * bz : 0;
+    const T ux = (T)slot[0][r];
+    const T uy[2] = {(T)slot[1][my0], (T)slot[1][my0 + 1]};
+    T uz[2] = {(T)0, (T)0};
+    if constexpr (DIM == 3) {
+        uz[0] = (T)slot[2][2 * by];
+        uz[1] = (T)slot[2][2 * by + 1];
+    }
+    const T t[3] = {(T)(L.om[3 * d] * inv_vol), (T)(L.om[3 * d + 1] * inv_vol), (T)(L.om[3 * d + 2] * inv_vol)};
+    T v0 = (T)((in.q0 + in.i0) * inv_vol), v1 = (T)((in.q1 + in.i1) * inv_vol);
+    fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by, bz);
+    d2v o;
+    o.x = (double)v0;
+    o.y = (double)v1;
+    *reinterpret_cast<d2v *>(p.psi + in.off) = o;
+    // outflow traces
+    slot[0][r] = (OCT & 1) ? o.x : o.y;
+    if (by == (((OCT >> 1) & 1) ? 0 : 1)) {
+        slot[1][my0] = o.x;
+        slot[1][my0 + 1] = o.y;
+    }
+    if constexpr (DIM == 3) {
+        if (bz == (((OCT >> 2) & 1) ? 0 : 1)) {
+            slot[2][2 * by] = o.x;
+            slot[2][2 * by + 1] = o.y;
+        }
+    }
+}
+
+// The first pass of every level is prefetched during the level before (two
+// input buffers with alternating roles, so no register copy waits for loads
+// in flight; the LDS-only barrier lets them cross it).
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
+{
+    constexpr int LPC = 1 << (DIM - 1);
+    const BoxCtx cx = box_stage<DIM, 1, OCT>(p, bp, box_base, lds);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int r = tid & (LPC - 1), qd = tid / LPC, nq = nt / LPC;
+    auto level = [&](int l, SplitIn &cur, SplitIn &nxt) {
+        const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
+        split_fetch<DIM, OCT>(p, bp, cx, l + 1, qd, r, nxt);
+        if (cur.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, cur);
+        for (int it = qd + nq; it < items; it += nq) {   // uniform over the lanes of a cell
+            SplitIn in;
+            split_fetch<DIM, OCT>(p, bp, cx, l, it, r, in);
+            if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in);
+        }
+        lds_barrier();
+    };
+    SplitIn b0, b1;
+    split_fetch<DIM, OCT>(p, bp, cx, 0, qd, r, b0);
+    for (int l = 0; l < bp.nl; l += 2) {
+        level(l, b0, b1);
+        if (l + 1 < bp.nl) level(l + 1, b1, b0);
+    }
+}
+
+template <int DIM, typename T>
+__global__ void __launch_bounds__(1024) k_sweep_box_split(SweepParams p, BoxPlan bp, int box_base)
+{
+    extern __shared__ double lds[];
+    const int oct = blockIdx.x % bp.noct;   // workgroup-uniform
+    switch (oct) {
+    case 0: box_task_split<DIM, T, 0>(p, bp, box_base, lds); break;
+    case 1: box_task_split<DIM, T, 1>(p, bp, box_base, lds); break;
+    case 2: box_task_split<DIM, T, 2>(p, bp, box_base, lds); break;
+    case 3: box_task_split<DIM, T, 3>(p, bp, box_base, lds); break;
+    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4>(p, bp, box_base, lds); break;
+    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5>(p, bp, box_base, lds); break;
+    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6>(p, bp, box_base, lds); break;
+    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7>(p, bp, box_base, lds); break;
+    }
+}
+
+template <int DIM, typename T>
+static hipError_t run_boxes_split(const DeviceSweep &ds, const SweepParams &prm, hipStream_t s)
+{
+    const BoxPlan &bp = ds.box;
+    const int nlev = (int)ds.box_ptr.size() - 1;
+    for (int k = 0; k < nlev; ++k) {
+        const int nbox = ds.box_ptr[k + 1] - ds.box_ptr[k];
+        if (nbox == 0) continue;
+        hipLaunchKernelGGL((k_sweep_box_split<DIM, T>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
+                           dim3(ds.box_threads), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
+    }
+    return hipGetLastError();
 }
 
 template <int DIM, int P, typename T, int PF>
@@ -362,13 +844,20 @@ static hipError_t run_boxes_pf(const DeviceSweep &ds, const SweepParams &prm, hi
 template <int DIM, int P>
 static hipError_t run_boxes_p(const DeviceSweep &ds, const SweepParams &prm, hipStream_t s)
 {
+    if constexpr (P == 1) {
+        // lane-split cells; the workgroup must hold whole cells
+        if (ds.box_threads % (1 << (DIM - 1))) return hipErrorInvalidValue;
+        if (ds.precision == 1) return run_boxes_split<DIM, float>(ds, prm, s);
+        return run_boxes_split<DIM, double>(ds, prm, s);
+    }
+    if (ds.box_threads > 256) return hipErrorInvalidValue;
     if (ds.precision == 1) return run_boxes_pf<DIM, P, float>(ds, prm, s);
     return run_boxes_pf<DIM, P, double>(ds, prm, s);
 }
 
 hipError_t launch_box_sweep(const DeviceSweep &ds, const SweepParams &prm, hipStream_t s)
 {
-    if (!prm.uniform || !ds.box_buf || ds.box_threads < 64 || ds.box_threads > 256) return hipErrorInvalidValue;
+    if (!prm.uniform || !ds.box_buf || ds.box_threads < 64 || ds.box_threads > 1024) return hipErrorInvalidValue;
     if (prm.dim == 2) {
         switch (prm.p) {
         case 0: return run_boxes_p<2, 0>(ds, prm, s);

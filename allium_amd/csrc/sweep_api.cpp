@@ -249,8 +249,11 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
         bs[d] = std::min(bs[d], h->mesh.nel[d]);
     }
     if (nd < 1 || nd > 64) return fail(ALLIUM_ERR_ARG, "directions per workgroup must be 1..64");
-    if (threads < 64 || threads > 256 || threads % 64)
-        return fail(ALLIUM_ERR_ARG, "box sweep workgroup size must be 64..256, a multiple of 64");
+    // p = 1 cells are split over 2^(dim-1) lanes and take workgroups up to 1024
+    const int max_threads = h->ed.p == 1 ? 1024 : 256;
+    if (threads < 64 || threads > max_threads || threads % 64)
+        return fail(ALLIUM_ERR_ARG, "box sweep workgroup size must be a multiple of 64 in 64.." +
+                                        std::to_string(max_threads));
     if (prefetch != 0 && prefetch != 2) return fail(ALLIUM_ERR_ARG, "box sweep prefetch must be 0 or 2");
     const int No = h->quad.size(), noct = 1 << dim;
     std::vector<std::vector<int>> oct(noct);
@@ -273,7 +276,9 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
     bp.maxoct = bp.nchunk * nd;
     bp.noct = noct;
     bp.nl = bs[0] + bs[1] + bs[2] - 2;
-    bp.kstride = n * n + 1;   // odd stride: the directions' matrices start in different LDS banks
+    // odd stride: the directions' matrices start in different LDS banks; p = 1
+    // cells are solved by fast diagonalisation and need no matrix in LDS
+    bp.kstride = h->ed.p == 1 ? 0 : n * n + 1;
     long plane = 0;
     for (int ax = 0; ax < 3; ax++) {
         int v = 1;
@@ -310,6 +315,12 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
                     errf = std::max(errf, std::fabs(h->ed.Fface[((size_t)sd * nf + k) * nf + l] - c));
                     reff = std::max(reff, std::fabs(c));
                 }
+        // p = 1 also takes the time mass (sigma + tau shift) as the same tensor product
+        if (h->ed.p == 1)
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < n; j++)
+                    err = std::max(err, std::fabs(h->ed.Mm[(size_t)i * n + j] -
+                                                  bp.vol * allium::host_unit_mass(dim, 1, i, j)));
         if (!(err <= 1e-13 * ref) || !(errf <= 1e-13 * reff))
             return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps: element matrices are not the tensor-product masses "
                                                 "of an axis-aligned box");
@@ -608,10 +619,14 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
                        allium::box_sweep_supported(h->mesh.dim, h->ed.p);
     if (boxes) {
         const bool d3 = h->mesh.dim == 3;
+        // p = 1: lane-split cells (no prefetch variant), workgroups up to 1024
+        const bool split = h->ed.p == 1;
+        const std::vector<int> ths = split ? std::vector<int>{256, 512, 1024} : std::vector<int>{128, 256};
+        const std::vector<int> pfs = split ? std::vector<int>{0} : std::vector<int>{0, 2};
         for (int b : d3 ? std::vector<int>{4, 8, 16} : std::vector<int>{8, 16, 32})
             for (int nd : {1, 2, 3, 5, 10})
-                for (int th : {128, 256})
-                    for (int pf : {0, 2}) cands.push_back({5, th, 1, b, b, d3 ? b : 1, nd, pf});
+                for (int th : ths)
+                    for (int pf : pfs) cands.push_back({5, th, 1, b, b, d3 ? b : 1, nd, pf});
     }
     // keep the caller's box tiling to restore it when a non-box mode wins
     const allium::BoxPlan keep_box = h->ds.box;
@@ -671,11 +686,13 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
 int allium_sweep_set_tuning(allium_sweep_t h, int mode, int block, int lanes)
 {
     // the level and box kernels are compiled with __launch_bounds__(256), the
-    // direction-sweep kernel (mode 3) with __launch_bounds__(512)
-    const int maxb = mode == 3 ? 512 : 256;
+    // direction-sweep kernel (mode 3) with __launch_bounds__(512) and the
+    // lane-split p = 1 box kernel with __launch_bounds__(1024)
+    const int maxb = !h ? 0 : mode == 3 ? 512 : (mode >= 4 && h->ed.p == 1) ? 1024 : 256;
     if (!h || mode < 0 || mode > 5 || block < 64 || block > maxb || block % 64 ||
         (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4))
-        return fail(ALLIUM_ERR_ARG, "bad tuning (mode 0..5, block 64..256 step 64 (512 for mode 3), lanes 0/1/2/4)");
+        return fail(ALLIUM_ERR_ARG, "bad tuning (mode 0..5, block 64..256 step 64 (512 for mode 3, 1024 for p = 1 "
+                                    "box sweeps), lanes 0/1/2/4)");
     if (mode >= 3 && lanes != 1)
         return fail(ALLIUM_ERR_ARG, "bad tuning: direction and box sweeps use one lane per cell");
     if (mode >= 4) {

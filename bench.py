@@ -114,9 +114,10 @@ def measure_traffic(args, tuning):
 def si_run(n, seed, maxit, learned=True, tuning=None):
     """Wall time to a 1e-8 FP64 scalar-flux residual on the 64^3 p=1 S8 c=0.9999
     medium (lognormal sigma_t, log-sd 1.5, Q=1): plain GPU source iteration
-    against source iteration with the learned correction (FP64 and
-    FP32-refined sweeps).  The network is trained first on the GPU from
-    seeded 32^3 media (not part of the timed solves; its time is reported)."""
+    against source iteration with MIP DSA, with the learned correction (FP64
+    and FP32-refined sweeps) and with the learned correction falling back to
+    DSA.  The network is trained first on the GPU from seeded 32^3 media (not
+    part of the timed solves; its time is reported), as is the DSA setup."""
     import allium_amd as A
     from allium_amd import cnn
     from allium_amd import problems as P
@@ -144,15 +145,29 @@ def si_run(n, seed, maxit, learned=True, tuning=None):
 
     phi0, plain = solve()
     out = {"plain": plain}
+
+    def compare(name, **kw):
+        phi1, res = solve(**kw)
+        res["rel_diff_vs_plain"] = float(np.linalg.norm(phi1 - phi0) / np.linalg.norm(phi0))
+        res["speedup_vs_plain"] = plain["wall_s"] / res["wall_s"]
+        out[name] = res
+        return res
+
+    # MIP diffusion synthetic acceleration (setup on the host, not timed in the solve)
+    t = time.perf_counter()
+    dsa = A.DSA(Linv, cfg["sigma_s"])
+    dsa_setup = time.perf_counter() - t
+    res = compare("dsa_fp64", accel=dsa)
+    info = dsa.info()
+    res.update(setup_s=dsa_setup, levels=info["level_size"], cg_iterations_per_application=info["cg_iterations"] /
+               max(1, info["applications"]), dsa_gpu_s=info["seconds"])
     if not learned:
         return out
     net, info = cnn.fit(n=32, media=60, rounds=2, steps=1500, seed0=1000, sn=8, log=log)
     out["training"] = info
     for name, prec in (("learned_fp64", A.FP64), ("learned_fp32_refined", A.FP32_REFINED)):
-        phi1, res = solve(accel=net, precision=prec)
-        res["rel_diff_vs_plain"] = float(np.linalg.norm(phi1 - phi0) / np.linalg.norm(phi0))
-        res["speedup_vs_plain"] = plain["wall_s"] / res["wall_s"]
-        out[name] = res
+        compare(name, accel=net, precision=prec)
+    compare("learned_fp32_refined_dsa_fallback", accel=net, precision=A.FP32_REFINED, fallback=dsa)
     return out
 
 

@@ -123,6 +123,9 @@ BOX_TILINGS = [
     ((14, 13, 12), 1, 8, (8, 8, 8), 5, 256),
     ((14, 13, 12), 1, 8, (16, 16, 16), 2, 128),
     ((14, 13, 12), 1, 8, (1, 1, 1), 10, 128),
+    ((14, 13, 12), 1, 8, (8, 8, 8), 5, 1024),
+    ((14, 13, 12), 1, 8, (6, 7, 5), 10, 512),
+    ((33, 31), 1, 8, (16, 16), 10, 1024),
     ((7, 6, 5), 0, 6, (3, 2, 4), 4, 192),
     ((9, 7), 2, 8, (4, 3), 3, 64),
     ((33, 31), 1, 8, (16, 16), 10, 256),
@@ -164,7 +167,10 @@ def test_box_tiling_errors():
     with pytest.raises(A.AlliumError, match="LDS"):
         Linv.set_box_tiling(16, 16, 16, dirs_per_group=10, threads=256)
     with pytest.raises(A.AlliumError, match="workgroup size"):
-        Linv.set_box_tiling(8, 8, 8, dirs_per_group=5, threads=512)
+        Linv.set_box_tiling(8, 8, 8, dirs_per_group=5, threads=1280)
+    Lp0, _ = _pair((8, 8, 8), 0, 8, np.ones((1, 512)))
+    with pytest.raises(A.AlliumError, match="workgroup size"):
+        Lp0.set_box_tiling(8, 8, 8, dirs_per_group=5, threads=512)
     Linv.set_box_tiling(8, 8, 8, dirs_per_group=5, threads=128)
     assert Linv.box_tiling() == {"bx": 8, "by": 8, "bz": 8, "dirs_per_group": 5, "threads": 128, "prefetch": 2}
     with pytest.raises(A.AlliumError, match="prefetch"):
