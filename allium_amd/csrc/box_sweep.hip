@@ -49,6 +49,11 @@ struct BoxLds {
     const int *dirs;    // nd direction ids (-1: none)
     const uint32_t *lcl;
     const int *lpl;
+    // p = 1 lane-split path: per direction the psi offset of the group and
+    // direction (-1: none); per box cell in level order its element (-1:
+    // outside the mesh)
+    const long long *dbase;
+    const int *cell_e;
 };
 
 // Inputs of one cell that do not depend on the sweep: the element, its total
@@ -402,11 +407,13 @@ __device__ __forceinline__ BoxCtx box_stage(const SweepParams &p, const BoxPlan 
     // LDS layout (doubles first, then 32-bit tables)
     double *Kw = lds + (long long)bp.nd * nplane * PS;
     double *omw = Kw + (long long)bp.nd * bp.kstride;
-    int *dirw = reinterpret_cast<int *>(omw + 3 * bp.nd);
+    long long *dbw = reinterpret_cast<long long *>(omw + 3 * bp.nd);
+    int *dirw = reinterpret_cast<int *>(dbw + (P == 1 ? bp.nd : 0));
     uint32_t *lcw = reinterpret_cast<uint32_t *>(dirw + bp.nd);
     const int vol = bp.bs[0] * bp.bs[1] * bp.bs[2];
     int *lpw = reinterpret_cast<int *>(lcw + vol);
-    const BoxLds L{lds, Kw, omw, dirw, lcw, lpw};
+    int *cew = lpw + bp.nl + 1;
+    const BoxLds L{lds, Kw, omw, dirw, lcw, lpw, dbw, cew};
 
     // ---- stage the workgroup tables ----------------------------------------
     if constexpr (P != 1)
@@ -423,6 +430,24 @@ __device__ __forceinline__ BoxCtx box_stage(const SweepParams &p, const BoxPlan 
     }
     for (int i = tid; i < vol; i += nt) lcw[i] = bp.lcells[i];
     for (int i = tid; i <= bp.nl; i += nt) lpw[i] = bp.lptr[i];
+    if constexpr (P == 1) {
+        for (int i = tid; i < bp.nd; i += nt) {
+            const int a = i < ndir ? gdirs[i] : -1;
+            dbw[i] = a >= 0 ? ((long long)g * p.Nomega + a) * p.Ndof : -1;
+        }
+        for (int i = tid; i < vol; i += nt) {
+            const uint32_t cw = bp.lcells[i];
+            const int lc[3] = {(int)(cw & 1023u), (int)((cw >> 10) & 1023u), (int)(cw >> 20)};
+            int c[3] = {0, 0, 0};
+            bool in_box = true;
+#pragma unroll
+            for (int ax = 0; ax < DIM; ++ax) {
+                in_box = in_box && lc[ax] < ext[ax];
+                c[ax] = ((OCT >> ax) & 1) ? lo[ax] + ext[ax] - 1 - lc[ax] : lo[ax] + lc[ax];
+            }
+            cew[i] = in_box ? c[0] + p.nel[0] * (c[1] + p.nel[1] * c[2]) : -1;
+        }
+    }
     // ---- inflow planes from the upwind neighbours (zero on the boundary) ---
     {
         double *pl = lds;
@@ -535,117 +560,133 @@ __device__ __forceinline__ T quad_swap(T x)
         return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
     }
 }
-constexpr int kSwapY = 0xB1;   // quad_perm [1,0,3,2]: partner along y (lane bit 0)
-constexpr int kSwapZ = 0x4E;   // quad_perm [2,3,0,1]: partner along z (lane bit 1)
+constexpr int kSwapY = 0xB1;    // quad_perm [1,0,3,2]: partner along y (lane bit 0)
+constexpr int kSwapZ = 0x4E;    // quad_perm [2,3,0,1]: partner along z (lane bit 1)
+constexpr int kSwapYZ = 0x1B;   // quad_perm [3,2,1,0]: partner along y and z
 
-template <typename T, int CTRL>
-__device__ __forceinline__ void lane_pass(T &v0, T &v1, int bit, T a00, T a01, T a10, T a11)
-{
-    const T p0 = quad_swap<T, CTRL>(v0), p1 = quad_swap<T, CTRL>(v1);
-    const T x0 = bit ? p0 : v0, x1 = bit ? v0 : p0;   // the pair (entry 0, entry 1) for node i0 = 0
-    const T y0 = bit ? p1 : v1, y1 = bit ? v1 : p1;   // and for i0 = 1
-    v0 = bit ? a10 * x0 + a11 * x1 : a00 * x0 + a01 * x1;
-    v1 = bit ? a10 * y0 + a11 * y1 : a00 * y0 + a01 * y1;
-}
-
-// (zr, zi) -> p = z0 - i z1 on the bit-0 lane, m = z0 + i z1 on the bit-1 lane
-template <typename T, int CTRL>
-__device__ __forceinline__ void lane_split(T &zr, T &zi, int bit)
-{
-    const T pr = quad_swap<T, CTRL>(zr), pi = quad_swap<T, CTRL>(zi);
-    const T nr = bit ? pr - zi : zr + pi;
-    const T ni = bit ? pi + zr : zi - pr;
-    zr = nr;
-    zi = ni;
-}
-
-// inverse: z0 = (p + m) / 2 on the bit-0 lane, z1 = i (p - m) / 2 on the bit-1 lane
-template <typename T, int CTRL>
-__device__ __forceinline__ void lane_merge(T &zr, T &zi, int bit)
-{
-    const T pr = quad_swap<T, CTRL>(zr), pi = quad_swap<T, CTRL>(zi);
-    const T nr = bit ? (T)0.5 * (zi - pi) : (T)0.5 * (zr + pr);
-    const T ni = bit ? (T)0.5 * (pr - zr) : (T)0.5 * (zi + pi);
-    zr = nr;
-    zi = ni;
-}
+// Per-lane constants of the split solve.  A 2x2 operator a along an axis
+// becomes v <- cm v + cp v(partner) with (cm, cp) = (a00, a01) on the bit-0
+// lane and (a11, a10) on the bit-1 lane; J = [[0,1],[-1,0]] is s * partner
+// with s = +1 / -1 on the bit-0 / bit-1 lane.
+template <typename T>
+struct LaneK {
+    T wi_m[2], wi_p[2];   // W^{-1} along y, z
+    T w_m[2], w_p[2];     // W along y, z
+    T s[2];               // J sign along y, z
+    T cf[2];              // (m^{-1} e_in)[own index] along y, z
+};
 
 template <int DIM, typename T, int OCT>
-__device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig, T ux, const T (&uy)[2],
-                                            const T (&uz)[2], int by, int bz)
+__device__ __forceinline__ LaneK<T> lane_constants(int by, int bz)
 {
     constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
-    auto passx = [&](T a00, T a01, T a10, T a11) {
-        const T w0 = v0, w1 = v1;
-        v0 = a00 * w0 + a01 * w1;
-        v1 = a10 * w0 + a11 * w1;
-    };
-    // r = M^{-1} b (the caller scaled b by 1 / vol)
-    passx((T)4, (T)-2, (T)-2, (T)4);
-    lane_pass<T, kSwapY>(v0, v1, by, (T)4, (T)-2, (T)-2, (T)4);
-    if constexpr (DIM == 3) lane_pass<T, kSwapZ>(v0, v1, bz, (T)4, (T)-2, (T)-2, (T)4);
-    // inflow faces: t_ax (m^{-1} e_in) (x) u_ax
+    LaneK<T> k;
+    const int bit[2] = {by, bz};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const bool neg = (OCT >> (j + 1)) & 1;
+        const int b = bit[j];
+        // W+^{-1} = [[1,0],[1/r2,1/r2]], W-^{-1} = [[0,1],[1/r2,1/r2]]
+        k.wi_m[j] = b ? ir2 : (neg ? (T)0 : (T)1);
+        k.wi_p[j] = b ? ir2 : (neg ? (T)1 : (T)0);
+        // W+ = [[1,0],[-1,r2]], W- = [[-1,r2],[1,0]]
+        k.w_m[j] = b ? (neg ? (T)0 : r2) : (neg ? (T)-1 : (T)1);
+        k.w_p[j] = b ? (neg ? (T)1 : (T)-1) : (neg ? r2 : (T)0);
+        k.s[j] = b ? (T)-1 : (T)1;
+        k.cf[j] = (b == (neg ? 1 : 0)) ? (T)4 : (T)-2;
+    }
+    return k;
+}
+
+template <typename T, int CTRL>
+__device__ __forceinline__ void lane_op(T &v0, T &v1, T cm, T cp)
+{
+    const T p0 = quad_swap<T, CTRL>(v0), p1 = quad_swap<T, CTRL>(v1);
+    v0 = cm * v0 + cp * p0;
+    v1 = cm * v1 + cp * p1;
+}
+
+// psi on this lane's nodes (0, r), (1, r) from (v0, v1) = b / vol there:
+// r = M^{-1} b + inflow terms, y = W^{-1} r, then
+//   S^{-1} = (B^2 - c^2)^{-1} (B + c K) (A_x - beta_y J_y - beta_z J_z),
+// with A_x = alpha + beta_x J_x, B = A_x^2 + beta_y^2 + beta_z^2,
+// c = 2 beta_y beta_z and K = J_y J_z (commuting, J^2 = -I, K^2 = I), so B and
+// B^2 - c^2 act inside the lane's x pair; psi = W y.
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig, T ux, const T (&uy)[2],
+                                            const T (&uz)[2], const LaneK<T> &k)
+{
+    constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
+    constexpr bool nx = OCT & 1;
+    // M^{-1}: m^{-1} = [[4,-2],[-2,4]] on every axis
     {
-        constexpr bool nx = OCT & 1, ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
-        const T cx0 = nx ? (T)-2 : (T)4, cx1 = nx ? (T)4 : (T)-2;
-        v0 += t[0] * cx0 * ux;
-        v1 += t[0] * cx1 * ux;
-        const T cy = (by == (ny ? 1 : 0)) ? (T)4 : (T)-2;
-        v0 += t[1] * cy * uy[0];
-        v1 += t[1] * cy * uy[1];
+        const T w0 = v0, w1 = v1;
+        v0 = (T)4 * w0 - (T)2 * w1;
+        v1 = (T)4 * w1 - (T)2 * w0;
+    }
+    lane_op<T, kSwapY>(v0, v1, (T)4, (T)-2);
+    if constexpr (DIM == 3) lane_op<T, kSwapZ>(v0, v1, (T)4, (T)-2);
+    // inflow faces: t_ax (m^{-1} e_in) (x) u_ax
+    v0 += t[0] * (nx ? (T)-2 : (T)4) * ux;
+    v1 += t[0] * (nx ? (T)4 : (T)-2) * ux;
+    v0 += t[1] * k.cf[0] * uy[0];
+    v1 += t[1] * k.cf[0] * uy[1];
+    if constexpr (DIM == 3) {
+        v0 += t[2] * k.cf[1] * uz[0];
+        v1 += t[2] * k.cf[1] * uz[1];
+    }
+    // W^{-1}
+    {
+        const T w0 = v0, w1 = v1;
+        v0 = nx ? w1 : w0;
+        v1 = ir2 * (w0 + w1);
+    }
+    lane_op<T, kSwapY>(v0, v1, k.wi_m[0], k.wi_p[0]);
+    if constexpr (DIM == 3) lane_op<T, kSwapZ>(v0, v1, k.wi_m[1], k.wi_p[1]);
+    // S^{-1}
+    T alpha = sig + (T)2 * (t[0] + t[1]);
+    if constexpr (DIM == 3) alpha += (T)2 * t[2];
+    const T bx = r2 * t[0], by = r2 * t[1], bz = DIM == 3 ? r2 * t[2] : (T)0;
+    T u0 = alpha * v0 + bx * v1, u1 = alpha * v1 - bx * v0;   // A_x v
+    {
+        const T gy = by * k.s[0];
+        u0 -= gy * quad_swap<T, kSwapY>(v0);
+        u1 -= gy * quad_swap<T, kSwapY>(v1);
         if constexpr (DIM == 3) {
-            const T cz = (bz == (nz ? 1 : 0)) ? (T)4 : (T)-2;
-            v0 += t[2] * cz * uz[0];
-            v1 += t[2] * cz * uz[1];
+            const T gz = bz * k.s[1];
+            u0 -= gz * quad_swap<T, kSwapZ>(v0);
+            u1 -= gz * quad_swap<T, kSwapZ>(v1);
         }
     }
-    // y = W^{-1} r
-    if constexpr (OCT & 1) passx((T)0, (T)1, ir2, ir2);
-    else passx((T)1, (T)0, ir2, ir2);
-    if constexpr ((OCT >> 1) & 1) lane_pass<T, kSwapY>(v0, v1, by, (T)0, (T)1, ir2, ir2);
-    else lane_pass<T, kSwapY>(v0, v1, by, (T)1, (T)0, ir2, ir2);
+    const T a = alpha * alpha - bx * bx + by * by + bz * bz, b = (T)2 * alpha * bx;
     if constexpr (DIM == 3) {
-        if constexpr ((OCT >> 2) & 1) lane_pass<T, kSwapZ>(v0, v1, bz, (T)0, (T)1, ir2, ir2);
-        else lane_pass<T, kSwapZ>(v0, v1, bz, (T)1, (T)0, ir2, ir2);
+        const T c = (T)2 * by * bz, gk = c * k.s[0] * k.s[1];
+        const T w0 = a * u0 + b * u1 + gk * quad_swap<T, kSwapYZ>(u0);
+        const T w1 = a * u1 - b * u0 + gk * quad_swap<T, kSwapYZ>(u1);
+        const T e = a * a - b * b - c * c, f = (T)2 * a * b;
+        const T inv = fast_rcp(e * e + f * f);
+        v0 = (e * w0 - f * w1) * inv;
+        v1 = (e * w1 + f * w0) * inv;
+    } else {
+        const T inv = fast_rcp(a * a + b * b);
+        v0 = (a * u0 - b * u1) * inv;
+        v1 = (a * u1 + b * u0) * inv;
     }
-    // complexify along x, split y (and z), divide by alpha + i(beta_x +- beta_y +- beta_z)
-    T zr = v0, zi = -v1;
-    lane_split<T, kSwapY>(zr, zi, by);
-    if constexpr (DIM == 3) lane_split<T, kSwapZ>(zr, zi, bz);
-    T alpha = sig, li = r2 * t[0];
-    alpha += (T)2 * (t[0] + t[1]);
-    li += by ? -r2 * t[1] : r2 * t[1];
-    if constexpr (DIM == 3) {
-        alpha += (T)2 * t[2];
-        li += bz ? -r2 * t[2] : r2 * t[2];
-    }
+    // W
     {
-        const T inv = (T)1 / (alpha * alpha + li * li);
-        const T ar = zr, ai = zi;
-        zr = (ar * alpha + ai * li) * inv;
-        zi = (ai * alpha - ar * li) * inv;
+        const T w0 = v0, w1 = v1;
+        v0 = nx ? -w0 + r2 * w1 : w0;
+        v1 = nx ? w0 : -w0 + r2 * w1;
     }
-    if constexpr (DIM == 3) lane_merge<T, kSwapZ>(zr, zi, bz);
-    lane_merge<T, kSwapY>(zr, zi, by);
-    v0 = zr;
-    v1 = -zi;
-    // psi = W y
-    if constexpr (OCT & 1) passx((T)-1, r2, (T)1, (T)0);
-    else passx((T)1, (T)0, (T)-1, r2);
-    if constexpr ((OCT >> 1) & 1) lane_pass<T, kSwapY>(v0, v1, by, (T)-1, r2, (T)1, (T)0);
-    else lane_pass<T, kSwapY>(v0, v1, by, (T)1, (T)0, (T)-1, r2);
-    if constexpr (DIM == 3) {
-        if constexpr ((OCT >> 2) & 1) lane_pass<T, kSwapZ>(v0, v1, bz, (T)-1, r2, (T)1, (T)0);
-        else lane_pass<T, kSwapZ>(v0, v1, bz, (T)1, (T)0, (T)-1, r2);
-    }
+    lane_op<T, kSwapY>(v0, v1, k.w_m[0], k.w_p[0]);
+    if constexpr (DIM == 3) lane_op<T, kSwapZ>(v0, v1, k.w_m[1], k.w_p[1]);
 }
 
 // one lane's inputs of one cell, loaded ahead of its solve
 struct SplitIn {
     double q0, q1, i0, i1, sig;
     long long off;
-    int e, d;
-    int lc[3];
+    int e, d, ci;   // element, direction slot, box cell in level order
     bool ok;
 };
 
@@ -657,26 +698,19 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
     in.ok = false;
     in.q0 = in.q1 = in.i0 = in.i1 = in.sig = 0.0;
     in.off = 0;
-    in.e = in.d = 0;
-    in.lc[0] = in.lc[1] = in.lc[2] = 0;
+    in.e = in.d = in.ci = 0;
     if (l >= bp.nl) return;
     const BoxLds &L = cx.L;
     const int c0 = L.lpl[l], cnt = L.lpl[l + 1] - c0;
     if (it >= cnt * cx.ndir) return;
-    const int d = it / cnt, k = it - d * cnt;
-    const uint32_t cw = L.lcl[c0 + k];
-    in.lc[0] = (int)(cw & 1023u);
-    in.lc[1] = (int)((cw >> 10) & 1023u);
-    in.lc[2] = (int)(cw >> 20);
-    const int a = L.dirs[d];
-    if (a < 0 || in.lc[0] >= cx.ext[0] || in.lc[1] >= cx.ext[1] || in.lc[2] >= cx.ext[2]) return;
-    int c[3] = {0, 0, 0};
-#pragma unroll
-    for (int ax = 0; ax < DIM; ++ax)
-        c[ax] = ((OCT >> ax) & 1) ? cx.lo[ax] + cx.ext[ax] - 1 - in.lc[ax] : cx.lo[ax] + in.lc[ax];
-    in.e = c[0] + p.nel[0] * (c[1] + p.nel[1] * c[2]);
+    const int d = fast_div(it, cnt, 1.0f / (float)cnt), ci = c0 + (it - d * cnt);
+    const int e = L.cell_e[ci];
+    const long long base = L.dbase[d];
+    if (e < 0 || base < 0) return;
+    in.e = e;
     in.d = d;
-    in.off = ((long long)cx.g * p.Nomega + a) * p.Ndof + (long long)in.e * N + 2 * r;
+    in.ci = ci;
+    in.off = base + (long long)e * N + 2 * r;
     in.ok = true;
     if (p.src) {
         const d2v v = *reinterpret_cast<const d2v *>(p.src + in.off);
@@ -684,33 +718,35 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
         in.q1 = v.y;
     }
     if (p.iso) {
-        const d2v v = *reinterpret_cast<const d2v *>(p.iso + (long long)cx.g * p.Ndof + (long long)in.e * N + 2 * r);
+        const d2v v = *reinterpret_cast<const d2v *>(p.iso + (long long)cx.g * p.Ndof + (long long)e * N + 2 * r);
         in.i0 = v.x;
         in.i1 = v.y;
     }
-    in.sig = p.sigma[(long long)cx.g * p.Ne + in.e];
+    in.sig = p.sigma[(long long)cx.g * p.Ne + e];
 }
 
 template <int DIM, typename T, int OCT>
 __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int r,
-                                            const SplitIn &in)
+                                            const SplitIn &in, const LaneK<T> &lk)
 {
     constexpr int LPC = 1 << (DIM - 1), NF = LPC, PS = box_slot_stride(NF);
     const BoxLds &L = cx.L;
     const int by = r & 1, bz = (r >> 1) & 1, d = in.d;
     const double inv_vol = 1.0 / bp.vol;
-    // trace slots of this cell (the other axes' local coordinates)
+    // trace slots of this cell: the other axes' local coordinates
     double *slot[DIM];
     {
+        const uint32_t cw = L.lcl[in.ci];
+        const int lc[3] = {(int)(cw & 1023u), (int)((cw >> 10) & 1023u), (int)(cw >> 20)};
         double *pl = L.planes;
 #pragma unroll
         for (int ax = 0; ax < DIM; ++ax) {
             int sidx = 0, mul = 1;
 #pragma unroll
             for (int o = 0; o < DIM; ++o)
-                if (o != ax) { sidx += mul * in.lc[o]; mul *= bp.bs[o]; }
-            slot[ax] = pl + ((long long)d * bp.psz[ax] + sidx) * PS;
-            pl += (long long)bp.nd * bp.psz[ax] * PS;
+                if (o != ax) { sidx += mul * lc[o]; mul *= bp.bs[o]; }
+            slot[ax] = pl + (d * bp.psz[ax] + sidx) * PS;
+            pl += bp.nd * bp.psz[ax] * PS;
         }
     }
     // x face: node (., r); y face: (i0, i_z) -> i0 + 2 i_z; z face: (i0, i_y) -> i0 + 2 i_y
@@ -724,7 +760,7 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
     }
     const T t[3] = {(T)(L.om[3 * d] * inv_vol), (T)(L.om[3 * d + 1] * inv_vol), (T)(L.om[3 * d + 2] * inv_vol)};
     T v0 = (T)((in.q0 + in.i0) * inv_vol), v1 = (T)((in.q1 + in.i1) * inv_vol);
-    fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by, bz);
+    fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, lk);
     d2v o;
     o.x = (double)v0;
     o.y = (double)v1;
@@ -743,53 +779,81 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
     }
 }
 
-// The first pass of every level is prefetched during the level before (two
-// input buffers with alternating roles, so no register copy waits for loads
-// in flight; the LDS-only barrier lets them cross it).
-template <int DIM, typename T, int OCT>
+// PF 0: each cell's inputs are loaded when it is solved.  PF 2: the first
+// pass of every level is prefetched during the level before (two input
+// buffers with alternating roles, so no register copy waits for loads in
+// flight; the LDS-only barrier lets them cross it), at the price of registers.
+template <int DIM, typename T, int OCT, int PF>
 __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
 {
     constexpr int LPC = 1 << (DIM - 1);
     const BoxCtx cx = box_stage<DIM, 1, OCT>(p, bp, box_base, lds);
     const int tid = threadIdx.x, nt = blockDim.x;
     const int r = tid & (LPC - 1), qd = tid / LPC, nq = nt / LPC;
+    const LaneK<T> lk = lane_constants<DIM, T, OCT>(r & 1, (r >> 1) & 1);
+    if constexpr (PF == 0) {
+        for (int l = 0; l < bp.nl; ++l) {
+            const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
+            for (int it = qd; it < items; it += nq) {   // uniform over the lanes of a cell
+                SplitIn in;
+                split_fetch<DIM, OCT>(p, bp, cx, l, it, r, in);
+                if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in, lk);
+            }
+            lds_barrier();
+        }
+        return;
+    }
+    // AHEAD levels in flight: the first pass of level l + AHEAD is fetched
+    // while level l solves (PF 2: one level ahead, PF 3: two)
+    constexpr int AHEAD = PF == 3 ? 2 : 1;
     auto level = [&](int l, SplitIn &cur, SplitIn &nxt) {
         const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
-        split_fetch<DIM, OCT>(p, bp, cx, l + 1, qd, r, nxt);
-        if (cur.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, cur);
+        split_fetch<DIM, OCT>(p, bp, cx, l + AHEAD, qd, r, nxt);
+        if (cur.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, cur, lk);
         for (int it = qd + nq; it < items; it += nq) {   // uniform over the lanes of a cell
             SplitIn in;
             split_fetch<DIM, OCT>(p, bp, cx, l, it, r, in);
-            if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in);
+            if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in, lk);
         }
         lds_barrier();
     };
-    SplitIn b0, b1;
-    split_fetch<DIM, OCT>(p, bp, cx, 0, qd, r, b0);
-    for (int l = 0; l < bp.nl; l += 2) {
-        level(l, b0, b1);
-        if (l + 1 < bp.nl) level(l + 1, b1, b0);
+    if constexpr (AHEAD == 1) {
+        SplitIn b0, b1;
+        split_fetch<DIM, OCT>(p, bp, cx, 0, qd, r, b0);
+        for (int l = 0; l < bp.nl; l += 2) {
+            level(l, b0, b1);
+            if (l + 1 < bp.nl) level(l + 1, b1, b0);
+        }
+    } else {
+        SplitIn b0, b1, b2;
+        split_fetch<DIM, OCT>(p, bp, cx, 0, qd, r, b0);
+        split_fetch<DIM, OCT>(p, bp, cx, 1, qd, r, b1);
+        for (int l = 0; l < bp.nl; l += 3) {
+            level(l, b0, b2);
+            if (l + 1 < bp.nl) level(l + 1, b1, b0);
+            if (l + 2 < bp.nl) level(l + 2, b2, b1);
+        }
     }
 }
 
-template <int DIM, typename T>
+template <int DIM, typename T, int PF>
 __global__ void __launch_bounds__(1024) k_sweep_box_split(SweepParams p, BoxPlan bp, int box_base)
 {
     extern __shared__ double lds[];
     const int oct = blockIdx.x % bp.noct;   // workgroup-uniform
     switch (oct) {
-    case 0: box_task_split<DIM, T, 0>(p, bp, box_base, lds); break;
-    case 1: box_task_split<DIM, T, 1>(p, bp, box_base, lds); break;
-    case 2: box_task_split<DIM, T, 2>(p, bp, box_base, lds); break;
-    case 3: box_task_split<DIM, T, 3>(p, bp, box_base, lds); break;
-    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4>(p, bp, box_base, lds); break;
-    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5>(p, bp, box_base, lds); break;
-    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6>(p, bp, box_base, lds); break;
-    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7>(p, bp, box_base, lds); break;
+    case 0: box_task_split<DIM, T, 0, PF>(p, bp, box_base, lds); break;
+    case 1: box_task_split<DIM, T, 1, PF>(p, bp, box_base, lds); break;
+    case 2: box_task_split<DIM, T, 2, PF>(p, bp, box_base, lds); break;
+    case 3: box_task_split<DIM, T, 3, PF>(p, bp, box_base, lds); break;
+    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4, PF>(p, bp, box_base, lds); break;
+    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5, PF>(p, bp, box_base, lds); break;
+    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6, PF>(p, bp, box_base, lds); break;
+    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7, PF>(p, bp, box_base, lds); break;
     }
 }
 
-template <int DIM, typename T>
+template <int DIM, typename T, int PF>
 static hipError_t run_boxes_split(const DeviceSweep &ds, const SweepParams &prm, hipStream_t s)
 {
     const BoxPlan &bp = ds.box;
@@ -797,7 +861,7 @@ static hipError_t run_boxes_split(const DeviceSweep &ds, const SweepParams &prm,
     for (int k = 0; k < nlev; ++k) {
         const int nbox = ds.box_ptr[k + 1] - ds.box_ptr[k];
         if (nbox == 0) continue;
-        hipLaunchKernelGGL((k_sweep_box_split<DIM, T>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
+        hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
                            dim3(ds.box_threads), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
     }
     return hipGetLastError();
@@ -847,8 +911,14 @@ static hipError_t run_boxes_p(const DeviceSweep &ds, const SweepParams &prm, hip
     if constexpr (P == 1) {
         // lane-split cells; the workgroup must hold whole cells
         if (ds.box_threads % (1 << (DIM - 1))) return hipErrorInvalidValue;
-        if (ds.precision == 1) return run_boxes_split<DIM, float>(ds, prm, s);
-        return run_boxes_split<DIM, double>(ds, prm, s);
+        if (ds.precision == 1) {
+            if (ds.box_prefetch == 3) return run_boxes_split<DIM, float, 3>(ds, prm, s);
+            if (ds.box_prefetch == 2) return run_boxes_split<DIM, float, 2>(ds, prm, s);
+            return run_boxes_split<DIM, float, 0>(ds, prm, s);
+        }
+        if (ds.box_prefetch == 3) return run_boxes_split<DIM, double, 3>(ds, prm, s);
+        if (ds.box_prefetch == 2) return run_boxes_split<DIM, double, 2>(ds, prm, s);
+        return run_boxes_split<DIM, double, 0>(ds, prm, s);
     }
     if (ds.box_threads > 256) return hipErrorInvalidValue;
     if (ds.precision == 1) return run_boxes_pf<DIM, P, float>(ds, prm, s);

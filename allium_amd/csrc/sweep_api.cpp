@@ -254,7 +254,8 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
     if (threads < 64 || threads > max_threads || threads % 64)
         return fail(ALLIUM_ERR_ARG, "box sweep workgroup size must be a multiple of 64 in 64.." +
                                         std::to_string(max_threads));
-    if (prefetch != 0 && prefetch != 2) return fail(ALLIUM_ERR_ARG, "box sweep prefetch must be 0 or 2");
+    if (prefetch != 0 && prefetch != 2 && !(prefetch == 3 && h->ed.p == 1))
+        return fail(ALLIUM_ERR_ARG, "box sweep prefetch must be 0 or 2 (or 3 at p = 1)");
     const int No = h->quad.size(), noct = 1 << dim;
     std::vector<std::vector<int>> oct(noct);
     for (int a = 0; a < No; a++) {
@@ -288,8 +289,10 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
         plane += bp.psz[ax];
     }
     const long vol = (long)bs[0] * bs[1] * bs[2];
+    // p = 1 adds per-direction psi offsets and a per-cell element table
+    const long split = h->ed.p == 1 ? 1 : 0;
     const long lds = 8L * ((long)nd * allium::box_slot_stride(nf) * plane + (long)nd * bp.kstride + 3L * nd) +
-                     4L * (nd + vol + bp.nl + 1);
+                     4L * (nd + vol + bp.nl + 1) + split * (8L * nd + 4L * vol);
     // the box kernel builds Mu and the face masses as tensor products of the
     // exact 1D masses: check that the assembled matrices are those
     {
@@ -619,10 +622,10 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
                        allium::box_sweep_supported(h->mesh.dim, h->ed.p);
     if (boxes) {
         const bool d3 = h->mesh.dim == 3;
-        // p = 1: lane-split cells (no prefetch variant), workgroups up to 1024
+        // p = 1: lane-split cells, workgroups up to 1024
         const bool split = h->ed.p == 1;
         const std::vector<int> ths = split ? std::vector<int>{256, 512, 1024} : std::vector<int>{128, 256};
-        const std::vector<int> pfs = split ? std::vector<int>{0} : std::vector<int>{0, 2};
+        const std::vector<int> pfs = split ? std::vector<int>{0, 2, 3} : std::vector<int>{0, 2};
         for (int b : d3 ? std::vector<int>{4, 8, 16} : std::vector<int>{8, 16, 32})
             for (int nd : {1, 2, 3, 5, 10})
                 for (int th : ths)
