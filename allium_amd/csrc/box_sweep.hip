@@ -564,47 +564,10 @@ constexpr int kSwapY = 0xB1;    // quad_perm [1,0,3,2]: partner along y (lane bi
 constexpr int kSwapZ = 0x4E;    // quad_perm [2,3,0,1]: partner along z (lane bit 1)
 constexpr int kSwapYZ = 0x1B;   // quad_perm [3,2,1,0]: partner along y and z
 
-// Per-lane constants of the split solve.  A 2x2 operator a along an axis
-// becomes v <- cm v + cp v(partner) with (cm, cp) = (a00, a01) on the bit-0
-// lane and (a11, a10) on the bit-1 lane; J = [[0,1],[-1,0]] is s * partner
-// with s = +1 / -1 on the bit-0 / bit-1 lane.
-template <typename T>
-struct LaneK {
-    T wi_m[2], wi_p[2];   // W^{-1} along y, z
-    T w_m[2], w_p[2];     // W along y, z
-    T s[2];               // J sign along y, z
-    T cf[2];              // (m^{-1} e_in)[own index] along y, z
-};
-
-template <int DIM, typename T, int OCT>
-__device__ __forceinline__ LaneK<T> lane_constants(int by, int bz)
-{
-    constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
-    LaneK<T> k;
-    const int bit[2] = {by, bz};
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const bool neg = (OCT >> (j + 1)) & 1;
-        const int b = bit[j];
-        // W+^{-1} = [[1,0],[1/r2,1/r2]], W-^{-1} = [[0,1],[1/r2,1/r2]]
-        k.wi_m[j] = b ? ir2 : (neg ? (T)0 : (T)1);
-        k.wi_p[j] = b ? ir2 : (neg ? (T)1 : (T)0);
-        // W+ = [[1,0],[-1,r2]], W- = [[-1,r2],[1,0]]
-        k.w_m[j] = b ? (neg ? (T)0 : r2) : (neg ? (T)-1 : (T)1);
-        k.w_p[j] = b ? (neg ? (T)1 : (T)-1) : (neg ? r2 : (T)0);
-        k.s[j] = b ? (T)-1 : (T)1;
-        k.cf[j] = (b == (neg ? 1 : 0)) ? (T)4 : (T)-2;
-    }
-    return k;
-}
-
-template <typename T, int CTRL>
-__device__ __forceinline__ void lane_op(T &v0, T &v1, T cm, T cp)
-{
-    const T p0 = quad_swap<T, CTRL>(v0), p1 = quad_swap<T, CTRL>(v1);
-    v0 = cm * v0 + cp * p0;
-    v1 = cm * v1 + cp * p1;
-}
+// Operators along y and z act between the lanes of a cell: lane bit b is the
+// node index on that axis, and the partner lane (quad_perm swap) holds the
+// other node.  J = [[0,1],[-1,0]] is the partner's value with sign + on the
+// bit-0 lane and - on the bit-1 lane.
 
 // psi on this lane's nodes (0, r), (1, r) from (v0, v1) = b / vol there:
 // r = M^{-1} b + inflow terms, y = W^{-1} r, then
@@ -614,55 +577,77 @@ __device__ __forceinline__ void lane_op(T &v0, T &v1, T cm, T cp)
 // B^2 - c^2 act inside the lane's x pair; psi = W y.
 template <int DIM, typename T, int OCT>
 __device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig, T ux, const T (&uy)[2],
-                                            const T (&uz)[2], const LaneK<T> &k)
+                                            const T (&uz)[2], bool hy, bool hz)
 {
     constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
-    constexpr bool nx = OCT & 1;
+    constexpr bool nx = OCT & 1, ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
     // M^{-1}: m^{-1} = [[4,-2],[-2,4]] on every axis
     {
         const T w0 = v0, w1 = v1;
         v0 = (T)4 * w0 - (T)2 * w1;
         v1 = (T)4 * w1 - (T)2 * w0;
     }
-    lane_op<T, kSwapY>(v0, v1, (T)4, (T)-2);
-    if constexpr (DIM == 3) lane_op<T, kSwapZ>(v0, v1, (T)4, (T)-2);
-    // inflow faces: t_ax (m^{-1} e_in) (x) u_ax
+    {
+        const T p0 = quad_swap<T, kSwapY>(v0), p1 = quad_swap<T, kSwapY>(v1);
+        v0 = (T)4 * v0 - (T)2 * p0;
+        v1 = (T)4 * v1 - (T)2 * p1;
+    }
+    if constexpr (DIM == 3) {
+        const T p0 = quad_swap<T, kSwapZ>(v0), p1 = quad_swap<T, kSwapZ>(v1);
+        v0 = (T)4 * v0 - (T)2 * p0;
+        v1 = (T)4 * v1 - (T)2 * p1;
+    }
+    // inflow faces: t_ax (m^{-1} e_in) (x) u_ax; (m^{-1} e_in) = 4 on the inflow node, -2 on the other
     v0 += t[0] * (nx ? (T)-2 : (T)4) * ux;
     v1 += t[0] * (nx ? (T)4 : (T)-2) * ux;
-    v0 += t[1] * k.cf[0] * uy[0];
-    v1 += t[1] * k.cf[0] * uy[1];
-    if constexpr (DIM == 3) {
-        v0 += t[2] * k.cf[1] * uz[0];
-        v1 += t[2] * k.cf[1] * uz[1];
+    {
+        const T wy = t[1] * ((hy == ny) ? (T)4 : (T)-2);
+        v0 += wy * uy[0];
+        v1 += wy * uy[1];
     }
-    // W^{-1}
+    if constexpr (DIM == 3) {
+        const T wz = t[2] * ((hz == nz) ? (T)4 : (T)-2);
+        v0 += wz * uz[0];
+        v1 += wz * uz[1];
+    }
+    // W^{-1}: W+^{-1} = [[1,0],[1/r2,1/r2]], W-^{-1} = [[0,1],[1/r2,1/r2]]
     {
         const T w0 = v0, w1 = v1;
         v0 = nx ? w1 : w0;
         v1 = ir2 * (w0 + w1);
     }
-    lane_op<T, kSwapY>(v0, v1, k.wi_m[0], k.wi_p[0]);
-    if constexpr (DIM == 3) lane_op<T, kSwapZ>(v0, v1, k.wi_m[1], k.wi_p[1]);
+    auto winv = [&](auto swp, bool hi, bool neg) {
+        const T p0 = swp(v0), p1 = swp(v1);
+        const T a0 = ir2 * (v0 + p0), a1 = ir2 * (v1 + p1);
+        v0 = hi ? a0 : (neg ? p0 : v0);
+        v1 = hi ? a1 : (neg ? p1 : v1);
+    };
+    winv([](T x) { return quad_swap<T, kSwapY>(x); }, hy, ny);
+    if constexpr (DIM == 3) winv([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
     // S^{-1}
     T alpha = sig + (T)2 * (t[0] + t[1]);
     if constexpr (DIM == 3) alpha += (T)2 * t[2];
     const T bx = r2 * t[0], by = r2 * t[1], bz = DIM == 3 ? r2 * t[2] : (T)0;
     T u0 = alpha * v0 + bx * v1, u1 = alpha * v1 - bx * v0;   // A_x v
     {
-        const T gy = by * k.s[0];
-        u0 -= gy * quad_swap<T, kSwapY>(v0);
-        u1 -= gy * quad_swap<T, kSwapY>(v1);
+        // - beta_y J_y v: J_y v = +partner on the bit-0 lane, -partner on the bit-1 lane
+        const T gy0 = by * quad_swap<T, kSwapY>(v0), gy1 = by * quad_swap<T, kSwapY>(v1);
+        u0 = hy ? u0 + gy0 : u0 - gy0;
+        u1 = hy ? u1 + gy1 : u1 - gy1;
         if constexpr (DIM == 3) {
-            const T gz = bz * k.s[1];
-            u0 -= gz * quad_swap<T, kSwapZ>(v0);
-            u1 -= gz * quad_swap<T, kSwapZ>(v1);
+            const T gz0 = bz * quad_swap<T, kSwapZ>(v0), gz1 = bz * quad_swap<T, kSwapZ>(v1);
+            u0 = hz ? u0 + gz0 : u0 - gz0;
+            u1 = hz ? u1 + gz1 : u1 - gz1;
         }
     }
     const T a = alpha * alpha - bx * bx + by * by + bz * bz, b = (T)2 * alpha * bx;
     if constexpr (DIM == 3) {
-        const T c = (T)2 * by * bz, gk = c * k.s[0] * k.s[1];
-        const T w0 = a * u0 + b * u1 + gk * quad_swap<T, kSwapYZ>(u0);
-        const T w1 = a * u1 - b * u0 + gk * quad_swap<T, kSwapYZ>(u1);
+        // K u = s_y s_z u(partner along y and z): + when the bits agree
+        const T c = (T)2 * by * bz;
+        const T k0 = c * quad_swap<T, kSwapYZ>(u0), k1 = c * quad_swap<T, kSwapYZ>(u1);
+        const bool same = hy == hz;
+        const T w0 = a * u0 + b * u1 + (same ? k0 : -k0);
+        const T w1 = a * u1 - b * u0 + (same ? k1 : -k1);
         const T e = a * a - b * b - c * c, f = (T)2 * a * b;
         const T inv = fast_rcp(e * e + f * f);
         v0 = (e * w0 - f * w1) * inv;
@@ -672,14 +657,24 @@ __device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig
         v0 = (a * u0 - b * u1) * inv;
         v1 = (a * u1 + b * u0) * inv;
     }
-    // W
+    // W: W+ = [[1,0],[-1,r2]], W- = [[-1,r2],[1,0]]
     {
         const T w0 = v0, w1 = v1;
         v0 = nx ? -w0 + r2 * w1 : w0;
         v1 = nx ? w0 : -w0 + r2 * w1;
     }
-    lane_op<T, kSwapY>(v0, v1, k.w_m[0], k.w_p[0]);
-    if constexpr (DIM == 3) lane_op<T, kSwapZ>(v0, v1, k.w_m[1], k.w_p[1]);
+    auto wfwd = [&](auto swp, bool hi, bool neg) {
+        const T p0 = swp(v0), p1 = swp(v1);
+        if (neg) {
+            v0 = hi ? p0 : r2 * p0 - v0;
+            v1 = hi ? p1 : r2 * p1 - v1;
+        } else {
+            v0 = hi ? r2 * v0 - p0 : v0;
+            v1 = hi ? r2 * v1 - p1 : v1;
+        }
+    };
+    wfwd([](T x) { return quad_swap<T, kSwapY>(x); }, hy, ny);
+    if constexpr (DIM == 3) wfwd([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
 }
 
 // one lane's inputs of one cell, loaded ahead of its solve
@@ -727,7 +722,7 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
 
 template <int DIM, typename T, int OCT>
 __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int r,
-                                            const SplitIn &in, const LaneK<T> &lk)
+                                            const SplitIn &in)
 {
     constexpr int LPC = 1 << (DIM - 1), NF = LPC, PS = box_slot_stride(NF);
     const BoxLds &L = cx.L;
@@ -760,7 +755,7 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
     }
     const T t[3] = {(T)(L.om[3 * d] * inv_vol), (T)(L.om[3 * d + 1] * inv_vol), (T)(L.om[3 * d + 2] * inv_vol)};
     T v0 = (T)((in.q0 + in.i0) * inv_vol), v1 = (T)((in.q1 + in.i1) * inv_vol);
-    fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, lk);
+    fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by != 0, bz != 0);
     d2v o;
     o.x = (double)v0;
     o.y = (double)v1;
@@ -790,14 +785,13 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     const BoxCtx cx = box_stage<DIM, 1, OCT>(p, bp, box_base, lds);
     const int tid = threadIdx.x, nt = blockDim.x;
     const int r = tid & (LPC - 1), qd = tid / LPC, nq = nt / LPC;
-    const LaneK<T> lk = lane_constants<DIM, T, OCT>(r & 1, (r >> 1) & 1);
     if constexpr (PF == 0) {
         for (int l = 0; l < bp.nl; ++l) {
             const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
             for (int it = qd; it < items; it += nq) {   // uniform over the lanes of a cell
                 SplitIn in;
                 split_fetch<DIM, OCT>(p, bp, cx, l, it, r, in);
-                if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in, lk);
+                if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in);
             }
             lds_barrier();
         }
@@ -809,11 +803,11 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     auto level = [&](int l, SplitIn &cur, SplitIn &nxt) {
         const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
         split_fetch<DIM, OCT>(p, bp, cx, l + AHEAD, qd, r, nxt);
-        if (cur.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, cur, lk);
+        if (cur.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, cur);
         for (int it = qd + nq; it < items; it += nq) {   // uniform over the lanes of a cell
             SplitIn in;
             split_fetch<DIM, OCT>(p, bp, cx, l, it, r, in);
-            if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in, lk);
+            if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in);
         }
         lds_barrier();
     };
