@@ -429,14 +429,15 @@ class DSA:
 
 
 def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000, precision=FP64,
-                     want_psi=True, accel=None, history=None, prolong=1, patience=2, fallback=None):
+                     want_psi=True, accel=None, history=None, prolong=1, patience=3, fallback=None):
     """chive's SI: returns (phi, psi, iterations, norms) (exe/chive.cpp:700-735).
 
     ``accel``: a DSA object (the SLISolver preconditioner: every iteration is
     corrected and the norm is that of the corrected change), or a
-    cnn.FluxCorrector applied after every iteration (switched off once the
-    residual has failed to drop ``patience`` times in a row, after which
-    ``fallback``, a DSA object or None, takes over); ``prolong`` 1 spreads the
+    cnn.FluxCorrector applied after every iteration (switched off once
+    ``patience`` iterations have passed without a new lowest residual; the
+    iterate of the lowest residual is restored and ``fallback``, a DSA object
+    or None, takes over); ``prolong`` 1 spreads the
     network's element corrections continuously over the vertices (p = 1), 0
     keeps them element-constant.  ``history``: a (maxit, 2, Ne) float64 array
     that receives, for every iteration but the last, the element-averaged

@@ -99,6 +99,8 @@ SIGNATURES = {
     "allium_cnn_create": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.POINTER(c_vp)]),
     "allium_cnn_destroy": (None, [c_vp]),
     "allium_cnn_num_params": (ctypes.c_long, []),
+    "allium_cnn_set_dilation": (ctypes.c_int, [c_vp, c_ip]),
+    "allium_cnn_get_dilation": (ctypes.c_int, [c_vp, c_ip]),
     "allium_cnn_get_params": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
     "allium_cnn_set_params": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
     "allium_cnn_forward_device": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp,

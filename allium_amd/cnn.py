@@ -45,12 +45,24 @@ class DeviceArray32:
 class FluxCorrector:
     """The correction network on the GPU (BF16 MFMA kernels, FP32 master weights)."""
 
-    def __init__(self, seed=0):
+    def __init__(self, seed=0, dilation=None):
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         check(self._lib.allium_cnn_create(ctypes.c_ulonglong(seed), ctypes.byref(h)))
         self._h = h
         self.nparams = int(self._lib.allium_cnn_num_params())
+        if dilation is not None:
+            self.set_dilation(dilation)
+
+    def set_dilation(self, dilation):
+        """Per-layer dilation of the 3x3x3 taps (5 values in 1..8)."""
+        d = (ctypes.c_int * 5)(*[int(v) for v in dilation])
+        check(self._lib.allium_cnn_set_dilation(self._h, d))
+
+    def dilation(self):
+        d = (ctypes.c_int * 5)()
+        check(self._lib.allium_cnn_get_dilation(self._h, d))
+        return tuple(d)
 
     @property
     def handle(self):
@@ -145,7 +157,7 @@ def generate_training_data(n=32, media=200, seed0=1000, iters=(1, 2, 3, 4, 6, 8,
         q = Linv.form_source(Q=1.0)
         hist = np.zeros((maxit, 2, n ** 3))
         phi, _, it, _ = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=maxit, want_psi=False,
-                                           history=hist, accel=accel, patience=2)
+                                           history=hist, accel=accel, patience=3)
         phi_avg = phi.reshape(-1, 8).mean(axis=1)
         X, T = pairs_from_history(hist[:it - 1], phi_avg, cfg["sigma_t"].reshape(-1), cfg["c"], shape, set(iters))
         Xs += X
@@ -155,20 +167,29 @@ def generate_training_data(n=32, media=200, seed0=1000, iters=(1, 2, 3, 4, 6, 8,
     return np.stack(Xs).astype(np.float32), np.stack(Ts).astype(np.float32)
 
 
-def fit(n=32, media=60, rounds=2, steps=1500, seed0=1000, sn=8, log=None, seed=1):
+# dilation of the 3x3x3 taps per layer (receptive field 33 cells) and the
+# iterations whose (change, converged - iterate) pairs train the network:
+# the asymptotic phase of source iteration is where most iterations go
+DILATION = (1, 2, 4, 8, 1)
+ITERS = (1, 2, 3, 4, 6, 8, 11, 15, 20, 28, 40, 55)
+
+
+def fit(n=32, media=60, rounds=2, steps=1500, seed0=1000, sn=8, log=None, seed=1, maxit=1000, dilation=DILATION,
+        iters=ITERS):
     """Train a corrector on plain-iteration pairs, then for ``rounds`` more
     rounds add pairs from source iterations that use the current network (the
     iterates the correction actually meets) and train again.  Returns
     (corrector, info)."""
     t0 = time.perf_counter()
-    X, T = generate_training_data(n=n, media=media, seed0=seed0, sn=sn, maxit=1000)
-    net = FluxCorrector(seed)
+    X, T = generate_training_data(n=n, media=media, seed0=seed0, sn=sn, maxit=maxit, iters=iters)
+    net = FluxCorrector(seed, dilation=dilation)
     L = train(net, X, T, steps=steps, batch=8, lr=1e-3, seed=seed)
     info = {"rounds": [{"pairs": int(X.shape[0]), "loss": float(np.mean(L[-50:]))}]}
     if log:
         log(f"  round 0: {X.shape[0]} pairs, loss {info['rounds'][-1]['loss']:.3f}")
     for r in range(1, rounds + 1):
-        X2, T2 = generate_training_data(n=n, media=media, seed0=seed0 + 1000 * r, sn=sn, maxit=1000, accel=net)
+        X2, T2 = generate_training_data(n=n, media=media, seed0=seed0 + 1000 * r, sn=sn, maxit=maxit, accel=net,
+                                        iters=iters)
         X, T = np.concatenate([X, X2]), np.concatenate([T, T2])
         L = train(net, X, T, steps=steps, batch=8, lr=5e-4, seed=seed + r)
         info["rounds"].append({"pairs": int(X.shape[0]), "loss": float(np.mean(L[-50:]))})
@@ -176,6 +197,7 @@ def fit(n=32, media=60, rounds=2, steps=1500, seed0=1000, sn=8, log=None, seed=1
             log(f"  round {r}: {X.shape[0]} pairs, loss {info['rounds'][-1]['loss']:.3f}")
     info["seconds"] = time.perf_counter() - t0
     info["media_per_round"], info["grid"], info["steps_per_round"] = media, n, steps
+    info["dilation"], info["iterations_sampled"] = list(net.dilation()), list(iters)
     return net, info
 
 

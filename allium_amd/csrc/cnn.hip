@@ -44,7 +44,9 @@ constexpr int HID = 32;
 struct Geo {
     int B, nz, ny, nx;
     long long T;   // voxels
+    int dil;       // dilation of the 3x3x3 taps: offsets are dil * (-1, 0, 1) per axis
 };
+constexpr int kMaxDil = 8;
 
 // ---------------------------------------------------------------------------
 // convolution (forward, and backward-data with flipped/transposed weights)
@@ -85,7 +87,7 @@ __global__ void __launch_bounds__(256) k_conv(Geo g, const bf16 *__restrict__ in
 #pragma unroll
         for (int t = 0; t < 8; ++t) a[t] = (bf16)0.f;
         if (cv && mv) {
-            const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+            const int dz = (o / 9 - 1) * g.dil, dy = ((o / 3) % 3 - 1) * g.dil, dx = (o % 3 - 1) * g.dil;
             const int xx = x + dx, yy = y + dy, zz = z + dz;
             if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz) {
                 const long long mn = m + dx + (long long)g.nx * (dy + (long long)g.ny * dz);
@@ -130,8 +132,8 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
                                                float *__restrict__ part)
 {
     constexpr int NTO = COP / 16, NTI = (CIP + 15) / 16, CIT = NTI * 16, NPAIR = NTO * NTI;
-    constexpr int XW = 32;        // voxels per K step
-    constexpr int XS = XW + 8;    // staged x range (one-voxel halo each side), padded
+    constexpr int XW = 32;                // voxels per K step
+    constexpr int XS = XW + 2 * kMaxDil;  // staged x range (dil-voxel halo each side)
     static_assert(NPAIR <= 4, "one tile pair per wave");
     __shared__ __attribute__((aligned(16))) bf16 gT[COP][XW];
     __shared__ __attribute__((aligned(16))) bf16 aT[9][CIP][XS];
@@ -153,12 +155,13 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
                 const int xx = x0 + xi;
                 gT[co][xi] = xx < g.nx ? gz[(row + xx) * COP + co] : (bf16)0.f;
             }
-            for (int i = threadIdx.x; i < 9 * (XW + 2) * CIP; i += blockDim.x) {
+            const int dl = g.dil, xr = XW + 2 * dl;
+            for (int i = threadIdx.x; i < 9 * xr * CIP; i += blockDim.x) {
                 const int ci = i % CIP;
                 const int rest = i / CIP;
-                const int xi = rest % (XW + 2), r9 = rest / (XW + 2);
-                const int dz = r9 / 3 - 1, dy = r9 % 3 - 1;
-                const int xx = x0 - 1 + xi, yy = y + dy, zz = z + dz;
+                const int xi = rest % xr, r9 = rest / xr;
+                const int dz = (r9 / 3 - 1) * dl, dy = (r9 % 3 - 1) * dl;
+                const int xx = x0 - dl + xi, yy = y + dy, zz = z + dz;
                 bf16 v = (bf16)0.f;
                 if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
                     v = a[((((long long)b * g.nz + zz) * g.ny + yy) * g.nx + xx) * CIP + ci];
@@ -174,7 +177,8 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
                     const int r9 = (dz + 1) * 3 + (dy + 1);
                     bf16x8 bv;
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) bv[t] = ci < CIP ? aT[r9][ci][1 + dx + 8 * grp + t] : (bf16)0.f;
+                    for (int t = 0; t < 8; ++t)
+                        bv[t] = ci < CIP ? aT[r9][ci][g.dil * (1 + dx) + 8 * grp + t] : (bf16)0.f;
                     acc[o] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[o], 0, 0, 0);
                 }
             }
@@ -422,6 +426,7 @@ struct allium_cnn {
     float *out = nullptr, *part = nullptr, *bpart = nullptr, *xf = nullptr;
     double *lpart = nullptr, *ravg = nullptr, *rpart = nullptr, *scale = nullptr;
     long step = 0;
+    int dil[NL] = {1, 1, 1, 1, 1};
     hipStream_t stream = nullptr;
     std::vector<void *> owned;
     ~allium_cnn()
@@ -486,19 +491,26 @@ int pack_weights(allium_cnn *h, hipStream_t s)
 }
 
 // forward from the packed bf16 input h->xin; output FP32 h->out
+Geo layer_geo(const allium_cnn *h, const Geo &g, int l)
+{
+    Geo gl = g;
+    gl.dil = h->dil[l];
+    return gl;
+}
+
 int forward_packed(allium_cnn *h, const Geo &g, hipStream_t s)
 {
     const int grid = (int)((g.T + 63) / 64);
-    hipLaunchKernelGGL((k_conv<8, 32, 0>), dim3(grid), dim3(256), 0, s, g, h->xin, h->Wf[0], h->bias[0], h->act[0],
-                       nullptr, nullptr);
-    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, g, h->act[0], h->Wf[1], h->bias[1], h->act[1],
-                       nullptr, nullptr);
-    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, g, h->act[1], h->Wf[2], h->bias[2], h->act[2],
-                       nullptr, nullptr);
-    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, g, h->act[2], h->Wf[3], h->bias[3], h->act[3],
-                       nullptr, nullptr);
-    hipLaunchKernelGGL((k_conv<32, 16, 1>), dim3(grid), dim3(256), 0, s, g, h->act[3], h->Wf[4], h->bias[4], nullptr,
-                       h->out, nullptr);
+    hipLaunchKernelGGL((k_conv<8, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 0), h->xin, h->Wf[0],
+                       h->bias[0], h->act[0], nullptr, nullptr);
+    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 1), h->act[0], h->Wf[1],
+                       h->bias[1], h->act[1], nullptr, nullptr);
+    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 2), h->act[1], h->Wf[2],
+                       h->bias[2], h->act[2], nullptr, nullptr);
+    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 3), h->act[2], h->Wf[3],
+                       h->bias[3], h->act[3], nullptr, nullptr);
+    hipLaunchKernelGGL((k_conv<32, 16, 1>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 4), h->act[3], h->Wf[4],
+                       h->bias[4], nullptr, h->out, nullptr);
     CNN_TRY(hipGetLastError());
     return 0;
 }
@@ -511,6 +523,7 @@ int check_geo(int B, int nz, int ny, int nx, Geo &g)
     g.ny = ny;
     g.nx = nx;
     g.T = (long long)B * nz * ny * nx;
+    g.dil = 1;
     return 0;
 }
 
@@ -535,14 +548,15 @@ int gradient(allium_cnn *h, const Geo &g, const float *d_x, const float *d_t, do
         float *gW = h->grad + layer_offset(l);
         float *gB = gW + 27L * CO_[l] * CI_[l];
         const int CIT = ((CIP_[l] + 15) / 16) * 16;
+        const Geo gl = layer_geo(h, g, l);
         if (l == NL - 1) {
-            hipLaunchKernelGGL((k_wgrad<32, 16>), dim3(planes), dim3(256), 0, s, g, gz, ain, h->part);
+            hipLaunchKernelGGL((k_wgrad<32, 16>), dim3(planes), dim3(256), 0, s, gl, gz, ain, h->part);
             hipLaunchKernelGGL(k_bias_grad<16>, dim3(kBiasBlocks), dim3(256), 0, s, g.T, gz, h->bpart);
         } else if (l == 0) {
-            hipLaunchKernelGGL((k_wgrad<8, 32>), dim3(planes), dim3(256), 0, s, g, gz, ain, h->part);
+            hipLaunchKernelGGL((k_wgrad<8, 32>), dim3(planes), dim3(256), 0, s, gl, gz, ain, h->part);
             hipLaunchKernelGGL(k_bias_grad<32>, dim3(kBiasBlocks), dim3(256), 0, s, g.T, gz, h->bpart);
         } else {
-            hipLaunchKernelGGL((k_wgrad<32, 32>), dim3(planes), dim3(256), 0, s, g, gz, ain, h->part);
+            hipLaunchKernelGGL((k_wgrad<32, 32>), dim3(planes), dim3(256), 0, s, gl, gz, ain, h->part);
             hipLaunchKernelGGL(k_bias_grad<32>, dim3(kBiasBlocks), dim3(256), 0, s, g.T, gz, h->bpart);
         }
         hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(27L * CO_[l] * CI_[l])), dim3(256), 0, s, h->part, planes,
@@ -551,10 +565,10 @@ int gradient(allium_cnn *h, const Geo &g, const float *d_x, const float *d_t, do
         if (l > 0) {
             // gradient of the previous pre-activation: conv with flipped weights, ReLU mask of its output
             if (l == NL - 1)
-                hipLaunchKernelGGL((k_conv<16, 32, 2>), dim3(grid), dim3(256), 0, s, g, gz, h->Wb[l], nullptr, gnext,
+                hipLaunchKernelGGL((k_conv<16, 32, 2>), dim3(grid), dim3(256), 0, s, gl, gz, h->Wb[l], nullptr, gnext,
                                    nullptr, h->act[l - 1]);
             else
-                hipLaunchKernelGGL((k_conv<32, 32, 2>), dim3(grid), dim3(256), 0, s, g, gz, h->Wb[l], nullptr, gnext,
+                hipLaunchKernelGGL((k_conv<32, 32, 2>), dim3(grid), dim3(256), 0, s, gl, gz, h->Wb[l], nullptr, gnext,
                                    nullptr, h->act[l - 1]);
             const bf16 *t = gnext;
             gnext = const_cast<bf16 *>(gz);
@@ -612,6 +626,22 @@ int correct(allium_cnn *h, int nz, int ny, int nx, int n, const double *d_w, con
 extern "C" {
 
 long allium_cnn_num_params(void) { return NPARAM; }
+
+int allium_cnn_set_dilation(allium_cnn_t h, const int *dilation)
+{
+    if (!h || !dilation) return cfail(-1, "null argument");
+    for (int l = 0; l < NL; ++l)
+        if (dilation[l] < 1 || dilation[l] > kMaxDil) return cfail(-1, "dilation must be 1..8 per layer");
+    for (int l = 0; l < NL; ++l) h->dil[l] = dilation[l];
+    return 0;
+}
+
+int allium_cnn_get_dilation(allium_cnn_t h, int *dilation)
+{
+    if (!h || !dilation) return cfail(-1, "null argument");
+    for (int l = 0; l < NL; ++l) dilation[l] = h->dil[l];
+    return 0;
+}
 
 int allium_cnn_create(unsigned long long seed, allium_cnn_t *out)
 {

@@ -202,14 +202,18 @@ __global__ void __launch_bounds__(kBlock) k_st_prolong(StLevel F, StLevel C, con
     xf[v] += acc;
 }
 
+// one wave per row of the row-major inverse: lanes stride over the columns,
+// then a wave reduction
 __global__ void __launch_bounds__(kBlock) k_dense(long long N, const double *__restrict__ Ainv,
                                                   const double *__restrict__ b, double *__restrict__ x)
 {
-    const long long r = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (r >= N) return;
+    const long long r = (long long)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    if (r >= N) return;   // wave-uniform
     double acc = 0.0;
-    for (long long c = 0; c < N; c++) acc = fma(Ainv[c * N + r], b[c], acc);
-    x[r] = acc;
+    for (long long c = lane; c < N; c += 64) acc = fma(Ainv[r * N + c], b[c], acc);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) x[r] = acc;
 }
 
 __global__ void __launch_bounds__(kBlock) k_dot(long long n, const double *__restrict__ a, const double *__restrict__ b,
@@ -326,7 +330,8 @@ hipError_t st_prolong_add(const StLevel &F, const StLevel &C, const double *xc, 
 
 hipError_t dense_apply(long long N, const double *Ainv, const double *b, double *x, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dense, dim3(grid_of(N)), dim3(kBlock), 0, s, N, Ainv, b, x);
+    hipLaunchKernelGGL(k_dense, dim3((unsigned)((N + kBlock / 64 - 1) / (kBlock / 64))), dim3(kBlock), 0, s, N, Ainv,
+                       b, x);
     return hipGetLastError();
 }
 

@@ -296,7 +296,7 @@ struct allium_dsa {
     allium::DgLevel dg{};
     std::vector<allium::StLevel> st;
     std::vector<double *> sb, sx, st_t, sr;   // per vertex level: rhs, solution, ping-pong, residual
-    double *Ainv = nullptr;                   // dense coarsest inverse (column-major) or null
+    double *Ainv = nullptr;                   // dense coarsest inverse (row-major) or null
     double *sigs = nullptr;
     // CG and V-cycle work vectors (phi-sized)
     double *bv = nullptr, *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv = nullptr, *qv = nullptr;
@@ -543,10 +543,8 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
                 Ainv[(size_t)v * Nc + xx + Lc.nv[0] * (yy + Lc.nv[1] * zz)] += Lc.A[(size_t)k * Lc.N + v];
             }
         }
+        // row-major: each device wave reads one row
         if (!invert(Nc, Ainv.data())) return fail(ALLIUM_ERR_NUMERIC, "DSA: singular coarsest operator");
-        // column-major for the device: entry (r, c) at c * N + r
-        for (int r = 0; r < Nc; r++)
-            for (int c = r + 1; c < Nc; c++) std::swap(Ainv[(size_t)r * Nc + c], Ainv[(size_t)c * Nc + r]);
     }
     // upload: DG blocks column-major
     auto colmajor = [&](const std::vector<double> &src, std::vector<double> &dst) {

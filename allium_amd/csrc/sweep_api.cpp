@@ -884,25 +884,32 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
     if (cnn && (opt->accel_prolong < 0 || opt->accel_prolong > 1 || (opt->accel_prolong == 1 && n != 8)))
         return fail(ALLIUM_ERR_ARG, "accel_prolong must be 0, or 1 with p = 1");
     bool cnn_on = cnn != nullptr;
-    int fails = 0;
     const int patience = std::max(1, opt->accel_patience);
-    double prev_nrm = 0.0;
-    // after x += r: apply the learned correction (until the residual has
-    // failed to drop `patience` times in a row) and record the history
+    double best = 0.0;
+    int best_it = 0;
+    if (cnn) HIP_TRY(h->xb.ensure(sizeof(double) * np));
+    // after x += r: apply the learned correction until `patience` iterations
+    // have passed without a new lowest residual; then the iterate of the
+    // lowest residual is restored and plain SI (or the DSA fallback) goes on
+    // from it.  Record the history.
     auto after_update = [&](double nrm) -> int {
         if (cnn_on) {
-            fails = (it > 1 && !(nrm < prev_nrm)) ? fails + 1 : 0;
-            if (fails >= patience) {
+            if (it == 1 || nrm < best) {
+                best = nrm;
+                best_it = it;
+                HIP_TRY(hipMemcpyAsync(h->xb.p, h->x.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+            } else if (it - best_it >= patience) {
                 cnn_on = false;
                 dsa_on = dsa != nullptr;
-            } else {
+                HIP_TRY(hipMemcpyAsync(h->x.p, h->xb.p, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+            }
+            if (cnn_on) {
                 int rc2 = allium_cnn_internal::correct(cnn, h->mesh.nel[2], h->mesh.nel[1], h->mesh.nel[0], n,
                                                        h->wavg.d(), h->r.d(), h->d_sigma, h->coef.d(), h->x.d(), s,
                                                        opt->accel_prolong);
                 if (rc2) return rc2;
             }
         }
-        prev_nrm = nrm;
         if (opt->history) {
             double *hrow = opt->history + (size_t)(it - 1) * 2 * Ne;
             HIP_TRY(allium_cnn_internal::elem_avg(Ne, n, h->wavg.d(), h->x.d(), h->havg.d(), s));

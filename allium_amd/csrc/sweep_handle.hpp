@@ -40,7 +40,7 @@ struct allium_sweep {
     // matrices kept for host-side rebuilds
     std::vector<double> Kst;
     // scratch for host-pointer entry points and the source iteration
-    DevBuf a, b, c, x, y, iso, coef, part, q, r, wavg, havg;
+    DevBuf a, b, c, x, y, iso, coef, part, q, r, wavg, havg, xb;
     hipStream_t stream = nullptr;
     ~allium_sweep()
     {

@@ -9,8 +9,8 @@
  * correction.  For callers that want SI with fewer sweeps on structured
  * one-group problems.
  *
- * Network: 5 convolution layers with 3x3x3 kernels, 3 -> 32 -> 32 -> 32 -> 32
- * -> 1 channels, ReLU after the first four, zero padding.  Weights and
+ * Network: 5 convolution layers with 3x3x3 kernels (optionally dilated), 3 ->
+ * 32 -> 32 -> 32 -> 32 -> 1 channels, ReLU after the first four, zero padding.  Weights and
  * activations are bfloat16 on the GPU with FP32 accumulation (MFMA); the
  * master parameters and the Adam moments are FP32.
  *
@@ -32,6 +32,11 @@ int allium_cnn_create(unsigned long long seed, allium_cnn_t *out);
 void allium_cnn_destroy(allium_cnn_t h);
 /* number of FP32 parameters in the packed layout */
 long allium_cnn_num_params(void);
+/* per-layer dilation of the 3x3x3 taps (offsets dilation * {-1, 0, 1} per
+ * axis, 1..8; default 1 for every layer): a wider receptive field with the
+ * same parameters, e.g. {1, 2, 4, 8, 1} sees 33 cells */
+int allium_cnn_set_dilation(allium_cnn_t h, const int *dilation);
+int allium_cnn_get_dilation(allium_cnn_t h, int *dilation);
 int allium_cnn_get_params(allium_cnn_t h, float *params);
 int allium_cnn_set_params(allium_cnn_t h, const float *params);
 /* inference on device buffers */

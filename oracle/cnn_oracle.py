@@ -14,7 +14,8 @@ Layout (same as allium_amd/csrc/cnn.hip):
   weights      W[l] [27, CO, CI] with offset o = (dz+1)*9 + (dy+1)*3 + (dx+1)
   biases       b[l] [CO]
   network      5 layers 3 -> 32 -> 32 -> 32 -> 32 -> 1, ReLU after the first four,
-               zero padding ("same" size)
+               zero padding ("same" size); layer l's taps optionally dilated
+               by dil[l] (offsets dil * (-1, 0, 1) per axis)
   loss         mean over voxels of (out - target)^2
 """
 import numpy as np
@@ -53,40 +54,42 @@ def unpack(flat, channels=CHANNELS):
 def shifted(x, dz, dy, dx):
     """y[..., z, y, x, :] = x[..., z+dz, y+dy, x+dx, :] with zeros outside."""
     B, Z, Y, X, C = x.shape
-    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (1, 1), (0, 0)))
-    return xp[:, 1 + dz:1 + dz + Z, 1 + dy:1 + dy + Y, 1 + dx:1 + dx + X, :]
+    p = max(abs(dz), abs(dy), abs(dx), 1)
+    xp = np.pad(x, ((0, 0), (p, p), (p, p), (p, p), (0, 0)))
+    return xp[:, p + dz:p + dz + Z, p + dy:p + dy + Y, p + dx:p + dx + X, :]
 
 
-def conv(x, W, b):
-    """out[v, co] = b[co] + sum_o sum_ci W[o, co, ci] x[v + off(o), ci]."""
+def conv(x, W, b, dil=1):
+    """out[v, co] = b[co] + sum_o sum_ci W[o, co, ci] x[v + dil * off(o), ci]."""
     out = np.zeros(x.shape[:4] + (W.shape[1],)) + b
     for o, (dz, dy, dx) in enumerate(OFFSETS):
-        out += np.einsum("bzyxc,oc->bzyxo", shifted(x, dz, dy, dx), W[o], optimize=True)
+        out += np.einsum("bzyxc,oc->bzyxo", shifted(x, dil * dz, dil * dy, dil * dx), W[o], optimize=True)
     return out
 
 
-def conv_input_grad(g, W):
-    """Adjoint of conv in its input: gx[v, ci] = sum_o sum_co W[o, co, ci] g[v - off(o), co]."""
+def conv_input_grad(g, W, dil=1):
+    """Adjoint of conv in its input: gx[v, ci] = sum_o sum_co W[o, co, ci] g[v - dil * off(o), co]."""
     out = np.zeros(g.shape[:4] + (W.shape[2],))
     for o, (dz, dy, dx) in enumerate(OFFSETS):
-        out += np.einsum("bzyxo,oc->bzyxc", shifted(g, -dz, -dy, -dx), W[o], optimize=True)
+        out += np.einsum("bzyxo,oc->bzyxc", shifted(g, -dil * dz, -dil * dy, -dil * dx), W[o], optimize=True)
     return out
 
 
-def conv_weight_grad(g, x):
-    """gW[o, co, ci] = sum_v g[v, co] x[v + off(o), ci]."""
-    return np.stack([np.einsum("bzyxo,bzyxc->oc", g, shifted(x, dz, dy, dx), optimize=True)
+def conv_weight_grad(g, x, dil=1):
+    """gW[o, co, ci] = sum_v g[v, co] x[v + dil * off(o), ci]."""
+    return np.stack([np.einsum("bzyxo,bzyxc->oc", g, shifted(x, dil * dz, dil * dy, dil * dx), optimize=True)
                      for (dz, dy, dx) in OFFSETS])
 
 
-def forward(params, x, bf16=False):
-    """x [B, Z, Y, X, 3] -> (out [B, Z, Y, X], cache)."""
+def forward(params, x, bf16=False, dil=None):
+    """x [B, Z, Y, X, 3] -> (out [B, Z, Y, X], cache); dil: per-layer dilation (default 1)."""
     r = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    dil = dil or [1] * len(params)
     a = r(x)
     acts, zs = [a], []
     L = len(params)
     for l, (W, b) in enumerate(params):
-        z = conv(a, r(W), b)
+        z = conv(a, r(W), b, dil[l])
         zs.append(z)
         if l < L - 1:
             a = r(np.maximum(z, 0.0))
@@ -94,10 +97,11 @@ def forward(params, x, bf16=False):
     return zs[-1][..., 0], {"acts": acts, "zs": zs}
 
 
-def loss_and_grad(params, x, target, bf16=False):
+def loss_and_grad(params, x, target, bf16=False, dil=None):
     """Mean squared error and its gradient with respect to every parameter."""
     r = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
-    out, cache = forward(params, x, bf16)
+    dil = dil or [1] * len(params)
+    out, cache = forward(params, x, bf16, dil)
     n = out.size
     diff = out - target
     loss = float(np.sum(diff * diff) / n)
@@ -106,9 +110,9 @@ def loss_and_grad(params, x, target, bf16=False):
     for l in range(len(params) - 1, -1, -1):
         W, b = params[l]
         a_in = cache["acts"][l]
-        grads[l] = (conv_weight_grad(g, a_in), g.sum(axis=(0, 1, 2, 3)))
+        grads[l] = (conv_weight_grad(g, a_in, dil[l]), g.sum(axis=(0, 1, 2, 3)))
         if l > 0:
-            gin = conv_input_grad(g, r(W))
+            gin = conv_input_grad(g, r(W), dil[l])
             g = r(gin * (cache["acts"][l] > 0))
     return loss, grads
 

@@ -62,6 +62,42 @@ def test_gradients_match_oracle():
         assert eW < 2e-2 and eb < 2e-2
 
 
+DILATIONS = [(1, 2, 4, 8, 1), (2, 1, 3, 1, 2)]
+
+
+@pytest.mark.parametrize("dil", DILATIONS)
+def test_dilated_forward_and_gradients_match_oracle(dil):
+    print(f"cnn dilation {dil}", flush=True)
+    rng = np.random.default_rng(15)
+    params = _random_params(16)
+    flat32 = C.pack(params).astype(np.float32)
+    p32 = C.unpack(flat32.astype(np.float64))
+    net = cnn.FluxCorrector(0, dilation=dil)
+    assert net.dilation() == dil
+    net.set_params(flat32)
+    shape = (2, 9, 10, 11)
+    x = rng.normal(size=shape + (3,)).astype(np.float32)
+    t = rng.normal(size=shape).astype(np.float32)
+    ref, _ = C.forward(p32, x.astype(np.float64), bf16=True, dil=list(dil))
+    assert _rel(net.forward(x), ref) < 5e-3
+    loss_ref, grads_ref = C.loss_and_grad(p32, x.astype(np.float64), t.astype(np.float64), bf16=True, dil=list(dil))
+    loss, g = net.gradient(x, t)
+    assert abs(loss - loss_ref) < 1e-2 * abs(loss_ref)
+    for l, ((gW, gb), (rW, rb)) in enumerate(zip(C.unpack(g.astype(np.float64)), grads_ref)):
+        eW, eb = _rel(gW, rW), _rel(gb, rb)
+        print(f"  layer {l}: weights {eW:.2e} bias {eb:.2e}")
+        assert eW < 2e-2 and eb < 2e-2
+
+
+def test_dilation_errors():
+    net = cnn.FluxCorrector(0)
+    assert net.dilation() == (1, 1, 1, 1, 1)
+    with pytest.raises(A.AlliumError):
+        net.set_dilation((1, 2, 9, 1, 1))
+    with pytest.raises(A.AlliumError):
+        net.set_dilation((0, 1, 1, 1, 1))
+
+
 def test_training_reduces_loss():
     """Adam on the GPU fits a smoothed linear map of the first input channel."""
     print("cnn training", flush=True)
