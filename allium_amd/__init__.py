@@ -429,7 +429,8 @@ class DSA:
 
 
 def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000, precision=FP64,
-                     want_psi=True, accel=None, history=None, prolong=1, patience=3, fallback=None):
+                     want_psi=True, accel=None, history=None, prolong=1, patience=3, fallback=None,
+                     raw_norms=None):
     """chive's SI: returns (phi, psi, iterations, norms) (exe/chive.cpp:700-735).
 
     ``accel``: a DSA object (the SLISolver preconditioner: every iteration is
@@ -442,7 +443,9 @@ def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000,
     keeps them element-constant.  ``history``: a (maxit, 2, Ne) float64 array
     that receives, for every iteration but the last, the element-averaged
     scalar flux after it and the element-averaged change it made before any
-    correction."""
+    correction.  ``raw_norms``: a float64 array of maxit that receives the
+    unpreconditioned change of every iteration (before any DSA or learned
+    correction)."""
     lib = Linv._lib
     hist_p = None
     if history is not None:
@@ -450,9 +453,14 @@ def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000,
             raise ValueError("history must be a contiguous float64 array of maxit x 2 x Ne")
         hist_p = _d(history)
     kind = 0 if accel is None else (2 if isinstance(accel, DSA) else 1)
+    raw_p = None
+    if raw_norms is not None:
+        if raw_norms.dtype != np.float64 or not raw_norms.flags.c_contiguous or raw_norms.size < maxit:
+            raise ValueError("raw_norms must be a contiguous float64 array of maxit")
+        raw_p = _d(raw_norms)
     opt = _lib.SIOptions(abstol, reltol, int(maxit), int(precision), kind,
                          None if accel is None else accel.handle, None if fallback is None else fallback.handle,
-                         int(prolong), int(patience), hist_p)
+                         int(prolong), int(patience), hist_p, raw_p)
     phi = np.empty(Linv.phi_size)
     psi = np.empty(Linv.size) if want_psi else None
     norms = np.zeros(maxit + 1)

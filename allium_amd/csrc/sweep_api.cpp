@@ -929,12 +929,15 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
     // ||phi_{k+1} - phi_k|| from the update's partial sums, or with DSA on the
     // preconditioned change z = r + K^{-1} Ms r (SLISolver with a preconditioner)
     auto residual_norm = [&](double &nrm) -> int {
+        if (!dsa_on || opt->raw_norms) {
+            HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            nrm = 0.0;
+            for (int i = 0; i < nb; i++) nrm += hp[i];
+            nrm = std::sqrt(nrm);
+            if (opt->raw_norms) opt->raw_norms[it - 1] = nrm;
+        }
         if (dsa_on) return allium_dsa_internal::correct(dsa, h->r.d(), h->x.d(), s, &nrm);
-        HIP_TRY(hipMemcpyAsync(hp.data(), h->part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        nrm = 0.0;
-        for (int i = 0; i < nb; i++) nrm += hp[i];
-        nrm = std::sqrt(nrm);
         return ALLIUM_OK;
     };
     if (opt->precision == ALLIUM_FP32_REFINED) {

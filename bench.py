@@ -85,7 +85,7 @@ def measure_traffic(args, tuning):
         d = tempfile.mkdtemp(prefix="allium_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run",
                "--", sys.executable, os.path.abspath(__file__), "--n", str(args.n), "--seed", str(args.seed),
-               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-trt", "--no-config3", "--no-traffic", "--tuning", tun,
+               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-trt", "--no-config3", "--no-config1", "--no-traffic", "--tuning", tun,
                "--copy-mib", str(copy_mib), "--precision", args.precision, *box]
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=os.environ.get("TMPDIR", "/tmp"))
         path = os.path.join(d, "run_counter_collection.csv")
@@ -136,12 +136,15 @@ def si_run(n, seed, maxit, learned=True, tuning=None):
     q = Linv.form_source(Q=1.0)
 
     def solve(**kw):
+        raw = np.zeros(maxit)
         A.synchronize()
         t = time.perf_counter()
-        phi, _, it, norms = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=maxit, want_psi=False, **kw)
+        phi, _, it, norms = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=maxit, want_psi=False,
+                                               raw_norms=raw, **kw)
         wall = time.perf_counter() - t
         return phi, {"iterations": it, "final_residual": float(norms[-1]), "wall_s": wall,
-                     "converged": bool(norms[-1] < 1e-8)}
+                     "converged": bool(norms[-1] < 1e-8),
+                     "final_unpreconditioned_residual": float(raw[it - 1])}
 
     phi0, plain = solve()
     out = {"plain": plain}
@@ -168,6 +171,39 @@ def si_run(n, seed, maxit, learned=True, tuning=None):
     for name, prec in (("learned_fp64", A.FP64), ("learned_fp32_refined", A.FP32_REFINED)):
         compare(name, accel=net, precision=prec)
     compare("learned_fp32_refined_dsa_fallback", accel=net, precision=A.FP32_REFINED, fallback=dsa)
+    return out
+
+
+def config1_run(threads):
+    """chive-style 2D steady one-group SI on the unit square, 32x32 quads, p=1,
+    S8 (40 dirs), sigma_t=1, c=0.9, Q=1, vacuum, to a 1e-8 residual: the GPU
+    (plain and with MIP DSA) and the CPU oracle (OpenMP) on the same input."""
+    import oracle
+    import allium_amd as A
+    from allium_amd import problems as P
+
+    cfg = P.problem_2d_cfg1()
+    om, w = A.level_symmetric(8, 2)
+    Linv = A.InverseAdvectionOperator(A.Mesh(cfg["nel"], vertices=cfg["vertices"]), 1, (om, w), cfg["sigma_t"])
+    Linv.set_launch(-1)
+    q = Linv.form_source(Q=cfg["Q"])
+    out = {}
+    for name, kw in (("gpu_plain", {}), ("gpu_dsa", {"accel": A.DSA(Linv, cfg["sigma_s"])})):
+        A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=1000, want_psi=False, **kw)   # warm-up
+        A.synchronize()
+        t = time.perf_counter()
+        phi, _, it, norms = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=1000, want_psi=False, **kw)
+        out[name] = {"iterations": it, "wall_s": time.perf_counter() - t, "final_residual": float(norms[-1])}
+        if name == "gpu_plain":
+            phi_gpu = phi
+    pr = oracle.Problem(2, cfg["nel"], cfg["vertices"], 1, om, w, cfg["sigma_t"])
+    qo = pr.form_source(Q=cfg["Q"])
+    t = time.perf_counter()
+    phi_o, _, it_o, norms_o = pr.source_iteration(cfg["sigma_s"], qo, abstol=1e-8, maxit=1000, threads=threads)
+    out["cpu_oracle"] = {"iterations": it_o, "wall_s": time.perf_counter() - t, "threads": threads,
+                         "final_residual": float(norms_o[-1])}
+    out["rel_diff_gpu_vs_cpu"] = float(np.linalg.norm(phi_gpu - phi_o) / np.linalg.norm(phi_o))
+    out["unknowns"] = int(Linv.size)
     return out
 
 
@@ -251,7 +287,8 @@ def main():
                     help="-1 autotune, 0 levels, 1 graph, 3 direction sweeps, 4 box sweeps, 5 box sweeps in a graph")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-config1", action="store_true", help="skip the 2D chive-style SI (GPU and CPU oracle)")
     ap.add_argument("--no-si", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="plain source iteration only (no network training)")
     ap.add_argument("--no-trt", action="store_true", help="skip the 20-step multigroup TRT run")
@@ -367,6 +404,12 @@ def main():
             log("TRT:", {k: v for k, v in out["trt_config4"].items() if k != "transport_iterations"})
         except Exception as exc:  # reported, not hidden
             out["trt_config4"] = {"error": str(exc)}
+    if not args.no_config1 and not args.no_cpu:
+        try:
+            out["chive_config1"] = config1_run(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+            log("config 1:", out["chive_config1"])
+        except Exception as exc:  # reported, not hidden
+            out["chive_config1"] = {"error": str(exc)}
     if not args.no_cpu:
         cb = cpu_baseline(args.n, args.seed, args.cpu_reps)
         log(f"cpu baseline {cb['value']:.3e} unknowns/s on {cb['cores']} threads")

@@ -129,10 +129,14 @@ def test_dsa_si_thick_medium(precision):
     ss = st * 0.9999
     Linv = A.InverseAdvectionOperator(A.Mesh(nel), 1, 4, st)
     q = Linv.form_source(Q=1.0)
-    phi0, _, it0, _ = A.source_iteration(Linv, ss, q, abstol=1e-10, maxit=5000, want_psi=False)
+    raw0, raw1 = np.zeros(5000), np.zeros(5000)
+    phi0, _, it0, n0 = A.source_iteration(Linv, ss, q, abstol=1e-10, maxit=5000, want_psi=False, raw_norms=raw0)
+    np.testing.assert_allclose(raw0[:it0], n0, rtol=1e-14)   # without acceleration the two norms agree
     dsa = A.DSA(Linv, ss)
     phi1, _, it1, n1 = A.source_iteration(Linv, ss, q, abstol=1e-10, maxit=5000, want_psi=False,
-                                          accel=dsa, precision=precision)
+                                          accel=dsa, precision=precision, raw_norms=raw1)
+    # the unpreconditioned change is reported beside ||z|| and also falls
+    assert np.all(np.isfinite(raw1[:it1])) and raw1[it1 - 1] < 1e-3 * raw1[0]
     info = dsa.info()
     print(f"  plain {it0} its, DSA {it1} its; CG its per DSA {info['cg_iterations'] / max(1, info['applications']):.1f}")
     assert n1[-1] < 1e-10
