@@ -430,7 +430,7 @@ class DSA:
 
 def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000, precision=FP64,
                      want_psi=True, accel=None, history=None, prolong=1, patience=3, fallback=None,
-                     raw_norms=None):
+                     raw_norms=None, inner_iterations=None):
     """chive's SI: returns (phi, psi, iterations, norms) (exe/chive.cpp:700-735).
 
     ``accel``: a DSA object (the SLISolver preconditioner: every iteration is
@@ -445,7 +445,8 @@ def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000,
     scalar flux after it and the element-averaged change it made before any
     correction.  ``raw_norms``: a float64 array of maxit that receives the
     unpreconditioned change of every iteration (before any DSA or learned
-    correction)."""
+    correction); ``inner_iterations``: an int32 array of maxit that receives
+    the DSA CG iterations of every iteration."""
     lib = Linv._lib
     hist_p = None
     if history is not None:
@@ -458,9 +459,15 @@ def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000,
         if raw_norms.dtype != np.float64 or not raw_norms.flags.c_contiguous or raw_norms.size < maxit:
             raise ValueError("raw_norms must be a contiguous float64 array of maxit")
         raw_p = _d(raw_norms)
+    inner_p = None
+    if inner_iterations is not None:
+        if inner_iterations.dtype != np.int32 or not inner_iterations.flags.c_contiguous or \
+                inner_iterations.size < maxit:
+            raise ValueError("inner_iterations must be a contiguous int32 array of maxit")
+        inner_p = inner_iterations.ctypes.data_as(_lib.c_ip)
     opt = _lib.SIOptions(abstol, reltol, int(maxit), int(precision), kind,
                          None if accel is None else accel.handle, None if fallback is None else fallback.handle,
-                         int(prolong), int(patience), hist_p, raw_p)
+                         int(prolong), int(patience), hist_p, raw_p, inner_p)
     phi = np.empty(Linv.phi_size)
     psi = np.empty(Linv.size) if want_psi else None
     norms = np.zeros(maxit + 1)

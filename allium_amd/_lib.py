@@ -35,7 +35,7 @@ class SIOptions(ctypes.Structure):
     _fields_ = [("abstol", ctypes.c_double), ("reltol", ctypes.c_double), ("maxit", ctypes.c_int),
                 ("precision", ctypes.c_int), ("accel", ctypes.c_int), ("accel_handle", c_vp),
                 ("accel_fallback", c_vp), ("accel_prolong", ctypes.c_int), ("accel_patience", ctypes.c_int),
-                ("history", c_dp), ("raw_norms", c_dp)]
+                ("history", c_dp), ("raw_norms", c_dp), ("inner_iterations", c_ip)]
 
 
 class DSADesc(ctypes.Structure):

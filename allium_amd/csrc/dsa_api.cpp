@@ -450,6 +450,8 @@ int correct(allium_dsa *d, const double *d_r, double *d_x, hipStream_t s, double
 
 const allium_sweep *handle_of(const allium_dsa *d) { return d->h; }
 
+int last_inner(const allium_dsa *d) { return d->info.last_cg_iterations; }
+
 }  // namespace allium_dsa_internal
 
 extern "C" {

@@ -14,5 +14,7 @@ namespace allium_dsa_internal {
 int correct(allium_dsa *d, const double *d_r, double *d_x, hipStream_t s, double *nrm);
 // the sweep handle the DSA object was built on
 const allium_sweep *handle_of(const allium_dsa *d);
+// CG iterations of the last solve
+int last_inner(const allium_dsa *d);
 
 }  // namespace allium_dsa_internal

@@ -937,7 +937,12 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
             nrm = std::sqrt(nrm);
             if (opt->raw_norms) opt->raw_norms[it - 1] = nrm;
         }
-        if (dsa_on) return allium_dsa_internal::correct(dsa, h->r.d(), h->x.d(), s, &nrm);
+        if (opt->inner_iterations) opt->inner_iterations[it - 1] = 0;
+        if (dsa_on) {
+            int rc2 = allium_dsa_internal::correct(dsa, h->r.d(), h->x.d(), s, &nrm);
+            if (rc2) return rc2;
+            if (opt->inner_iterations) opt->inner_iterations[it - 1] = allium_dsa_internal::last_inner(dsa);
+        }
         return ALLIUM_OK;
     };
     if (opt->precision == ALLIUM_FP32_REFINED) {

@@ -1,0 +1,145 @@
+"""chive-style steady one-group source iteration on the GPU.
+
+Reads a deck with the structure of chive's Lua inputs (exe/chive.cpp; e.g.
+test/chive/tdl.lua) written as YAML or a dict, runs the SI path of
+exe/chive.cpp:550-750 (FormTransportSource, InverseAdvectionOperator,
+SLISolver on TransportOperator, optional "scalar mip" DSA preconditioner,
+final sweep) and reports with chive's YAML keys ("transport iterations" with
+it / norm / inner solver, "outer iterations", "inner iteration" max / min /
+avg / total; exe/chive.cpp:40-75,738-746), so scripts that parse chive's
+output read this one too.
+
+Deck keys (YAML):
+
+    mesh: {num_elements: [8, 8], extents: [1, 1]}
+    materials: {mat: {total: 1.0e4, scattering: 9999.9999, source: 1.0e-4}}
+    material_map: mat                 # one name, or a name per element (x fastest)
+    boundary_conditions: {vacuum: {type: inflow, value: 0}}
+    boundary_map: vacuum              # one name, or 2*dim names (side = axis*2 + {low, high})
+    sn: {type: level symmetric, order: 4}
+    driver:
+      fe_order: 1
+      solver: {type: sli, abstol: 1.0e-6, reltol: 0, max_iter: 50}
+      preconditioner: {type: scalar mip, kappa: 4, solver: {type: cg, reltol: 1.0e-10, max_iter: 200}}
+
+Material `source` and boundary `value` are per-steradian isotropic densities,
+as in chive (exe/chive.cpp:236-247,294-306).  Boundary type "reflective"
+maps to a reflective side.
+
+Usage: ``python -m allium_amd.driver deck.yaml`` prints the output YAML.
+"""
+import argparse
+import sys
+import time
+
+import numpy as np
+import yaml
+
+import allium_amd as A
+
+
+def _per_element(value, ne, what):
+    if isinstance(value, str):
+        return [value] * ne
+    if len(value) != ne:
+        raise ValueError(f"{what} needs one entry per element ({ne})")
+    return list(value)
+
+
+def run(deck):
+    """Solve the deck; returns the output mapping (chive's keys)."""
+    mesh_d = deck["mesh"]
+    nel = tuple(int(v) for v in mesh_d["num_elements"])
+    dim = len(nel)
+    ext = tuple(float(v) for v in mesh_d.get("extents", [1.0] * dim))
+    ne = int(np.prod(nel))
+    mats = deck["materials"]
+    names = _per_element(deck.get("material_map", next(iter(mats))), ne, "material_map")
+    sigma_t = np.array([float(mats[m]["total"]) for m in names])
+    sigma_s = np.array([float(mats[m].get("scattering", 0.0)) for m in names])
+    Q = np.array([float(mats[m].get("source", 0.0)) for m in names])
+    bcs = deck.get("boundary_conditions", {"vacuum": {"type": "inflow", "value": 0.0}})
+    bmap = deck.get("boundary_map", next(iter(bcs)))
+    side_names = [bmap] * (2 * dim) if isinstance(bmap, str) else list(bmap)
+    if len(side_names) != 2 * dim:
+        raise ValueError(f"boundary_map needs one name or {2 * dim} names")
+    bc, inflow = [], []
+    for nm in side_names:
+        spec = bcs[nm]
+        typ = str(spec.get("type", "inflow")).lower()
+        if typ in ("reflective", "reflect"):
+            bc.append(A.BC_REFLECTIVE)
+            inflow.append(0.0)
+        elif typ in ("inflow", "vacuum"):
+            bc.append(A.BC_INFLOW)
+            inflow.append(float(spec.get("value", 0.0)))
+        else:
+            raise ValueError(f"boundary condition type {typ} is not supported")
+    sn = deck.get("sn", {"type": "level symmetric", "order": 4})
+    if "level" not in str(sn.get("type", "level symmetric")).lower():
+        raise ValueError("only level-symmetric quadrature is supported")
+    drv = deck.get("driver", {})
+    p = int(drv.get("fe_order", 1))
+    sol = drv.get("solver", {"type": "sli"})
+    if str(sol.get("type", "sli")).lower() != "sli":
+        raise ValueError("only the sli solver (source iteration) is supported")
+    abstol, reltol = float(sol.get("abstol", 0.0)), float(sol.get("reltol", 0.0))
+    maxit = int(sol.get("max_iter", 50))
+    prec = drv.get("preconditioner")
+    if prec and str(prec.get("type", "")).lower() not in ("scalar mip", "mip"):
+        raise ValueError(f"preconditioner {prec.get('type')} is not supported (scalar mip only)")
+
+    out = {}
+    t0 = time.perf_counter()
+    mesh = A.Mesh(nel, extents=ext, bc=bc)
+    Linv = A.InverseAdvectionOperator(mesh, p, int(sn.get("order", 4)), sigma_t.reshape(1, -1))
+    Linv.set_launch(-1)
+    q = Linv.form_source(Q=Q.reshape(1, -1), inflow=np.array(inflow).reshape(1, -1))
+    accel = None
+    out["preconditioner"] = "none"
+    if prec:
+        inner = prec.get("solver", {})
+        kappa = float(prec.get("kappa", (p + 1) ** 2))
+        accel = A.DSA(Linv, sigma_s, kappa=kappa, bound_below=bool(prec.get("bound_stabilization_below", True)),
+                      sym=float(prec.get("sigma", -1.0)), cg_reltol=float(inner.get("reltol", 1e-10)),
+                      cg_abstol=float(inner.get("abstol", 0.0)), cg_maxit=int(inner.get("max_iter", 200)))
+        out["preconditioner"] = {"type": "scalar mip", "kappa": kappa,
+                                 "solver": {"type": "cg", "reltol": float(inner.get("reltol", 1e-10)),
+                                            "abstol": float(inner.get("abstol", 0.0)),
+                                            "max_iter": int(inner.get("max_iter", 200))}}
+    out["setup time"] = f"{time.perf_counter() - t0:.3e} s"
+    inner_its = np.zeros(maxit, dtype=np.int32)
+    t1 = time.perf_counter()
+    phi, psi, it, norms = A.source_iteration(Linv, sigma_s, q, abstol=abstol, reltol=reltol, maxit=maxit,
+                                             accel=accel, inner_iterations=inner_its)
+    solve = time.perf_counter() - t1
+    rows = []
+    for k in range(it):
+        row = {"it": k + 1, "norm": f"{norms[k]:.3e}"}
+        if accel is not None:
+            row["inner solver"] = {"it": int(inner_its[k])}
+        rows.append(row)
+    out["transport iterations"] = rows
+    out["outer iterations"] = int(it)
+    if accel is not None:
+        ii = inner_its[:it]
+        out["inner iteration"] = {"max": int(ii.max()), "min": int(ii.min()), "avg": float(ii.mean()),
+                                  "total": int(ii.sum())}
+    out["converged"] = bool(norms[-1] < max(reltol * norms[0], abstol))
+    out["timings"] = {"solve": f"{solve:.3e} s"}
+    out["scalar flux"] = {"l2": float(np.linalg.norm(phi)), "min": float(phi.min()), "max": float(phi.max())}
+    out["balance"] = float(A.AdvectionOperator(Linv).ComputeBalance(psi))
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="chive-style source iteration on the GPU")
+    ap.add_argument("deck", help="YAML deck")
+    a = ap.parse_args(argv)
+    with open(a.deck) as fh:
+        deck = yaml.safe_load(fh)
+    sys.stdout.write(yaml.safe_dump(run(deck), sort_keys=False))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,52 @@
+"""The chive-style driver (allium_amd/driver.py): deck validation on the CPU;
+the reference's thick-diffusion-limit acceptance check on the GPU."""
+import copy
+import os
+
+import numpy as np
+import pytest
+import yaml
+
+from allium_amd import driver
+
+DECK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "decks", "tdl.yaml")
+
+
+def _deck():
+    with open(DECK) as fh:
+        return yaml.safe_load(fh)
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (lambda d: d["driver"]["solver"].update(type="gmres"), "sli"),
+    (lambda d: d["sn"].update(type="abu shumays"), "level-symmetric"),
+    (lambda d: d.update(material_map=["mat"] * 3), "one entry per element"),
+    (lambda d: d.update(boundary_map=["vacuum"] * 3), "boundary_map"),
+    (lambda d: d["boundary_conditions"]["vacuum"].update(type="periodic"), "not supported"),
+    (lambda d: d["driver"].update(preconditioner={"type": "p1sa"}), "scalar mip only"),
+])
+def test_deck_errors(edit, msg):
+    d = _deck()
+    edit(d)
+    with pytest.raises(ValueError, match=msg):
+        driver.run(d)
+
+
+@pytest.mark.gpu
+def test_tdl_deck_outer_iterations():
+    """test/chive/test_tdl.py asserts `outer iterations < 12` for the TDL deck
+    with a DSA preconditioner; plain SI runs to max_iter there."""
+    print("driver tdl", flush=True)
+    d = _deck()
+    plain = driver.run(copy.deepcopy(d))
+    assert plain["outer iterations"] == 50 and not plain["converged"]
+    d["driver"]["preconditioner"] = {"type": "scalar mip", "solver": {"type": "cg", "reltol": 1e-12,
+                                                                      "max_iter": 400}}
+    out = driver.run(d)
+    print(yaml.safe_dump({k: out[k] for k in ("outer iterations", "inner iteration")}))
+    assert out["converged"] and out["outer iterations"] < 12
+    rows = out["transport iterations"]
+    assert len(rows) == out["outer iterations"] and rows[0]["it"] == 1
+    assert float(rows[-1]["norm"]) < 1e-6 and rows[-1]["inner solver"]["it"] > 0
+    assert out["inner iteration"]["total"] == sum(r["inner solver"]["it"] for r in rows)
+    assert np.isfinite(out["balance"])
