@@ -6,7 +6,8 @@
 // and of the right-hand side (source, isotropic source, upwind traces), then
 // the cell's lanes run an LU without pivoting (A + A^T is positive definite)
 // in which the owner of pivot row k publishes it through LDS, and a
-// column-oriented back substitution.  A 256-thread workgroup holds 8 cells.
+// column-oriented back substitution.  A 256-thread workgroup holds 8 cells,
+// two per wave; the lanes of a cell synchronise within their wave only.
 // The launch covers one level of up to kMaxBatch independent instances, so
 // a batch of uneven meshes sweeps in max(levels) launches instead of
 // sum(levels).
@@ -16,6 +17,17 @@
 namespace allium {
 
 bool coop_supported(int dim, int p) { return dim == 3 && p == 2; }
+
+// The 32 lanes of a cell are half of one wave and its pivot row is LDS that
+// no other cell touches, so the exchange needs no workgroup barrier: LDS
+// executes a wave's instructions in order, and the wavefront-scope fences keep
+// the compiler from moving the reads above the owner lane's write.
+__device__ __forceinline__ void cell_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int DIM, int P, bool UNI, typename T>
 __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepParams *__restrict__ prms,
@@ -127,14 +139,14 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
             for (int j = k; j < N; ++j) piv[slot][j] = A[j];
             piv[slot][N] = bb;
         }
-        __syncthreads();
+        cell_sync();
         if (r > k && r < N) {
             const T l = A[k] / piv[slot][k];
 #pragma unroll
             for (int j = k + 1; j < N; ++j) A[j] = fma(-l, piv[slot][j], A[j]);
             bb = fma(-l, piv[slot][N], bb);
         }
-        __syncthreads();
+        cell_sync();
     }
     // ---- back substitution, one unknown per step ----------------------------
     T x = (T)0;
@@ -144,10 +156,10 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
             x = bb / A[k];
             piv[slot][0] = x;
         }
-        __syncthreads();
+        cell_sync();
         const T xk = piv[slot][0];
         if (r < k) bb = fma(-A[k], xk, bb);
-        __syncthreads();
+        cell_sync();
     }
     if (row) p.psi[off + rr] = (double)x;
 }
