@@ -6,6 +6,7 @@ Python mirror of the reference's operator interface for the sweep path:
 * ``AdvectionOperator``         (src/sweep.hpp:158-167)  ``Mult``, ``ComputeBalance``
 * ``DiscreteToMoment``          (src/transport_op.hpp:99-123)
 * ``source_iteration``          (SLISolver on TransportOperator, src/linalg.cpp:103-153)
+* ``krylov_solve``              (chive's "gmres" / "fgmres" outer solvers, src/io.cpp:179-230)
 
 Every call goes through the C ABI of ``lib/liballium_amd.so`` (HIP kernels for
 gfx950).  Host numpy arrays and ``DeviceVector`` (HBM) are both accepted.
@@ -446,7 +447,9 @@ def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000,
     correction.  ``raw_norms``: a float64 array of maxit that receives the
     unpreconditioned change of every iteration (before any DSA or learned
     correction); ``inner_iterations``: an int32 array of maxit that receives
-    the DSA CG iterations of every iteration."""
+    the DSA CG iterations of every iteration.  A DeviceVector ``source``
+    (HBM-resident) runs the device entry point: phi comes back to the host,
+    psi stays a DeviceVector."""
     lib = Linv._lib
     hist_p = None
     if history is not None:
@@ -468,13 +471,86 @@ def source_iteration(Linv, sigma_s, source, abstol=1e-8, reltol=0.0, maxit=1000,
     opt = _lib.SIOptions(abstol, reltol, int(maxit), int(precision), kind,
                          None if accel is None else accel.handle, None if fallback is None else fallback.handle,
                          int(prolong), int(patience), hist_p, raw_p, inner_p)
-    phi = np.empty(Linv.phi_size)
-    psi = np.empty(Linv.size) if want_psi else None
     norms = np.zeros(maxit + 1)
     it = ctypes.c_int(0)
-    rc = lib.allium_source_iteration(Linv._h, _d(_f64(np.broadcast_to(sigma_s, (Linv.mesh.Ne,)))), _d(_f64(source)),
-                                     ctypes.byref(opt), _d(phi), None if psi is None else _d(psi), _d(norms),
-                                     ctypes.byref(it))
-    if rc not in (0, -7):
-        check(rc)
+    sig = _d(_f64(np.broadcast_to(sigma_s, (Linv.mesh.Ne,))))
+    if isinstance(source, DeviceVector):
+        dphi = DeviceVector(Linv.phi_size)
+        psi = DeviceVector(Linv.size) if want_psi else None
+        rc = lib.allium_source_iteration_device(Linv._h, sig, source.ptr, ctypes.byref(opt), dphi.ptr,
+                                                None if psi is None else psi.ptr, _d(norms), ctypes.byref(it))
+        if rc not in (0, -7):
+            check(rc)
+        phi = dphi.to_host()
+    else:
+        phi = np.empty(Linv.phi_size)
+        psi = np.empty(Linv.size) if want_psi else None
+        rc = lib.allium_source_iteration(Linv._h, sig, _d(_f64(source)), ctypes.byref(opt), _d(phi),
+                                         None if psi is None else _d(psi), _d(norms), ctypes.byref(it))
+        if rc not in (0, -7):
+            check(rc)
     return phi, psi, it.value, norms[: it.value].copy()
+
+
+SOLVERS = {"gmres": 1, "fgmres": 2}
+
+
+def krylov_solve(Linv, sigma_s, source, solver="fgmres", kdim=50, abstol=1e-8, reltol=0.0, maxit=100,
+                 want_psi=True, accel=None, fallback=None, prolong=1, patience=3, max_steps=0,
+                 inner_iterations=None):
+    """chive's "gmres" / "fgmres" outer solver on T = I - D L^{-1} D^T Ms
+    (src/io.cpp:179-230, exe/chive.cpp:700-735): returns (phi, psi,
+    iterations, norms) with norms[0] = beta0 and norms[j] the residual
+    estimate after step j (mfem's monitor values).
+
+    ``accel``: a DSA object (M = I + K^{-1} Ms; left preconditioner of GMRES,
+    right of FGMRES) or, with FGMRES only, a cnn.FluxCorrector (M v = v +
+    net(v)), replaced by ``fallback`` (a DSA object or None) once ``patience``
+    consecutive steps each cut the residual by less than 10 %, or after
+    ``max_steps`` steps (0: no limit); the number of steps that used the
+    network is left in ``krylov_solve.last_accel_steps``.  A DeviceVector
+    ``source`` (HBM-resident) runs the device entry point: phi comes back to
+    the host, psi stays a DeviceVector.
+    ``inner_iterations`` (int32, maxit) receives the DSA CG iterations of each
+    step.  FP64 sweeps; zero initial guess."""
+    lib = Linv._lib
+    if solver not in SOLVERS:
+        raise ValueError(f"solver must be one of {sorted(SOLVERS)}")
+    kind = 0 if accel is None else (2 if isinstance(accel, DSA) else 1)
+    inner_p = None
+    if inner_iterations is not None:
+        if inner_iterations.dtype != np.int32 or not inner_iterations.flags.c_contiguous or \
+                inner_iterations.size < maxit:
+            raise ValueError("inner_iterations must be a contiguous int32 array of maxit")
+        inner_p = inner_iterations.ctypes.data_as(_lib.c_ip)
+    steps = ctypes.c_int(0)
+    opt = _lib.KrylovOptions()
+    check(lib.allium_krylov_default_options(ctypes.byref(opt)))
+    opt.solver, opt.kdim, opt.abstol, opt.reltol, opt.maxit = SOLVERS[solver], int(kdim), abstol, reltol, int(maxit)
+    opt.accel = kind
+    opt.accel_handle = None if accel is None else accel.handle
+    opt.accel_fallback = None if fallback is None else fallback.handle
+    opt.accel_prolong, opt.accel_patience = int(prolong), int(patience)
+    opt.inner_iterations = inner_p
+    opt.accel_steps = ctypes.cast(ctypes.byref(steps), _lib.c_ip)
+    opt.accel_max_steps = int(max_steps)
+    norms = np.zeros(maxit + 2)
+    it = ctypes.c_int(0)
+    sig = _d(_f64(np.broadcast_to(sigma_s, (Linv.mesh.Ne,))))
+    if isinstance(source, DeviceVector):
+        dphi = DeviceVector(Linv.phi_size)
+        psi = DeviceVector(Linv.size) if want_psi else None
+        rc = lib.allium_krylov_solve_device(Linv._h, sig, source.ptr, ctypes.byref(opt), dphi.ptr,
+                                            None if psi is None else psi.ptr, _d(norms), ctypes.byref(it))
+        if rc not in (0, -7):
+            check(rc)
+        phi = dphi.to_host()
+    else:
+        phi = np.empty(Linv.phi_size)
+        psi = np.empty(Linv.size) if want_psi else None
+        rc = lib.allium_krylov_solve(Linv._h, sig, _d(_f64(source)), ctypes.byref(opt), _d(phi),
+                                     None if psi is None else _d(psi), _d(norms), ctypes.byref(it))
+        if rc not in (0, -7):
+            check(rc)
+    krylov_solve.last_accel_steps = steps.value
+    return phi, psi, it.value, norms[: it.value + 1].copy()

@@ -38,6 +38,14 @@ class SIOptions(ctypes.Structure):
                 ("history", c_dp), ("raw_norms", c_dp), ("inner_iterations", c_ip)]
 
 
+class KrylovOptions(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int), ("kdim", ctypes.c_int), ("abstol", ctypes.c_double),
+                ("reltol", ctypes.c_double), ("maxit", ctypes.c_int), ("accel", ctypes.c_int),
+                ("accel_handle", c_vp), ("accel_fallback", c_vp), ("accel_prolong", ctypes.c_int),
+                ("accel_patience", ctypes.c_int), ("inner_iterations", c_ip), ("accel_steps", c_ip),
+                ("accel_max_steps", ctypes.c_int)]
+
+
 class DSADesc(ctypes.Structure):
     _fields_ = [("kappa", ctypes.c_double), ("bound_below", ctypes.c_int), ("alpha", ctypes.c_double),
                 ("sym", ctypes.c_double), ("cg_reltol", ctypes.c_double), ("cg_abstol", ctypes.c_double),
@@ -82,6 +90,13 @@ SIGNATURES = {
     "allium_mass_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "allium_source_iteration": (ctypes.c_int, [c_vp, c_dp, c_dp, ctypes.POINTER(SIOptions), c_dp, c_dp, c_dp,
                                                c_ip]),
+    "allium_krylov_default_options": (ctypes.c_int, [ctypes.POINTER(KrylovOptions)]),
+    "allium_krylov_solve": (ctypes.c_int, [c_vp, c_dp, c_dp, ctypes.POINTER(KrylovOptions), c_dp, c_dp, c_dp,
+                                           c_ip]),
+    "allium_krylov_solve_device": (ctypes.c_int, [c_vp, c_dp, c_vp, ctypes.POINTER(KrylovOptions), c_vp, c_vp,
+                                                  c_dp, c_ip]),
+    "allium_source_iteration_device": (ctypes.c_int, [c_vp, c_dp, c_vp, ctypes.POINTER(SIOptions), c_vp, c_vp,
+                                                      c_dp, c_ip]),
     "allium_sweep_timing": (ctypes.c_int, [c_vp, c_dp, c_lp]),
     "allium_sweep_batch_create": (ctypes.c_int, [ctypes.POINTER(c_vp), ctypes.c_int, ctypes.POINTER(c_vp)]),
     "allium_sweep_batch_destroy": (None, [c_vp]),

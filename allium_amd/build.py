@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "liballium_amd.so")
-SOURCES = ["csrc/discretization.cpp", "csrc/sweep_api.cpp", "csrc/sweep_kernels.hip", "csrc/box_sweep.hip", "csrc/cnn.hip", "csrc/trt.hip", "csrc/coop_sweep.hip", "csrc/dsa.hip", "csrc/dsa_api.cpp"]
+SOURCES = ["csrc/discretization.cpp", "csrc/sweep_api.cpp", "csrc/sweep_kernels.hip", "csrc/box_sweep.hip", "csrc/cnn.hip", "csrc/trt.hip", "csrc/coop_sweep.hip", "csrc/dsa.hip", "csrc/dsa_api.cpp", "csrc/krylov.hip", "csrc/krylov_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

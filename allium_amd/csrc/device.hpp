@@ -135,8 +135,15 @@ struct DeviceSweep {
     void *graph_psi = nullptr;
     int graph_prec = -1;
     hipStream_t graph_stream = nullptr;
-    // timing (TimingLog "sweep", EventLog "sweeps")
+    // timing (TimingLog "sweep", EventLog "sweeps"): launch_sweep records ev0 /
+    // ev1 around each sweep.  The caller points them at a ring of event pairs
+    // and reads a pair only when it comes round again or the total is asked
+    // for, so timing never makes the host wait for a sweep.
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    static constexpr int kTimingRing = 64;
+    hipEvent_t ring0[kTimingRing] = {}, ring1[kTimingRing] = {};
+    bool ring_busy[kTimingRing] = {};
+    int ring_next = 0;
     double sweep_seconds = 0.0;
     long sweeps = 0;
     bool timing = true;

@@ -417,19 +417,60 @@ int check_sweep_error(allium_sweep *h, hipStream_t stream)
     return ALLIUM_OK;
 }
 
+// add the time of a finished ring slot to the total
+int harvest_timing(allium::DeviceSweep &ds, int slot)
+{
+    if (!ds.ring_busy[slot]) return ALLIUM_OK;
+    ds.ring_busy[slot] = false;
+    HIP_TRY(hipEventSynchronize(ds.ring1[slot]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ds.ring0[slot], ds.ring1[slot]));
+    ds.sweep_seconds += ms * 1e-3;
+    return ALLIUM_OK;
+}
+
+int harvest_all(allium::DeviceSweep &ds)
+{
+    for (int i = 0; i < allium::DeviceSweep::kTimingRing; i++) {
+        int rc = harvest_timing(ds, i);
+        if (rc) return rc;
+    }
+    return ALLIUM_OK;
+}
+
 int do_sweep(allium_sweep *h, const double *src, const double *iso, double *psi, hipStream_t stream)
 {
-    HIP_TRY(allium::launch_sweep(h->ds, src, iso, psi, stream));
-    if (h->ds.timing) {
-        HIP_TRY(hipEventSynchronize(h->ds.ev1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, h->ds.ev0, h->ds.ev1));
-        h->ds.sweep_seconds += ms * 1e-3;
+    allium::DeviceSweep &ds = h->ds;
+    if (!ds.timing) {
+        HIP_TRY(allium::launch_sweep(ds, src, iso, psi, stream));
+        return check_sweep_error(h, stream);
     }
+    // the events of this sweep come from the ring; a slot still in use holds a
+    // sweep 64 launches old, long finished in practice
+    const int slot = ds.ring_next;
+    ds.ring_next = (slot + 1) % allium::DeviceSweep::kTimingRing;
+    int rc = harvest_timing(ds, slot);
+    if (rc) return rc;
+    if (!ds.ring0[slot]) HIP_TRY(hipEventCreate(&ds.ring0[slot]));
+    if (!ds.ring1[slot]) HIP_TRY(hipEventCreate(&ds.ring1[slot]));
+    hipEvent_t e0 = ds.ev0, e1 = ds.ev1;
+    ds.ev0 = ds.ring0[slot];
+    ds.ev1 = ds.ring1[slot];
+    const hipError_t err = allium::launch_sweep(ds, src, iso, psi, stream);
+    ds.ev0 = e0;
+    ds.ev1 = e1;
+    HIP_TRY(err);
+    ds.ring_busy[slot] = true;
     return check_sweep_error(h, stream);
 }
 
 }  // namespace
+
+int allium_sweep_internal::sweep(allium_sweep *h, const double *src, const double *iso, double *psi,
+                                 hipStream_t stream)
+{
+    return do_sweep(h, src, iso, psi, stream);
+}
 
 extern "C" {
 
@@ -681,6 +722,8 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
         if (rc) return rc;
     }
     if (h->ds.graph_exec) { hipGraphExecDestroy(h->ds.graph_exec); h->ds.graph_exec = nullptr; }
+    int rc = harvest_all(h->ds);
+    if (rc) return rc;
     h->ds.sweeps = 0;
     h->ds.sweep_seconds = 0;
     return ALLIUM_OK;
@@ -820,15 +863,17 @@ int allium_mass_device(allium_sweep_t h, const double *d_coef, const double *d_x
     return ALLIUM_OK;
 }
 
-int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const double *source,
-                            const allium_si_options *opt, double *phi, double *psi, double *norms, int *iterations)
+// source, phi and psi are host pointers, or device pointers when `device`
+static int source_iteration(allium_sweep_t h, const double *sigma_s, const double *source,
+                            const allium_si_options *opt, double *phi, double *psi, double *norms, int *iterations,
+                            bool device)
 {
     if (!h || !sigma_s || !source || !opt || !phi) return fail(ALLIUM_ERR_ARG, "null argument");
     if (opt->maxit < 1) return fail(ALLIUM_ERR_ARG, "maxit < 1");
     if (!finite_all(sigma_s, h->mesh.Ne)) return fail(ALLIUM_ERR_NUMERIC, "non-finite sigma_s");
     hipStream_t s = h->stream;
     const long np = h->phi_size, nps = h->psi_size;
-    HIP_TRY(h->q.ensure(sizeof(double) * nps));
+    if (!device) HIP_TRY(h->q.ensure(sizeof(double) * nps));
     HIP_TRY(h->a.ensure(sizeof(double) * nps));
     HIP_TRY(h->b.ensure(sizeof(double) * np));
     HIP_TRY(h->x.ensure(sizeof(double) * np));
@@ -837,7 +882,9 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
     HIP_TRY(h->coef.ensure(sizeof(double) * h->mesh.Ne));
     const int nb = allium::si_update_blocks(np);
     HIP_TRY(h->part.ensure(sizeof(double) * nb));
-    HIP_TRY(hipMemcpyAsync(h->q.p, source, sizeof(double) * nps, hipMemcpyHostToDevice, s));
+    if (!device) HIP_TRY(hipMemcpyAsync(h->q.p, source, sizeof(double) * nps, hipMemcpyHostToDevice, s));
+    const double *dq = device ? source : h->q.d();
+    const hipMemcpyKind out_kind = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     HIP_TRY(hipMemcpyAsync(h->coef.p, sigma_s, sizeof(double) * h->mesh.Ne, hipMemcpyHostToDevice, s));
     const double inv_w = 1.0 / h->quad.wsum;
     const int prec_save = h->ds.precision;
@@ -954,7 +1001,7 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         for (it = 1;; it++) {
             HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
             HIP_TRY(allium::launch_apply_L(h->ds.prm, h->a.d(), h->c.d(), s));
-            HIP_TRY(allium::launch_axpby(nps, 1.0, h->q.d(), -1.0, h->c.d(), h->c.d(), s));
+            HIP_TRY(allium::launch_axpby(nps, 1.0, dq, -1.0, h->c.d(), h->c.d(), s));
             rc = do_sweep(h, h->c.d(), h->iso.d(), h->c.d(), s);
             if (rc) break;
             HIP_TRY(allium::launch_axpby(nps, 1.0, h->a.d(), 1.0, h->c.d(), h->a.d(), s));
@@ -975,10 +1022,10 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         h->ds.precision = prec_save;
         if (rc) return rc;
         // psi holds the FP64-corrected angular flux of the last iterate
-        if (psi) HIP_TRY(hipMemcpyAsync(psi, h->a.p, sizeof(double) * nps, hipMemcpyDeviceToHost, s));
+        if (psi) HIP_TRY(hipMemcpyAsync(psi, h->a.p, sizeof(double) * nps, out_kind, s));
     } else {
         h->ds.precision = 0;
-        rc = do_sweep(h, h->q.d(), nullptr, h->a.d(), s);
+        rc = do_sweep(h, dq, nullptr, h->a.d(), s);
         if (rc) { h->ds.precision = prec_save; return rc; }
         HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->b.d(), s));
         HIP_TRY(hipMemsetAsync(h->x.p, 0, sizeof(double) * np, s));
@@ -1002,17 +1049,30 @@ int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         if (rc) { h->ds.precision = prec_save; return rc; }
         // final sweep: psi = L^{-1}(D^T Ms phi + q)
         HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
-        rc = do_sweep(h, h->q.d(), h->iso.d(), h->a.d(), s);
+        rc = do_sweep(h, dq, h->iso.d(), h->a.d(), s);
         h->ds.precision = prec_save;
         if (rc) return rc;
-        if (psi) HIP_TRY(hipMemcpyAsync(psi, h->a.p, sizeof(double) * nps, hipMemcpyDeviceToHost, s));
+        if (psi) HIP_TRY(hipMemcpyAsync(psi, h->a.p, sizeof(double) * nps, out_kind, s));
     }
-    HIP_TRY(hipMemcpyAsync(phi, h->x.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(phi, h->x.p, sizeof(double) * np, out_kind, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (iterations) *iterations = it;
     const bool converged = norms ? (norms[it - 1] < r0) : true;
     if (!converged) return fail(ALLIUM_ERR_NOT_CONVERGED, "source iteration reached maxit");
     return ALLIUM_OK;
+}
+
+int allium_source_iteration(allium_sweep_t h, const double *sigma_s, const double *source,
+                            const allium_si_options *opt, double *phi, double *psi, double *norms, int *iterations)
+{
+    return source_iteration(h, sigma_s, source, opt, phi, psi, norms, iterations, false);
+}
+
+int allium_source_iteration_device(allium_sweep_t h, const double *sigma_s, const double *d_source,
+                                   const allium_si_options *opt, double *d_phi, double *d_psi, double *norms,
+                                   int *iterations)
+{
+    return source_iteration(h, sigma_s, d_source, opt, d_phi, d_psi, norms, iterations, true);
 }
 
 }  // extern "C"
@@ -1117,6 +1177,8 @@ int allium_sweep_batch_timing(allium_sweep_batch_t b, double *seconds, long *swe
 int allium_sweep_timing(allium_sweep_t h, double *sweep_seconds, long *sweeps)
 {
     if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    int rc = harvest_all(h->ds);
+    if (rc) return rc;
     if (sweep_seconds) *sweep_seconds = h->ds.sweep_seconds;
     if (sweeps) *sweeps = h->ds.sweeps;
     return ALLIUM_OK;

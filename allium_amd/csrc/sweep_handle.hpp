@@ -41,6 +41,8 @@ struct allium_sweep {
     std::vector<double> Kst;
     // scratch for host-pointer entry points and the source iteration
     DevBuf a, b, c, x, y, iso, coef, part, q, r, wavg, havg, xb;
+    // Krylov solver: basis V (and Z for FGMRES), work vectors, partial sums, Hessenberg
+    DevBuf kv, kz, kw, kt, kpart, kh;
     hipStream_t stream = nullptr;
     ~allium_sweep()
     {
@@ -50,9 +52,19 @@ struct allium_sweep {
         if (ds.graph_stream) hipStreamDestroy(ds.graph_stream);
         if (ds.ev0) hipEventDestroy(ds.ev0);
         if (ds.ev1) hipEventDestroy(ds.ev1);
+        for (int i = 0; i < allium::DeviceSweep::kTimingRing; i++) {
+            if (ds.ring0[i]) hipEventDestroy(ds.ring0[i]);
+            if (ds.ring1[i]) hipEventDestroy(ds.ring1[i]);
+        }
         if (stream) hipStreamDestroy(stream);
     }
 };
+
+namespace allium_sweep_internal {
+// psi = L^{-1}(src + iso), iso phi-sized and added in every direction (src or iso may be
+// NULL), with the handle's timing and error checks
+int sweep(allium_sweep *h, const double *src, const double *iso, double *psi, hipStream_t stream);
+}  // namespace allium_sweep_internal
 
 // quadrature of a descriptor: explicit directions or the level-symmetric set
 int allium_internal_quad(const allium_quad_desc *quad, int dim, allium::Quadrature &q);

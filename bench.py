@@ -114,10 +114,12 @@ def measure_traffic(args, tuning):
 def si_run(n, seed, maxit, learned=True, tuning=None):
     """Wall time to a 1e-8 FP64 scalar-flux residual on the 64^3 p=1 S8 c=0.9999
     medium (lognormal sigma_t, log-sd 1.5, Q=1): plain GPU source iteration
-    against source iteration with MIP DSA, with the learned correction (FP64
-    and FP32-refined sweeps) and with the learned correction falling back to
-    DSA.  The network is trained first on the GPU from seeded 32^3 media (not
-    part of the timed solves; its time is reported), as is the DSA setup."""
+    against source iteration with MIP DSA, GMRES(50) and FGMRES(50) on the same
+    operator, the learned correction in source iteration (FP64 and FP32-refined
+    sweeps, and falling back to DSA) and as FGMRES's preconditioner.  The
+    source is HBM-resident before each timed solve.  The network is trained
+    first on the GPU from seeded 32^3 media (not part of the timed solves; its
+    time is reported), as is the DSA setup."""
     import allium_amd as A
     from allium_amd import cnn
     from allium_amd import problems as P
@@ -133,12 +135,21 @@ def si_run(n, seed, maxit, learned=True, tuning=None):
             Linv.set_box_tiling(b["bx"], b["by"], b["bz"], dirs_per_group=b["dirs_per_group"], threads=b["threads"],
                                 prefetch=b["prefetch"])
         Linv.set_tuning(tuning["mode"], tuning["block"], tuning["lanes"])
-    q = Linv.form_source(Q=1.0)
+    q = A.DeviceVector.from_host(Linv.form_source(Q=1.0))
 
-    def solve(**kw):
-        raw = np.zeros(maxit)
+    def solve(krylov=None, **kw):
         A.synchronize()
         t = time.perf_counter()
+        if krylov:
+            phi, _, it, norms = A.krylov_solve(Linv, cfg["sigma_s"], q, solver=krylov, kdim=50, abstol=1e-8,
+                                               maxit=300, want_psi=False, **kw)
+            wall = time.perf_counter() - t
+            res = {"solver": krylov, "kdim": 50, "iterations": it, "final_residual": float(norms[-1]),
+                   "wall_s": wall, "converged": bool(norms[-1] <= 1e-8)}
+            if "accel" in kw:
+                res["learned_steps"] = A.krylov_solve.last_accel_steps
+            return phi, res
+        raw = np.zeros(maxit)
         phi, _, it, norms = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=maxit, want_psi=False,
                                                raw_norms=raw, **kw)
         wall = time.perf_counter() - t
@@ -160,6 +171,8 @@ def si_run(n, seed, maxit, learned=True, tuning=None):
     t = time.perf_counter()
     dsa = A.DSA(Linv, cfg["sigma_s"])
     dsa_setup = time.perf_counter() - t
+    compare("gmres_fp64", krylov="gmres")
+    compare("fgmres_fp64", krylov="fgmres")
     res = compare("dsa_fp64", accel=dsa)
     info = dsa.info()
     res.update(setup_s=dsa_setup, levels=info["level_size"], cg_iterations_per_application=info["cg_iterations"] /
@@ -171,6 +184,7 @@ def si_run(n, seed, maxit, learned=True, tuning=None):
     for name, prec in (("learned_fp64", A.FP64), ("learned_fp32_refined", A.FP32_REFINED)):
         compare(name, accel=net, precision=prec)
     compare("learned_fp32_refined_dsa_fallback", accel=net, precision=A.FP32_REFINED, fallback=dsa)
+    compare("fgmres_learned_fp64", krylov="fgmres", accel=net)
     return out
 
 

@@ -432,3 +432,21 @@ def test_p2_fp32_source_iteration_reaches_fp64_solution():
     ref_phi, _, _, _ = pr.source_iteration(c * sigma[0], q, abstol=1e-11)
     assert rel_err(phi64, ref_phi) < 1e-10
     assert rel_err(phi32, ref_phi) < 1e-9
+
+
+def test_sweep_timing_log_counts_every_sweep():
+    """The "sweep" timing log and "sweeps" count (src/sweep.cpp:665-668): the
+    per-sweep events are read lazily, so more sweeps than the event ring holds
+    (64) are all counted and timed."""
+    print("sweep timing log", flush=True)
+    Linv = A.InverseAdvectionOperator(A.Mesh((8, 8, 8)), 1, 4, np.ones(512))
+    src = A.DeviceVector.from_host(np.ones(Linv.size))
+    psi = A.DeviceVector(Linv.size)
+    t0, n0 = Linv.timing()
+    for _ in range(150):
+        Linv.Mult(src, psi)
+    t1, n1 = Linv.timing()
+    assert n1 - n0 == 150
+    assert t1 > t0 and (t1 - t0) / 150 < 0.1
+    t2, n2 = Linv.timing()
+    assert (t2, n2) == (t1, n1)
