@@ -1,10 +1,11 @@
 """chive-style steady one-group source iteration on the GPU.
 
 Reads a deck with the structure of chive's Lua inputs (exe/chive.cpp; e.g.
-test/chive/tdl.lua) written as YAML or a dict, runs the SI path of
+test/chive/tdl.lua) written as YAML or a dict, runs the outer-solve path of
 exe/chive.cpp:550-750 (FormTransportSource, InverseAdvectionOperator,
-SLISolver on TransportOperator, optional "scalar mip" DSA preconditioner,
-final sweep) and reports with chive's YAML keys ("transport iterations" with
+SLISolver, GMRESSolver or FGMRESSolver on TransportOperator as
+io::CreateIterativeSolver builds them (src/io.cpp:179-230), optional
+"scalar mip" DSA preconditioner, final sweep) and reports with chive's YAML keys ("transport iterations" with
 it / norm / inner solver, "outer iterations", "inner iteration" max / min /
 avg / total; exe/chive.cpp:40-75,738-746), so scripts that parse chive's
 output read this one too.
@@ -19,7 +20,7 @@ Deck keys (YAML):
     sn: {type: level symmetric, order: 4}
     driver:
       fe_order: 1
-      solver: {type: sli, abstol: 1.0e-6, reltol: 0, max_iter: 50}
+      solver: {type: sli, abstol: 1.0e-6, reltol: 0, max_iter: 50}   # or gmres / fgmres with kdim
       preconditioner: {type: scalar mip, kappa: 4, solver: {type: cg, reltol: 1.0e-10, max_iter: 200}}
 
 Material `source` and boundary `value` are per-steradian isotropic densities,
@@ -81,10 +82,14 @@ def run(deck):
     drv = deck.get("driver", {})
     p = int(drv.get("fe_order", 1))
     sol = drv.get("solver", {"type": "sli"})
-    if str(sol.get("type", "sli")).lower() != "sli":
-        raise ValueError("only the sli solver (source iteration) is supported")
+    stype = str(sol.get("type", "sli")).lower()
+    if stype not in ("sli", "gmres", "fgmres"):
+        raise ValueError(f"solver {stype} is not supported (sli, gmres, fgmres)")
     abstol, reltol = float(sol.get("abstol", 0.0)), float(sol.get("reltol", 0.0))
     maxit = int(sol.get("max_iter", 50))
+    kdim = int(sol.get("kdim", 50))
+    if stype != "sli" and not 1 <= kdim <= 200:
+        raise ValueError("kdim must be in 1..200")
     prec = drv.get("preconditioner")
     if prec and str(prec.get("type", "")).lower() not in ("scalar mip", "mip"):
         raise ValueError(f"preconditioner {prec.get('type')} is not supported (scalar mip only)")
@@ -108,24 +113,37 @@ def run(deck):
                                             "abstol": float(inner.get("abstol", 0.0)),
                                             "max_iter": int(inner.get("max_iter", 200))}}
     out["setup time"] = f"{time.perf_counter() - t0:.3e} s"
+    out["solver"] = {"type": stype, "abstol": abstol, "reltol": reltol, "max_iter": maxit}
+    if stype != "sli":
+        out["solver"]["kdim"] = kdim
     inner_its = np.zeros(maxit, dtype=np.int32)
     t1 = time.perf_counter()
-    phi, psi, it, norms = A.source_iteration(Linv, sigma_s, q, abstol=abstol, reltol=reltol, maxit=maxit,
-                                             accel=accel, inner_iterations=inner_its)
+    if stype == "sli":
+        phi, psi, it, norms = A.source_iteration(Linv, sigma_s, q, abstol=abstol, reltol=reltol, maxit=maxit,
+                                                 accel=accel, inner_iterations=inner_its)
+        # SLISolver's monitor: iteration k reports the norm of step k
+        steps = [(k + 1, norms[k]) for k in range(it)]
+        converged = bool(norms[-1] < max(reltol * norms[0], abstol))
+    else:
+        phi, psi, it, norms = A.krylov_solve(Linv, sigma_s, q, solver=stype, kdim=kdim, abstol=abstol, reltol=reltol,
+                                             maxit=maxit, accel=accel, inner_iterations=inner_its)
+        # mfem's Krylov monitors report the initial residual as iteration 0
+        steps = [(k, norms[k]) for k in range(norms.size)]
+        converged = bool(norms[-1] <= max(reltol * norms[0], abstol))
     solve = time.perf_counter() - t1
     rows = []
-    for k in range(it):
-        row = {"it": k + 1, "norm": f"{norms[k]:.3e}"}
-        if accel is not None:
-            row["inner solver"] = {"it": int(inner_its[k])}
+    for k, nrm in steps:
+        row = {"it": k, "norm": f"{nrm:.3e}"}
+        if accel is not None and k >= 1:
+            row["inner solver"] = {"it": int(inner_its[k - 1])}
         rows.append(row)
     out["transport iterations"] = rows
     out["outer iterations"] = int(it)
     if accel is not None:
-        ii = inner_its[:it]
+        ii = inner_its[:max(1, min(it, maxit))]
         out["inner iteration"] = {"max": int(ii.max()), "min": int(ii.min()), "avg": float(ii.mean()),
                                   "total": int(ii.sum())}
-    out["converged"] = bool(norms[-1] < max(reltol * norms[0], abstol))
+    out["converged"] = converged
     out["timings"] = {"solve": f"{solve:.3e} s"}
     out["scalar flux"] = {"l2": float(np.linalg.norm(phi)), "min": float(phi.min()), "max": float(phi.max())}
     out["balance"] = float(A.AdvectionOperator(Linv).ComputeBalance(psi))

@@ -111,7 +111,7 @@ def measure_traffic(args, tuning):
     return total, detail
 
 
-def si_run(n, seed, maxit, learned=True, tuning=None):
+def si_run(n, seed, maxit, learned=True, tuning=None, dsa_legs=True):
     """Wall time to a 1e-8 FP64 scalar-flux residual on the 64^3 p=1 S8 c=0.9999
     medium (lognormal sigma_t, log-sd 1.5, Q=1): plain GPU source iteration
     against source iteration with MIP DSA, GMRES(50) and FGMRES(50) on the same
@@ -167,23 +167,27 @@ def si_run(n, seed, maxit, learned=True, tuning=None):
         out[name] = res
         return res
 
-    # MIP diffusion synthetic acceleration (setup on the host, not timed in the solve)
-    t = time.perf_counter()
-    dsa = A.DSA(Linv, cfg["sigma_s"])
-    dsa_setup = time.perf_counter() - t
     compare("gmres_fp64", krylov="gmres")
     compare("fgmres_fp64", krylov="fgmres")
-    res = compare("dsa_fp64", accel=dsa)
-    info = dsa.info()
-    res.update(setup_s=dsa_setup, levels=info["level_size"], cg_iterations_per_application=info["cg_iterations"] /
-               max(1, info["applications"]), dsa_gpu_s=info["seconds"])
+    dsa = None
+    if dsa_legs:
+        # MIP diffusion synthetic acceleration (setup on the host, not timed in the solve)
+        t = time.perf_counter()
+        dsa = A.DSA(Linv, cfg["sigma_s"])
+        dsa_setup = time.perf_counter() - t
+        res = compare("dsa_fp64", accel=dsa)
+        info = dsa.info()
+        res.update(setup_s=dsa_setup, levels=info["level_size"],
+                   cg_iterations_per_application=info["cg_iterations"] / max(1, info["applications"]),
+                   dsa_gpu_s=info["seconds"])
     if not learned:
         return out
     net, info = cnn.fit(n=32, media=60, rounds=2, steps=1500, seed0=1000, sn=8, log=log)
     out["training"] = info
     for name, prec in (("learned_fp64", A.FP64), ("learned_fp32_refined", A.FP32_REFINED)):
         compare(name, accel=net, precision=prec)
-    compare("learned_fp32_refined_dsa_fallback", accel=net, precision=A.FP32_REFINED, fallback=dsa)
+    if dsa is not None:
+        compare("learned_fp32_refined_dsa_fallback", accel=net, precision=A.FP32_REFINED, fallback=dsa)
     compare("fgmres_learned_fp64", krylov="fgmres", accel=net)
     return out
 
@@ -304,7 +308,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-config1", action="store_true", help="skip the 2D chive-style SI (GPU and CPU oracle)")
     ap.add_argument("--no-si", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="plain source iteration only (no network training)")
+    ap.add_argument("--no-learned", action="store_true", help="no network training and no learned legs")
+    ap.add_argument("--no-dsa", action="store_true", help="skip the DSA legs of the 64^3 solves")
     ap.add_argument("--no-trt", action="store_true", help="skip the 20-step multigroup TRT run")
     ap.add_argument("--no-config3", action="store_true", help="skip the batched p=2 jittered instances")
     ap.add_argument("--si-maxit", type=int, default=3000)
@@ -402,7 +407,7 @@ def main():
     if not args.no_si:
         try:
             out["source_iteration"] = si_run(args.n, args.seed, args.si_maxit, learned=not args.no_learned,
-                                             tuning=tuning if args.tuning else None)
+                                             tuning=tuning if args.tuning else None, dsa_legs=not args.no_dsa)
             log("SI:", out["source_iteration"])
         except Exception as exc:  # reported, not hidden
             out["source_iteration"] = {"error": str(exc)}

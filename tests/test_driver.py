@@ -1,5 +1,7 @@
 """The chive-style driver (allium_amd/driver.py): deck validation on the CPU;
-the reference's thick-diffusion-limit acceptance check on the GPU."""
+the reference's thick-diffusion-limit acceptance check on the GPU, with
+source iteration and with the Krylov outer solvers its decks also name
+(test/chive/mms.lua, test/chive/easy_cp.lua: type = "gmres")."""
 import copy
 import os
 
@@ -18,7 +20,8 @@ def _deck():
 
 
 @pytest.mark.parametrize("edit,msg", [
-    (lambda d: d["driver"]["solver"].update(type="gmres"), "sli"),
+    (lambda d: d["driver"]["solver"].update(type="bicgstab"), "not supported"),
+    (lambda d: d["driver"]["solver"].update(type="gmres", kdim=0), "kdim"),
     (lambda d: d["sn"].update(type="abu shumays"), "level-symmetric"),
     (lambda d: d.update(material_map=["mat"] * 3), "one entry per element"),
     (lambda d: d.update(boundary_map=["vacuum"] * 3), "boundary_map"),
@@ -50,3 +53,31 @@ def test_tdl_deck_outer_iterations():
     assert float(rows[-1]["norm"]) < 1e-6 and rows[-1]["inner solver"]["it"] > 0
     assert out["inner iteration"]["total"] == sum(r["inner solver"]["it"] for r in rows)
     assert np.isfinite(out["balance"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", ["gmres", "fgmres"])
+def test_krylov_decks(solver):
+    """A gmres / fgmres deck reaches the fixed point source iteration reaches;
+    with the scalar MIP preconditioner the TDL deck needs fewer than 12
+    iterations, as chive's acceptance check asks of SLI there."""
+    print(f"driver {solver}", flush=True)
+    d = _deck()
+    d["materials"]["mat"].update(total=4.0, scattering=3.96, source=1.0)
+    d["driver"]["solver"] = {"type": "sli", "abstol": 1e-10, "max_iter": 3000}
+    ref = driver.run(copy.deepcopy(d))
+    d["driver"]["solver"] = {"type": solver, "abstol": 1e-10, "max_iter": 300, "kdim": 10}
+    out = driver.run(copy.deepcopy(d))
+    assert out["converged"] and ref["converged"]
+    assert out["outer iterations"] < ref["outer iterations"] / 2
+    assert out["scalar flux"]["l2"] == pytest.approx(ref["scalar flux"]["l2"], rel=1e-8)
+    rows = out["transport iterations"]
+    assert rows[0]["it"] == 0 and out["solver"]["kdim"] == 10
+    t = _deck()
+    t["driver"]["solver"] = {"type": solver, "abstol": 1e-6, "max_iter": 50}
+    t["driver"]["preconditioner"] = {"type": "scalar mip", "solver": {"type": "cg", "reltol": 1e-12,
+                                                                      "max_iter": 400}}
+    out = driver.run(t)
+    print(yaml.safe_dump({k: out[k] for k in ("outer iterations", "inner iteration")}))
+    assert out["converged"] and out["outer iterations"] < 12
+    assert out["inner iteration"]["total"] > 0
