@@ -228,6 +228,14 @@ class InverseAdvectionOperator:
         check(self._lib.allium_sweep_mult(self._h, _d(source), _d(psi)))
         return psi
 
+    def MultMoments(self, source, psi, phi, iso=None):
+        """psi = L^{-1}(source + iso) and phi = D psi in one call (DeviceVectors;
+        source or iso may be None): the sweep-and-moment pair of
+        TransportOperator::Mult."""
+        check(self._lib.allium_sweep_mult_moments_device(self._h, None if source is None else source.ptr,
+                                                         None if iso is None else iso.ptr, psi.ptr, phi.ptr, None))
+        return psi, phi
+
     def AssembleLocalMatrices(self, sigma_t):
         check(self._lib.allium_sweep_set_sigma(self._h, _d(_f64(sigma_t).reshape(-1))))
 

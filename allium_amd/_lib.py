@@ -90,6 +90,7 @@ SIGNATURES = {
     "allium_mass_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "allium_source_iteration": (ctypes.c_int, [c_vp, c_dp, c_dp, ctypes.POINTER(SIOptions), c_dp, c_dp, c_dp,
                                                c_ip]),
+    "allium_sweep_mult_moments_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "allium_krylov_default_options": (ctypes.c_int, [ctypes.POINTER(KrylovOptions)]),
     "allium_krylov_solve": (ctypes.c_int, [c_vp, c_dp, c_dp, ctypes.POINTER(KrylovOptions), c_dp, c_dp, c_dp,
                                            c_ip]),

@@ -149,8 +149,9 @@ struct DeviceSweep {
     bool timing = true;
 };
 
+// psi = L^{-1}(src + iso); with phi, the moment kernel then forms phi = D psi
 hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, double *psi,
-                        hipStream_t stream);
+                        hipStream_t stream, double *phi = nullptr);
 
 // Cooperative cell solves (local systems too large for one lane: 3D p = 2):
 // 32 lanes per cell, lane r owns row r.  One launch runs one level of up to

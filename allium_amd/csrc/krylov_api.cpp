@@ -13,7 +13,7 @@
 // the preconditioner change between steps, which the learned correction
 // needs: it is not linear.
 //
-// Device work per step: one operator application (a FP64 sweep plus mass and
+// Device work per step: one operator application (mass, a FP64 sweep, the
 // moments), i + 2 fused Gram-Schmidt launches over phi-sized vectors
 // (krylov.hip), and one host read of the new Hessenberg column.  The rotations
 // and the triangular solve run on the host (kdim <= 200 numbers).
@@ -97,9 +97,8 @@ struct Krylov {
     int apply_T(const double *x, double *out)
     {
         HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, x, h->iso.d(), s));
-        int rc = allium_sweep_internal::sweep(h, nullptr, h->iso.d(), h->a.d(), s);
+        int rc = allium_sweep_internal::sweep(h, nullptr, h->iso.d(), h->a.d(), s, h->y.d());
         if (rc) return rc;
-        HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->y.d(), s));
         HIP_TRY(allium::launch_axpby(np, 1.0, x, -1.0, h->y.d(), out, s));
         return ALLIUM_OK;
     }
@@ -285,9 +284,8 @@ int krylov_solve(allium_sweep_t h, const double *sigma_s, const double *source, 
     h->ds.precision = 0;
 
     // b = D L^{-1} q ; x = 0
-    int rc = allium_sweep_internal::sweep(h, dq, nullptr, h->a.d(), s);
+    int rc = allium_sweep_internal::sweep(h, dq, nullptr, h->a.d(), s, h->b.d());
     if (rc) return rc;
-    HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->b.d(), s));
     HIP_TRY(hipMemsetAsync(h->x.p, 0, sizeof(double) * np, s));
     double *b = h->b.d(), *x = h->x.d();
     if (opt->inner_iterations) std::fill(opt->inner_iterations, opt->inner_iterations + opt->maxit, 0);

@@ -438,11 +438,13 @@ int harvest_all(allium::DeviceSweep &ds)
     return ALLIUM_OK;
 }
 
-int do_sweep(allium_sweep *h, const double *src, const double *iso, double *psi, hipStream_t stream)
+// psi = L^{-1}(src + iso); phi (may be null) = D psi
+int do_sweep(allium_sweep *h, const double *src, const double *iso, double *psi, hipStream_t stream,
+             double *phi = nullptr)
 {
     allium::DeviceSweep &ds = h->ds;
     if (!ds.timing) {
-        HIP_TRY(allium::launch_sweep(ds, src, iso, psi, stream));
+        HIP_TRY(allium::launch_sweep(ds, src, iso, psi, stream, phi));
         return check_sweep_error(h, stream);
     }
     // the events of this sweep come from the ring; a slot still in use holds a
@@ -456,7 +458,7 @@ int do_sweep(allium_sweep *h, const double *src, const double *iso, double *psi,
     hipEvent_t e0 = ds.ev0, e1 = ds.ev1;
     ds.ev0 = ds.ring0[slot];
     ds.ev1 = ds.ring1[slot];
-    const hipError_t err = allium::launch_sweep(ds, src, iso, psi, stream);
+    const hipError_t err = allium::launch_sweep(ds, src, iso, psi, stream, phi);
     ds.ev0 = e0;
     ds.ev1 = e1;
     HIP_TRY(err);
@@ -467,9 +469,9 @@ int do_sweep(allium_sweep *h, const double *src, const double *iso, double *psi,
 }  // namespace
 
 int allium_sweep_internal::sweep(allium_sweep *h, const double *src, const double *iso, double *psi,
-                                 hipStream_t stream)
+                                 hipStream_t stream, double *phi)
 {
-    return do_sweep(h, src, iso, psi, stream);
+    return do_sweep(h, src, iso, psi, stream, phi);
 }
 
 extern "C" {
@@ -605,6 +607,15 @@ int allium_sweep_mult_iso_device(allium_sweep_t h, const double *d_source, const
 {
     if (!h || !d_iso || !d_psi) return fail(ALLIUM_ERR_ARG, "null argument");
     return do_sweep(h, d_source, d_iso, d_psi, (hipStream_t)stream);
+}
+
+int allium_sweep_mult_moments_device(allium_sweep_t h, const double *d_source, const double *d_iso, double *d_psi,
+                                     double *d_phi, void *stream)
+{
+    if (!h || !d_psi || !d_phi || (!d_source && !d_iso)) return fail(ALLIUM_ERR_ARG, "null argument");
+    if (d_phi == d_iso || d_phi == d_source || d_phi == d_psi)
+        return fail(ALLIUM_ERR_ARG, "phi must not alias the source, iso or psi");
+    return do_sweep(h, d_source, d_iso, d_psi, (hipStream_t)stream, d_phi);
 }
 
 int allium_sweep_mult(allium_sweep_t h, const double *source, double *psi)
@@ -1025,15 +1036,13 @@ static int source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         if (psi) HIP_TRY(hipMemcpyAsync(psi, h->a.p, sizeof(double) * nps, out_kind, s));
     } else {
         h->ds.precision = 0;
-        rc = do_sweep(h, dq, nullptr, h->a.d(), s);
+        rc = do_sweep(h, dq, nullptr, h->a.d(), s, h->b.d());
         if (rc) { h->ds.precision = prec_save; return rc; }
-        HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->b.d(), s));
         HIP_TRY(hipMemsetAsync(h->x.p, 0, sizeof(double) * np, s));
         for (it = 1;; it++) {
             HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
-            rc = do_sweep(h, nullptr, h->iso.d(), h->a.d(), s);
+            rc = do_sweep(h, nullptr, h->iso.d(), h->a.d(), s, h->y.d());
             if (rc) break;
-            HIP_TRY(allium::launch_moments(h->ds.prm, h->a.d(), h->y.d(), s));
             HIP_TRY(allium::launch_si_update(np, h->b.d(), h->y.d(), h->x.d(), h->part.d(), s,
                                              want_r ? h->r.d() : nullptr));
             double nrm = 0.0;

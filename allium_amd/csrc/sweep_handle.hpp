@@ -62,8 +62,9 @@ struct allium_sweep {
 
 namespace allium_sweep_internal {
 // psi = L^{-1}(src + iso), iso phi-sized and added in every direction (src or iso may be
-// NULL), with the handle's timing and error checks
-int sweep(allium_sweep *h, const double *src, const double *iso, double *psi, hipStream_t stream);
+// NULL), with the handle's timing and error checks; phi (may be NULL) then receives D psi
+int sweep(allium_sweep *h, const double *src, const double *iso, double *psi, hipStream_t stream,
+          double *phi = nullptr);
 }  // namespace allium_sweep_internal
 
 // quadrature of a descriptor: explicit directions or the level-symmetric set

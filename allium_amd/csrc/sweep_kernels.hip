@@ -718,12 +718,14 @@ hipError_t reset_persistent(DeviceSweep &ds, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, double *psi, hipStream_t stream)
+hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, double *psi, hipStream_t stream,
+                        double *phi)
 {
     SweepParams prm = ds.prm;
     prm.src = src;
     prm.iso = iso;
     prm.psi = psi;
+    if (phi && (phi == iso || (const double *)phi == src || phi == psi)) return hipErrorInvalidValue;
     hipError_t err = hipSuccess;
     if (ds.timing) hipEventRecord(ds.ev0, stream);
     if (ds.launch_mode == 1 || ds.launch_mode == 5) {
@@ -755,6 +757,7 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
     } else {
         err = dispatch(ds, prm, stream, ds.launch_mode);
     }
+    if (err == hipSuccess && phi) err = launch_moments(ds.prm, psi, phi, stream);
     if (ds.timing) hipEventRecord(ds.ev1, stream);
     ds.sweeps++;
     return err;

@@ -516,8 +516,7 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
     const double inv_w = 1.0 / h->wsum;
     // phi_src = D Linv (D^T em + q_psi)
     TRT_TRY(allium::launch_axpby(h->phi_size, inv_w, h->em, 0.0, h->em, h->iso, s));
-    ABI_TRY(allium_sweep_mult_iso_device(h->L, h->qpsi, h->iso, h->psi, s));
-    ABI_TRY(allium_moments_device(h->L, h->psi, h->phisrc, s));
+    ABI_TRY(allium_sweep_mult_moments_device(h->L, h->qpsi, h->iso, h->psi, h->phisrc, s));
     // source iteration on (I - D Linv D^T Ms) phi = phi_src (SLISolver, src/linalg.cpp:103-153)
     if (h->d.si_warm_start) ABI_TRY(allium_moments_device(h->L, h->psin, h->phi, s));
     else TRT_TRY(hipMemsetAsync(h->phi, 0, sizeof(double) * h->phi_size, s));
@@ -528,8 +527,7 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
     for (it = 1;; it++) {
         hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, inv_w, 0, h->iso);
         TRT_TRY(hipGetLastError());
-        ABI_TRY(allium_sweep_mult_iso_device(h->L, nullptr, h->iso, h->psi, s));
-        ABI_TRY(allium_moments_device(h->L, h->psi, h->phit, s));
+        ABI_TRY(allium_sweep_mult_moments_device(h->L, nullptr, h->iso, h->psi, h->phit, s));
         TRT_TRY(allium::launch_si_update(h->phi_size, h->phisrc, h->phit, h->phi, h->part, s));
         TRT_TRY(hipMemcpyAsync(hp.data(), h->part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
         TRT_TRY(hipStreamSynchronize(s));

@@ -161,6 +161,52 @@ def test_box_tilings_match_oracle(nel, p, sn, box, nd, threads):
     assert rel_err(Linv.Mult(q, np.empty_like(q)), ref) < 1e-4
 
 
+MOMENT_CASES = [BOX_TILINGS[1], BOX_TILINGS[6], BOX_TILINGS[7], BOX_TILINGS[8]]
+
+
+@pytest.mark.parametrize("nel,p,sn,box,nd,threads", MOMENT_CASES,
+                         ids=[f"{'x'.join(map(str, c[0]))}_p{c[1]}_b{'x'.join(map(str, c[3]))}"
+                              for c in MOMENT_CASES])
+def test_sweep_mult_moments(nel, p, sn, box, nd, threads):
+    """allium_sweep_mult_moments_device (the sweep and DiscreteToMoment pair of
+    TransportOperator::Mult) gives the psi of the sweep and the phi of the
+    moment kernel bit for bit, for a psi-sized source, an isotropic source and
+    both, in box and level launch modes."""
+    print(f"sweep + moments {nel} p{p} S{sn} box {box}", flush=True)
+    ne = int(np.prod(nel))
+    rng = np.random.default_rng(61)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+    Linv, pr = _pair(nel, p, sn, sigma)
+    q = pr.form_source(Q=1.0) + rng.uniform(0, 0.1, pr.size)
+    iso = rng.uniform(0.5, 1.5, Linv.phi_size)
+    dq, diso = A.DeviceVector.from_host(q), A.DeviceVector.from_host(iso)
+    psi0, psi1 = A.DeviceVector(Linv.size), A.DeviceVector(Linv.size)
+    phi0, phi1 = A.DeviceVector(Linv.phi_size), A.DeviceVector(Linv.phi_size)
+    D = A.DiscreteToMoment(Linv)
+    ref = pr.D(pr.sweep(q))
+    if len(nel) == 3:
+        Linv.set_box_tiling(*box, dirs_per_group=nd, threads=threads, prefetch=3 if p == 1 else 2)
+    else:
+        Linv.set_box_tiling(box[0], box[1], 1, dirs_per_group=nd, threads=threads, prefetch=2)
+    mult_iso = A._lib.load().allium_sweep_mult_iso_device
+    for mode in (4, 5, 0):
+        Linv.set_launch(mode)
+        for src, isov in ((dq, None), (None, diso), (dq, diso)):
+            if isov is None:
+                Linv.Mult(src, psi0)
+            else:
+                A.check(mult_iso(Linv._h, None if src is None else src.ptr, isov.ptr, psi0.ptr, None))
+            D.Mult(psi0, phi0)
+            Linv.MultMoments(src, psi1, phi1, iso=isov)
+            A.synchronize()
+            assert np.array_equal(psi0.to_host(), psi1.to_host()), (mode, src is None)
+            assert np.array_equal(phi0.to_host(), phi1.to_host()), (mode, src is None)
+        Linv.MultMoments(dq, psi1, phi1)
+        assert rel_err(phi1.to_host(), ref) < FP64_TOL
+    with pytest.raises(A.AlliumError):
+        Linv.MultMoments(dq, psi1, psi1)
+
+
 def test_box_tiling_errors():
     nel = (16, 16, 16)
     Linv, _ = _pair(nel, 1, 8, np.ones((1, 4096)))
