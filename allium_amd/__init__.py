@@ -228,6 +228,33 @@ class InverseAdvectionOperator:
         check(self._lib.allium_sweep_mult(self._h, _d(source), _d(psi)))
         return psi
 
+    FIXUPS = {"none": 0, "zero": 1, "zero and scale": 2, "ryosuke": 3}
+
+    def SetFixupOperator(self, kind, psi_min=0.0):
+        """Negative-flux fixup after every cell solve (src/fixup.cpp): "zero",
+        "zero and scale" or "ryosuke" (the deck's fixup type), "none" to turn
+        it off; resets the counts."""
+        check(self._lib.allium_sweep_set_fixup(self._h, self.FIXUPS[kind], float(psi_min)))
+
+    def UseFixup(self, use=True):
+        check(self._lib.allium_sweep_use_fixup(self._h, int(bool(use))))
+
+    def IsFixupOn(self):
+        t, m, on = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        check(self._lib.allium_sweep_get_fixup(self._h, ctypes.byref(t), ctypes.byref(m), ctypes.byref(on)))
+        return bool(on.value)
+
+    def SetFixupMonitorData(self, monitor):
+        """monitor: a DeviceVector of G x Ne counts (None stops monitoring)."""
+        self._fix_monitor = monitor
+        check(self._lib.allium_sweep_set_fixup_monitor(self._h, None if monitor is None else monitor.ptr))
+
+    def fixup_counts(self):
+        """(applied, failed): EventLog "fixup applied" / "fixup failed"."""
+        a, f = ctypes.c_longlong(), ctypes.c_longlong()
+        check(self._lib.allium_sweep_fixup_counts(self._h, ctypes.byref(a), ctypes.byref(f)))
+        return a.value, f.value
+
     def MultMoments(self, source, psi, phi, iso=None):
         """psi = L^{-1}(source + iso) and phi = D psi in one call (DeviceVectors;
         source or iso may be None): the sweep-and-moment pair of

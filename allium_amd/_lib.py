@@ -90,6 +90,12 @@ SIGNATURES = {
     "allium_mass_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "allium_source_iteration": (ctypes.c_int, [c_vp, c_dp, c_dp, ctypes.POINTER(SIOptions), c_dp, c_dp, c_dp,
                                                c_ip]),
+    "allium_sweep_set_fixup": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_double]),
+    "allium_sweep_use_fixup": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "allium_sweep_get_fixup": (ctypes.c_int, [c_vp, c_ip, c_dp, c_ip]),
+    "allium_sweep_set_fixup_monitor": (ctypes.c_int, [c_vp, c_vp]),
+    "allium_sweep_fixup_counts": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_longlong),
+                                                 ctypes.POINTER(ctypes.c_longlong)]),
     "allium_sweep_mult_moments_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "allium_krylov_default_options": (ctypes.c_int, [ctypes.POINTER(KrylovOptions)]),
     "allium_krylov_solve": (ctypes.c_int, [c_vp, c_dp, c_dp, ctypes.POINTER(KrylovOptions), c_dp, c_dp, c_dp,

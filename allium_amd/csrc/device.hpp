@@ -42,6 +42,12 @@ struct SweepParams {
     const double *iso;      // phi-sized isotropic source added to every direction (may be null)
     double *psi;
     const uint32_t *cells;  // (e << 10) | a, grouped by level
+    // negative-flux fixup after each cell solve (src/fixup.cpp): 0 off,
+    // 1 zero, 2 zero and scale, 3 ryosuke
+    int fixup;
+    double fix_min;
+    unsigned long long *fix_counts;   // applied, failed, ryosuke beyond 3 positive unknowns
+    double *fix_monitor;              // G x Ne application counts, or null
 };
 
 // One level of the direction-uniform wave plan: offset of the level in the
@@ -95,6 +101,9 @@ struct BoxPlan {
 struct DeviceSweep {
     SweepParams prm{};
     int precision = 0;       // 0 fp64 cell solves, 1 fp32 cell solves
+    // fixup state kept while it is switched off (UseFixup(false))
+    int fix_type = 0;
+    bool fix_on = false;
     int launch_mode = 0;     // 0 levels, 1 graph, 2 persistent, 3 direction sweeps, 4 box sweeps,
                              // 5 box sweeps in a graph
     int dir_block = 512;     // workgroup size of the direction-sweep kernel
@@ -149,6 +158,8 @@ struct DeviceSweep {
     bool timing = true;
 };
 
+// the level kernel's negative-flux fixups exist for these
+bool fixup_supported(int dim, int p);
 // psi = L^{-1}(src + iso); with phi, the moment kernel then forms phi = D psi
 hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, double *psi,
                         hipStream_t stream, double *phi = nullptr);

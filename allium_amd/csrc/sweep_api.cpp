@@ -409,6 +409,18 @@ bool finite_all(const double *v, long n)
 
 int check_sweep_error(allium_sweep *h, hipStream_t stream)
 {
+    if (h->ds.prm.fixup == 3) {
+        // RyosukeFixupOperator handles at most 3 positive unknowns; the reference aborts beyond
+        unsigned long long over = 0;
+        HIP_TRY(hipMemcpyAsync(&over, (unsigned long long *)h->fixbuf.p + 2, sizeof(over), hipMemcpyDeviceToHost,
+                               stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (over) {
+            HIP_TRY(hipMemsetAsync((unsigned long long *)h->fixbuf.p + 2, 0, sizeof(over), stream));
+            return fail(ALLIUM_ERR_NUMERIC, "ryosuke fixup: a cell has more than 3 positive unknowns after clipping "
+                                            "(src/fixup.cpp aborts here)");
+        }
+    }
     if (h->ds.launch_mode != 2) return ALLIUM_OK;
     int flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, h->ds.d_counters + h->ds.nkeys + 1, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -616,6 +628,63 @@ int allium_sweep_mult_moments_device(allium_sweep_t h, const double *d_source, c
     if (d_phi == d_iso || d_phi == d_source || d_phi == d_psi)
         return fail(ALLIUM_ERR_ARG, "phi must not alias the source, iso or psi");
     return do_sweep(h, d_source, d_iso, d_psi, (hipStream_t)stream, d_phi);
+}
+
+int allium_sweep_set_fixup(allium_sweep_t h, int type, double psi_min)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    if (type < ALLIUM_FIXUP_NONE || type > ALLIUM_FIXUP_RYOSUKE) return fail(ALLIUM_ERR_ARG, "unknown fixup type");
+    if (!std::isfinite(psi_min)) return fail(ALLIUM_ERR_NUMERIC, "non-finite psi_min");
+    if (type != ALLIUM_FIXUP_NONE && !allium::fixup_supported(h->mesh.dim, h->ed.p))
+        return fail(ALLIUM_ERR_UNSUPPORTED, "fixups are implemented for 2D p=0..2 and 3D p=0..1");
+    HIP_TRY(h->fixbuf.ensure(3 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(h->fixbuf.p, 0, 3 * sizeof(unsigned long long), h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    auto &ds = h->ds;
+    ds.fix_type = type;
+    ds.fix_on = type != ALLIUM_FIXUP_NONE;
+    ds.prm.fix_min = psi_min;
+    ds.prm.fix_counts = (unsigned long long *)h->fixbuf.p;
+    ds.prm.fixup = ds.fix_on ? type : 0;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_use_fixup(allium_sweep_t h, int use)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    auto &ds = h->ds;
+    ds.fix_on = use != 0 && ds.fix_type != ALLIUM_FIXUP_NONE;
+    ds.prm.fixup = ds.fix_on ? ds.fix_type : 0;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_get_fixup(allium_sweep_t h, int *type, double *psi_min, int *on)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    if (type) *type = h->ds.fix_type;
+    if (psi_min) *psi_min = h->ds.prm.fix_min;
+    if (on) *on = h->ds.fix_on ? 1 : 0;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_set_fixup_monitor(allium_sweep_t h, double *d_monitor)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    h->ds.prm.fix_monitor = d_monitor;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_fixup_counts(allium_sweep_t h, long long *applied, long long *failed)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    unsigned long long c[3] = {0, 0, 0};
+    if (h->fixbuf.p) {
+        HIP_TRY(hipMemcpyAsync(c, h->fixbuf.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    if (applied) *applied = (long long)c[0];
+    if (failed) *failed = (long long)c[1];
+    return ALLIUM_OK;
 }
 
 int allium_sweep_mult(allium_sweep_t h, const double *source, double *psi)

@@ -43,6 +43,8 @@ struct allium_sweep {
     DevBuf a, b, c, x, y, iso, coef, part, q, r, wavg, havg, xb;
     // Krylov solver: basis V (and Z for FGMRES), work vectors, partial sums, Hessenberg
     DevBuf kv, kz, kw, kt, kpart, kh;
+    // negative-flux fixup counters: applied, failed, ryosuke beyond 3 positive unknowns
+    DevBuf fixbuf;
     hipStream_t stream = nullptr;
     ~allium_sweep()
     {

@@ -22,7 +22,103 @@ namespace allium {
 // before its register LU.  OCT >= 0 fixes the direction's octant at compile
 // time (uniform geometry, direction-uniform waves): only the dim inflow sides
 // are gathered.  OCT < 0 decides inflow per lane with selects.
-template <int DIM, int P, bool UNI, typename T, int OCT>
+// Negative-flux fixup of one solved cell (src/fixup.cpp), given the column
+// sums wcol of the assembled matrix and the sum of its right-hand side:
+// 1 ZeroFixupOperator (clip at fix_min), 2 ZeroAndScaleFixupOperator (clip,
+// then scale to the balance sum(rhs) = wcol . x; failed if either sum is
+// negative), 3 RyosukeFixupOperator (clip, then solve for the positive
+// unknowns with their ratios kept and the balance restored; the reference
+// aborts beyond 3 positive unknowns, here a flag is raised).  Applications
+// and failures are counted (EventLog "fixup applied" / "fixup failed") and,
+// with a monitor, per element (SetFixupMonitorData).
+template <int N>
+__device__ __forceinline__ void fixup_cell(const SweepParams &p, const double (&wcol)[N], double bsum, double (&x)[N],
+                                           int e, int g)
+{
+    const double mn = p.fix_min;
+    int applied = 0;
+    if (p.fixup == 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (x[i] <= mn) { x[i] = mn; applied = 1; }
+    } else if (p.fixup == 2) {
+        double xmin = x[0];
+#pragma unroll
+        for (int i = 1; i < N; ++i) xmin = x[i] < xmin ? x[i] : xmin;
+        if (xmin <= mn) {
+            double now = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                if (x[i] <= mn) x[i] = mn;
+                now += wcol[i] * x[i];
+            }
+            if (bsum < 0.0 || now < 0.0) {
+                applied = -1;
+            } else {
+                const double sc = (bsum == 0.0 || now == 0.0) ? 0.0 : bsum / now;
+#pragma unroll
+                for (int i = 0; i < N; ++i) x[i] = sc == 0.0 ? 0.0 : x[i] * sc;
+                applied = 1;
+            }
+        }
+    } else {
+        int pos = 0, c0 = 0, c1 = 0, c2 = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            if (x[i] <= mn) {
+                x[i] = mn;
+            } else {
+                if (pos == 0) c0 = i;
+                else if (pos == 1) c1 = i;
+                else if (pos == 2) c2 = i;
+                ++pos;
+            }
+        }
+        if (pos > 0 && pos < N) {
+            if (pos > 3) {
+                atomicAdd(p.fix_counts + 2, 1ull);
+                applied = -1;
+            } else {
+                double x0 = 0.0, x1 = 0.0, x2 = 0.0, w0 = 0.0, w1 = 0.0, w2 = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    x0 = i == c0 ? x[i] : x0; w0 = i == c0 ? wcol[i] : w0;
+                    x1 = i == c1 ? x[i] : x1; w1 = i == c1 ? wcol[i] : w1;
+                    x2 = i == c2 ? x[i] : x2; w2 = i == c2 ? wcol[i] : w2;
+                }
+                double y0, y1 = 0.0, y2 = 0.0;
+                if (pos == 1) {
+                    y0 = bsum / w0;
+                } else if (pos == 2) {
+                    const double r1 = -x1 / x0;
+                    const double det = 1.0 / (w0 - w1 * r1);
+                    y0 = bsum * det;
+                    y1 = -bsum * r1 * det;
+                } else {
+                    const double r1 = -x1 / x0, r2 = -x2 / x0;
+                    const double det = 1.0 / (w0 - w1 * r1 - w2 * r2);
+                    y0 = bsum * det;
+                    y1 = -bsum * r1 * det;
+                    y2 = -bsum * r2 * det;
+                }
+                const bool fail = y0 < mn || (pos > 1 && y1 < mn) || (pos > 2 && y2 < mn);
+                if (fail) { y0 = y1 = y2 = mn; }
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    x[i] = i == c0 ? y0 : x[i];
+                    if (pos > 1) x[i] = i == c1 ? y1 : x[i];
+                    if (pos > 2) x[i] = i == c2 ? y2 : x[i];
+                }
+                applied = fail ? -1 : 1;
+            }
+        }
+    }
+    if (applied == 1) atomicAdd(p.fix_counts, 1ull);
+    else if (applied == -1) atomicAdd(p.fix_counts + 1, 1ull);
+    if (applied != 0 && p.fix_monitor) atomicAdd(p.fix_monitor + (long long)g * p.Ne + e, 1.0);
+}
+
+template <int DIM, int P, bool UNI, typename T, int OCT, bool FIX = false>
 __device__ __forceinline__ void cell_solve(const SweepParams &p, const int e, const int a, const int g)
 {
     using S = Sh<DIM, P>;
@@ -168,11 +264,29 @@ __device__ __forceinline__ void cell_solve(const SweepParams &p, const int e, co
         }
     }
 
-    // ---- phase 4: solve and store ----------------------------------------
+    // ---- phase 4: solve, fix up, store -----------------------------------
+    [[maybe_unused]] double wcol[N];
+    [[maybe_unused]] double bsum = 0.0;
+    if constexpr (FIX) {
+        // the fixups weigh the solution with the column sums of the matrix
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double sj = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) sj += (double)A[i][j];
+            wcol[j] = sj;
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) bsum += b[i];
+    }
     T x[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) x[i] = (T)b[i];
     lu_factor_solve<N, T>(A, x);
+    if constexpr (FIX) {
+        static_assert(sizeof(T) == sizeof(double), "fixups run on FP64 cell solves");
+        fixup_cell<N>(p, wcol, bsum, x, e, g);
+    }
     if constexpr (N % 2 == 0) {
         d2v *o2 = reinterpret_cast<d2v *>(p.psi + off);
 #pragma unroll
@@ -512,6 +626,18 @@ __global__ void __launch_bounds__(256) k_sweep_level(SweepParams p, long long cb
     cell_solve<DIM, P, UNI, T, -1>(p, (int)(cell >> 10), (int)(cell & 1023u), g);
 }
 
+// level kernel with the negative-flux fixup: one lane per cell, FP64
+template <int DIM, int P, bool UNI>
+__global__ void __launch_bounds__(256) k_sweep_level_fixup(SweepParams p, long long cbeg, int ncells)
+{
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= ncells * p.G) return;
+    int g = 0, c = item;
+    if (p.G > 1) { g = item / ncells; c = item - g * ncells; }
+    const uint32_t cell = p.cells[cbeg + c];
+    cell_solve<DIM, P, UNI, double, -1, true>(p, (int)(cell >> 10), (int)(cell & 1023u), g);
+}
+
 // Persistent sweep.  Tiles are handed out in level order by an atomic ticket,
 // so any tile a workgroup waits on was taken earlier by a running workgroup:
 // the wait always ends.  Each wait is also bounded by spin_limit; on overrun the
@@ -694,6 +820,44 @@ static hipError_t dispatch_dp(DeviceSweep &ds, const SweepParams &prm, hipStream
     return run_mode<DIM, P, false, double>(ds, prm, s, mode);
 }
 
+// sweeps with a fixup: the level kernel in every launch mode (the box and
+// direction kernels have no fixup), FP64 cells
+template <int DIM, int P>
+static hipError_t run_fixup_levels(DeviceSweep &ds, const SweepParams &prm, hipStream_t stream)
+{
+    const int block = 256;
+    for (int L = 0; L < ds.nlevels; ++L) {
+        const long long cb = ds.level_ptr[L];
+        const int nc = (int)(ds.level_ptr[L + 1] - cb);
+        if ((long)nc * prm.G == 0) continue;
+        const int grid = (int)(((long)nc * prm.G + block - 1) / block);
+        if (prm.uniform)
+            hipLaunchKernelGGL((k_sweep_level_fixup<DIM, P, true>), dim3(grid), dim3(block), 0, stream, prm, cb, nc);
+        else
+            hipLaunchKernelGGL((k_sweep_level_fixup<DIM, P, false>), dim3(grid), dim3(block), 0, stream, prm, cb, nc);
+    }
+    return hipGetLastError();
+}
+
+bool fixup_supported(int dim, int p) { return (dim == 2 && p >= 0 && p <= 2) || (dim == 3 && p >= 0 && p <= 1); }
+
+static hipError_t dispatch_fixup(DeviceSweep &ds, const SweepParams &prm, hipStream_t s)
+{
+    if (prm.dim == 2) {
+        switch (prm.p) {
+        case 0: return run_fixup_levels<2, 0>(ds, prm, s);
+        case 1: return run_fixup_levels<2, 1>(ds, prm, s);
+        case 2: return run_fixup_levels<2, 2>(ds, prm, s);
+        }
+    } else if (prm.dim == 3) {
+        switch (prm.p) {
+        case 0: return run_fixup_levels<3, 0>(ds, prm, s);
+        case 1: return run_fixup_levels<3, 1>(ds, prm, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
 static hipError_t dispatch(DeviceSweep &ds, const SweepParams &prm, hipStream_t s, int mode)
 {
     if (prm.dim == 2) {
@@ -728,7 +892,9 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
     if (phi && (phi == iso || (const double *)phi == src || phi == psi)) return hipErrorInvalidValue;
     hipError_t err = hipSuccess;
     if (ds.timing) hipEventRecord(ds.ev0, stream);
-    if (ds.launch_mode == 1 || ds.launch_mode == 5) {
+    if (prm.fixup) {
+        err = dispatch_fixup(ds, prm, stream);
+    } else if (ds.launch_mode == 1 || ds.launch_mode == 5) {
         // hipGraph of the level (1) or box-level (5) launches, re-captured when
         // the vectors change
         if (!ds.graph_exec || ds.graph_src != src || ds.graph_iso != iso || ds.graph_psi != psi ||

@@ -65,6 +65,8 @@ def _load():
     lib.orc_element_matrices.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
     lib.orc_face_blocks.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
     lib.orc_face_normal.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+    lib.orc_set_fixup.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp]
+    lib.orc_fixup_counts.argtypes = [ctypes.c_void_p, _lp]
     lib.orc_gll_nodes.argtypes = [ctypes.c_int, _dp]
     lib.orc_gauss_rule.argtypes = [ctypes.c_int, _dp, _dp]
     _lib = lib
@@ -136,6 +138,22 @@ class Problem:
 
     def set_time_absorption(self, tau):
         self._lib.orc_set_time_absorption(self._h, float(tau))
+
+    FIXUPS = {"none": 0, "zero": 1, "zero and scale": 2, "ryosuke": 3}
+
+    def set_fixup(self, kind, psi_min=0.0, monitor=None):
+        """Negative-flux fixup after every local solve (src/fixup.cpp): "none",
+        "zero", "zero and scale" or "ryosuke"; monitor: float64 array of G x Ne
+        that counts the applications per element.  Resets the counts."""
+        self._fix_monitor = monitor
+        self._lib.orc_set_fixup(self._h, self.FIXUPS[kind], float(psi_min),
+                                None if monitor is None else _d(monitor))
+
+    def fixup_counts(self):
+        """(applied, failed, ryosuke met more than 3 positive unknowns)"""
+        out = (ctypes.c_long * 3)()
+        self._lib.orc_fixup_counts(self._h, out)
+        return int(out[0]), int(out[1]), bool(out[2])
 
     def sweep(self, source, psi=None, threads=0):
         """psi = L^{-1} source (Mult_Upwind). psi may be source (in place)."""

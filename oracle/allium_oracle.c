@@ -54,6 +54,12 @@ struct orc_problem {
     long *lvl_ptr;
     int nlev;
     int *lvl;
+    /* negative-flux fixup (src/fixup.cpp): 0 off, 1 zero, 2 zero and scale, 3 ryosuke */
+    int fixup;
+    double fix_min;
+    long fix_applied, fix_failed;
+    int fix_error;     /* ryosuke with more than 3 positive unknowns */
+    double *fix_monitor;   /* G x Ne counts, or NULL */
 };
 
 /* ------------------------------------------------------------------------- */
@@ -705,6 +711,112 @@ static void add_mult(int n, const double *M, const double *x, double *y, double 
 }
 
 /* ------------------------------------------------------------------------- */
+/* negative-flux fixups (src/fixup.cpp), applied after each local solve      */
+/* (src/sweep.cpp:465-469).  A is the assembled local matrix (column-major,  */
+/* before the LU), rhs its right-hand side, x the solution.  Returns 1       */
+/* applied, -1 failed, 0 not applied; *err is set where the reference aborts */
+/* ------------------------------------------------------------------------- */
+
+/* weights(i) = sum_j A(j, i): column sums */
+static void column_sums(int n, const double *A, double *wcol)
+{
+    for (int i = 0; i < n; i++) {
+        wcol[i] = 0.0;
+        for (int j = 0; j < n; j++) wcol[i] += A[j + (size_t)i * n];
+    }
+}
+
+/* ZeroFixupOperator::Apply (src/fixup.cpp:3-13) */
+static int fixup_zero(int n, double mn, double *x)
+{
+    int applied = 0;
+    for (int i = 0; i < n; i++)
+        if (x[i] <= mn) { x[i] = mn; applied = 1; }
+    return applied;
+}
+
+/* ZeroAndScaleFixupOperator::Apply (src/fixup.cpp:15-57) */
+static int fixup_zero_scale(int n, double mn, const double *A, const double *rhs, double *x)
+{
+    double xmin = x[0];
+    for (int i = 1; i < n; i++) xmin = x[i] < xmin ? x[i] : xmin;
+    if (xmin > mn) return 0;
+    double wcol[ORC_MAXN];
+    column_sums(n, A, wcol);
+    for (int i = 0; i < n; i++)
+        if (x[i] <= mn) x[i] = mn;
+    double orig = 0.0, now = 0.0;
+    for (int i = 0; i < n; i++) orig += rhs[i];
+    for (int i = 0; i < n; i++) now += wcol[i] * x[i];
+    if (orig < 0.0 || now < 0.0) return -1;
+    if (orig == 0.0 || now == 0.0) {
+        for (int i = 0; i < n; i++) x[i] = 0.0;
+    } else {
+        const double sc = orig / now;
+        for (int i = 0; i < n; i++) x[i] *= sc;
+    }
+    return 1;
+}
+
+/* RyosukeFixupOperator::perform_nff (src/fixup.cpp:90-170): clip, then solve
+ * for the positive unknowns with their ratios kept and the balance
+ * sum_i colsum_i x_i = sum(rhs) restored; at most 3 positive unknowns */
+static int fixup_ryosuke(int n, double mn, const double *A, const double *rhs, double *x, int *err)
+{
+    int cnt[ORC_MAXN], pos = 0;
+    for (int i = 0; i < n; i++) {
+        if (x[i] <= mn) x[i] = mn;
+        else cnt[pos++] = i;
+    }
+    if (pos == 0 || pos == n) return 0;
+    if (pos > 3) { *err = 1; return -1; }
+    double wcol[ORC_MAXN];
+    column_sums(n, A, wcol);
+    double d0 = 0.0;
+    for (int j = 0; j < n; j++) d0 += rhs[j];
+    const double w0 = wcol[cnt[0]];
+    if (pos == 1) {
+        x[cnt[0]] = d0 / w0;
+    } else if (pos == 2) {
+        const double c2 = -x[cnt[1]] / x[cnt[0]];
+        const double det = 1.0 / (w0 - wcol[cnt[1]] * c2);
+        x[cnt[0]] = d0 * det;
+        x[cnt[1]] = -d0 * c2 * det;
+    } else {
+        const double c3 = -x[cnt[1]] / x[cnt[0]], c6 = -x[cnt[2]] / x[cnt[0]];
+        const double det = 1.0 / (w0 - wcol[cnt[1]] * c3 - wcol[cnt[2]] * c6);
+        x[cnt[0]] = d0 * det;
+        x[cnt[1]] = -d0 * c3 * det;
+        x[cnt[2]] = -d0 * c6 * det;
+    }
+    for (int i = 0; i < pos; i++)
+        if (x[cnt[i]] < mn) {
+            for (int j = 0; j < pos; j++) x[cnt[j]] = mn;
+            return -1;
+        }
+    return 1;
+}
+
+int orc_set_fixup(orc_problem *pr, int type, double psi_min, double *monitor)
+{
+    if (type < 0 || type > 3) return -1;
+    pr->fixup = type;
+    pr->fix_min = psi_min;
+    pr->fix_monitor = monitor;
+    pr->fix_applied = pr->fix_failed = 0;
+    pr->fix_error = 0;
+    return 0;
+}
+
+int orc_fixup_counts(const orc_problem *pr, long *out)
+{
+    out[0] = pr->fix_applied;
+    out[1] = pr->fix_failed;
+    out[2] = pr->fix_error;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Mult_Upwind (src/sweep.cpp:277-669)                                       */
 /* ------------------------------------------------------------------------- */
 
@@ -718,11 +830,14 @@ long orc_sweep(orc_problem *pr, const double *source, double *psi, int nthreads)
 #else
     (void)nthreads;
 #endif
+    long fa = 0, ff = 0;
+    int ferr = 0;
     for (int L = 0; L < pr->nlev; L++) {
         const long b = pr->lvl_ptr[L], eidx = pr->lvl_ptr[L + 1];
-#pragma omp parallel reduction(+ : failures)
+#pragma omp parallel reduction(+ : failures, fa, ff) reduction(| : ferr)
         {
             double grad[ORC_MAXN * ORC_MAXN], A[ORC_MAXN * ORC_MAXN], rhs[ORC_MAXN], psi2[ORC_MAXN], nor[3];
+            double A0[ORC_MAXN * ORC_MAXN], b0[ORC_MAXN];
 #pragma omp for schedule(dynamic, 32)
             for (long idx = b; idx < eidx; idx++) {
                 const int node = pr->order[idx];
@@ -781,12 +896,31 @@ long orc_sweep(orc_problem *pr, const double *source, double *psi, int nthreads)
                         const double *Mt = pr->Mt + (size_t)e * nn;
                         for (size_t i = 0; i < nn; i++) A[i] += pr->tau * Mt[i];
                     }
+                    if (pr->fixup) {
+                        memcpy(A0, A, sizeof(double) * nn);
+                        memcpy(b0, rhs, sizeof(double) * n);
+                    }
                     if (!linear_solve(n, A, rhs)) failures++;
+                    if (pr->fixup) {
+                        int applied = 0;
+                        if (pr->fixup == 1) applied = fixup_zero(n, pr->fix_min, rhs);
+                        else if (pr->fixup == 2) applied = fixup_zero_scale(n, pr->fix_min, A0, b0, rhs);
+                        else applied = fixup_ryosuke(n, pr->fix_min, A0, b0, rhs, &ferr);
+                        if (applied == 1) fa++;
+                        else if (applied == -1) ff++;
+                        if (pr->fix_monitor && applied) {
+#pragma omp atomic
+                            pr->fix_monitor[e + (size_t)g * pr->Ne] += 1.0;
+                        }
+                    }
                     for (int i = 0; i < n; i++) psi[off + i] = rhs[i];
                 }
             }
         }
     }
+    pr->fix_applied += fa;
+    pr->fix_failed += ff;
+    pr->fix_error |= ferr;
     return failures;
 }
 

@@ -51,6 +51,16 @@ int orc_set_time_absorption(orc_problem *pr, double tau);
  * number of local solves whose pivot fell below MFEM's LinearSolve TOL. */
 long orc_sweep(orc_problem *pr, const double *source, double *psi, int nthreads);
 
+/* negative-flux fixup after each local solve (SetFixupOperator / UseFixup,
+ * src/sweep.hpp:116-122; src/fixup.cpp): type 0 off, 1 zero, 2 zero and
+ * scale, 3 ryosuke; monitor (G x Ne, may be NULL) counts the applications per
+ * element (SetFixupMonitorData).  Resets the counts. */
+int orc_set_fixup(orc_problem *pr, int type, double psi_min, double *monitor);
+/* out[0] applied, out[1] failed (EventLog "fixup applied" / "fixup failed",
+ * src/sweep.cpp:649-650), out[2] 1 if ryosuke met more than 3 positive
+ * unknowns (where the reference aborts) */
+int orc_fixup_counts(const orc_problem *pr, long *out);
+
 /* AdvectionOperator::Mult and ComputeBalance */
 int orc_apply_L(const orc_problem *pr, const double *psi, double *Lpsi);
 double orc_balance(const orc_problem *pr, const double *psi);
