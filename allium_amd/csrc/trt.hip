@@ -586,6 +586,42 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
     return ALLIUM_OK;
 }
 
+int allium_trt_state_size(allium_trt_t h, long *size)
+{
+    if (!h || !size) return tfail(ALLIUM_ERR_ARG, "null argument");
+    *size = h->psi_size + h->Ndof;
+    return ALLIUM_OK;
+}
+
+int allium_trt_get_state(allium_trt_t h, double *x, int *cycle, double *time)
+{
+    if (!h || !x) return tfail(ALLIUM_ERR_ARG, "null argument");
+    TRT_TRY(hipMemcpyAsync(x, h->psi, sizeof(double) * h->psi_size, hipMemcpyDeviceToHost, h->s));
+    TRT_TRY(hipMemcpyAsync(x + h->psi_size, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToHost, h->s));
+    TRT_TRY(hipStreamSynchronize(h->s));
+    if (cycle) *cycle = h->step;
+    if (time) *time = h->time;
+    return ALLIUM_OK;
+}
+
+int allium_trt_set_state(allium_trt_t h, const double *x, int cycle, double time)
+{
+    if (!h || !x) return tfail(ALLIUM_ERR_ARG, "null argument");
+    if (cycle < 0 || !std::isfinite(time)) return tfail(ALLIUM_ERR_ARG, "bad cycle or time");
+    for (long k = 0; k < h->psi_size + h->Ndof; k++)
+        if (!std::isfinite(x[k])) return tfail(ALLIUM_ERR_NUMERIC, "non-finite state");
+    for (long k = 0; k < h->Ndof; k++)
+        if (!(x[h->psi_size + k] > 0.0)) return tfail(ALLIUM_ERR_NUMERIC, "temperature must be positive");
+    TRT_TRY(hipMemcpyAsync(h->psi, x, sizeof(double) * h->psi_size, hipMemcpyHostToDevice, h->s));
+    TRT_TRY(hipMemcpyAsync(h->T, x + h->psi_size, sizeof(double) * h->Ndof, hipMemcpyHostToDevice, h->s));
+    // what the state determines: the scalar flux and the opacities of the next step
+    ABI_TRY(allium_moments_device(h->L, h->psi, h->phi, h->s));
+    ABI_TRY(update_opacity(h));
+    h->step = cycle;
+    h->time = time;
+    return ALLIUM_OK;
+}
+
 int allium_trt_temperature(allium_trt_t h, double *T)
 {
     if (!h || !T) return tfail(ALLIUM_ERR_ARG, "null argument");

@@ -57,6 +57,7 @@ class TRT:
         check(self._lib.allium_trt_create(ctypes.byref(m), 1, ctypes.byref(q), ctypes.addressof(d), ctypes.byref(h)))
         self._h = h
         self.G = groups
+        self.dt = dt
         self.Ndof = self.mesh.Ne * 8
 
     def step(self):
@@ -73,6 +74,38 @@ class TRT:
         phi = np.empty(self.G * self.Ndof)
         check(self._lib.allium_trt_scalar_flux(self._h, phi.ctypes.data_as(_lib.c_dp)))
         return phi
+
+    def state(self):
+        """(x = [psi, T], step count, time): the state scallion restarts from."""
+        n = ctypes.c_long()
+        check(self._lib.allium_trt_state_size(self._h, ctypes.byref(n)))
+        x = np.empty(n.value)
+        cyc, t = ctypes.c_int(), ctypes.c_double()
+        check(self._lib.allium_trt_get_state(self._h, x.ctypes.data_as(_lib.c_dp), ctypes.byref(cyc), ctypes.byref(t)))
+        return x, cyc.value, t.value
+
+    def set_state(self, x, cycle, time):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        check(self._lib.allium_trt_set_state(self._h, x.ctypes.data_as(_lib.c_dp), int(cycle), float(time)))
+
+    def write_restart(self, root, output_cycle=0, keep=2):
+        """RestartWriter::Write of [psi, T] with this solver's cycle, time and time step."""
+        from .restart import write_restart
+        x, cyc, t = self.state()
+        write_restart(root, x, output_cycle=output_cycle, simulation_cycle=cyc, time=t, time_step=self.dt, keep=keep)
+
+    def load_restart(self, base, restart_id=0, use_restart_dt=True):
+        """LoadFromRestart into this solver; returns the file's metadata.  With
+        use_restart_dt the file's time step must be this solver's (scallion
+        then continues with it, exe/scallion.cpp:616-631)."""
+        from .restart import load_restart
+        n = ctypes.c_long()
+        check(self._lib.allium_trt_state_size(self._h, ctypes.byref(n)))
+        r = load_restart(base, restart_id, size=n.value)
+        if use_restart_dt and r["time_step"] != self.dt:
+            raise ValueError(f"restart time step {r['time_step']} differs from this solver's {self.dt}")
+        self.set_state(r["x"], r["simulation_cycle"], r["time"])
+        return r
 
     def set_launch(self, mode):
         check(self._lib.allium_sweep_set_launch(self._lib.allium_trt_sweep(self._h), mode))

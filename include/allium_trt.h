@@ -67,7 +67,15 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info);
 /* temperature dofs (Ne * n) and scalar flux (G * Ne * n, RIGHT layout) */
 int allium_trt_temperature(allium_trt_t h, double *T);
 int allium_trt_scalar_flux(allium_trt_t h, double *phi);
-/* the sweep handle the solver uses (tuning, timing) */
+/* The state scallion restarts from (exe/scallion.cpp:604-631, src/restart.cpp):
+ * the block vector x = [psi (psi_size), T (Ne * n)] with the step count and
+ * time.  set_state recomputes the scalar flux and the opacities from it, so a
+ * run continued from a saved state repeats the uninterrupted run bit for bit
+ * (the time step is the descriptor's). */
+int allium_trt_state_size(allium_trt_t h, long *size);
+int allium_trt_get_state(allium_trt_t h, double *x, int *cycle, double *time);
+int allium_trt_set_state(allium_trt_t h, const double *x, int cycle, double time);
+/* the sweep handle the solver uses (tuning, timing, fixups) */
 allium_sweep_t allium_trt_sweep(allium_trt_t h);
 
 /* host helpers: normalized Planck and Rosseland integrals over [0, x]
