@@ -690,34 +690,40 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
                                             int r, SplitIn &in)
 {
     constexpr int N = 2 << (DIM - 1);
-    in.ok = false;
-    in.q0 = in.q1 = in.i0 = in.i1 = in.sig = 0.0;
-    in.off = 0;
-    in.e = in.d = in.ci = 0;
-    if (l >= bp.nl) return;
+    // The three loads are issued on every path: a lane without a cell (or a
+    // level past the box) reads element 0 and never uses it, and a missing
+    // source reads zeros.  Loads behind branches would make the compiler's
+    // wait counts pessimistic at the join, and the wait for this level's
+    // operands would also drain the prefetch of the levels ahead.
     const BoxLds &L = cx.L;
-    const int c0 = L.lpl[l], cnt = L.lpl[l + 1] - c0;
-    if (it >= cnt * cx.ndir) return;
-    const int d = fast_div(it, cnt, 1.0f / (float)cnt), ci = c0 + (it - d * cnt);
+    const bool lv = l < bp.nl;
+    const int lc = lv ? l : 0;
+    const int c0 = L.lpl[lc], cnt = L.lpl[lc + 1] - c0;
+    const bool inr = lv && it < cnt * cx.ndir;
+    const int itc = inr ? it : 0;
+    const int d = fast_div(itc, cnt, 1.0f / (float)cnt), ci = c0 + (itc - d * cnt);
     const int e = L.cell_e[ci];
     const long long base = L.dbase[d];
-    if (e < 0 || base < 0) return;
-    in.e = e;
-    in.d = d;
-    in.ci = ci;
-    in.off = base + (long long)e * N + 2 * r;
-    in.ok = true;
-    if (p.src) {
-        const d2v v = *reinterpret_cast<const d2v *>(p.src + in.off);
-        in.q0 = v.x;
-        in.q1 = v.y;
-    }
-    if (p.iso) {
-        const d2v v = *reinterpret_cast<const d2v *>(p.iso + (long long)cx.g * p.Ndof + (long long)e * N + 2 * r);
-        in.i0 = v.x;
-        in.i1 = v.y;
-    }
-    in.sig = p.sigma[(long long)cx.g * p.Ne + e];
+    const bool ok = inr && e >= 0 && base >= 0;
+    const int es = ok ? e : 0;
+    const long long off = ok ? base + (long long)es * N + 2 * r : 0;
+    const double *srcp = p.src ? p.src + off : p.zeros;
+    const double *isop = p.iso ? p.iso + (long long)cx.g * p.Ndof + (long long)es * N + 2 * r : p.zeros;
+    const d2v vq = *reinterpret_cast<const d2v *>(srcp);
+    const d2v vi = *reinterpret_cast<const d2v *>(isop);
+    const double sg = p.sigma[(long long)cx.g * p.Ne + es];
+    in.ok = ok;
+    in.e = es;
+    in.d = ok ? d : 0;
+    in.ci = ok ? ci : 0;
+    in.off = off;
+    // no selects here: split_solve runs only where ok (all lanes of a cell
+    // agree), and a select would wait for the loads now instead of at use
+    in.q0 = vq.x;
+    in.q1 = vq.y;
+    in.i0 = vi.x;
+    in.i1 = vi.y;
+    in.sig = sg;
 }
 
 template <int DIM, typename T, int OCT>

@@ -41,6 +41,7 @@ struct SweepParams {
     const double *src;      // psi-sized source (may be null)
     const double *iso;      // phi-sized isotropic source added to every direction (may be null)
     double *psi;
+    const double *zeros;    // 64 zero doubles: the target of loads a kernel issues unconditionally
     const uint32_t *cells;  // (e << 10) | a, grouped by level
     // negative-flux fixup after each cell solve (src/fixup.cpp): 0 off,
     // 1 zero, 2 zero and scale, 3 ryosuke

@@ -175,6 +175,12 @@ int upload_all(allium_sweep *h)
     prm.Gd = dGd; prm.Fface = dF; prm.Ffacef = fF; prm.fnor = dN;
     HIP_TRY(upload(ds.owned, &h->d_sigma, h->sigma.data(), h->sigma.size()));
     prm.sigma = h->d_sigma;
+    {
+        const std::vector<double> z(64, 0.0);
+        double *dz = nullptr;
+        HIP_TRY(upload(ds.owned, &dz, z.data(), z.size()));
+        prm.zeros = dz;
+    }
     if (allium::coop_supported(dim, ed.p)) {
         // cooperative solves read their element's matrices row by row: order
         // each level by element, then direction, so that the cells one
