@@ -256,6 +256,25 @@ __global__ void k_time_source(long long cells, double tau, const double *__restr
     out[t] = fma(tau, a, qin ? qin[t] : 0.0);
 }
 
+// part[blk] = sum (a - b)^2 over a grid-stride share (summed on the host in block order)
+__global__ void k_diff2(long long n, const double *__restrict__ a, const double *__restrict__ b,
+                        double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double d = a[i] - b[i];
+        acc = fma(d, d, acc);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 __global__ void k_minmax(long long n, const double *__restrict__ x, double *__restrict__ part)
 {
     __shared__ double lo[256], hi[256];
@@ -306,6 +325,7 @@ struct allium_trt {
     std::vector<double> bounds, mid, sigma_h;
     TrtParams P{};
     std::vector<void *> owned;
+    double *Tk = nullptr;   // temperature before the current outer pass
     double *T = nullptr, *Tn = nullptr, *psi = nullptr, *psin = nullptr, *qin = nullptr, *qpsi = nullptr;
     double *sigma = nullptr, *em = nullptr, *phi = nullptr, *phisrc = nullptr, *phit = nullptr, *iso = nullptr;
     double *resid = nullptr, *Binv = nullptr, *dpw = nullptr, *part = nullptr, *dmid = nullptr;
@@ -383,8 +403,12 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     const allium_trt_desc &d = *desc;
     if (d.groups < 1 || d.groups > GMAX || !(d.dt > 0) || !(d.cv > 0) || !(d.T0 > 0) || !(d.T_inflow >= 0) ||
         !(d.opacity_coef > 0) || d.si_maxit < 1 || !(d.si_reltol >= 0) || !(d.si_abstol >= 0) ||
-        !(d.si_reltol > 0 || d.si_abstol > 0))
+        !(d.si_reltol > 0 || d.si_abstol > 0) || d.outer_maxit < 0 || !(d.outer_reltol >= 0) ||
+        !(d.outer_abstol >= 0) || (d.outer_maxit > 1 && !(d.outer_reltol > 0 || d.outer_abstol > 0)))
         return tfail(ALLIUM_ERR_ARG, "bad TRT description");
+    if (d.implicit_opacity && d.outer_maxit <= 1)
+        return tfail(ALLIUM_ERR_ARG, "implicit opacity needs the outer fixed-point iteration (outer_maxit > 1), "
+                                     "as in exe/scallion.cpp:986");
     auto h = std::make_unique<allium_trt>();
     h->d = d;
     h->G = d.groups;
@@ -444,6 +468,7 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     h->sigma_h = sig0;
     TRT_TRY(h->alloc(&h->T, h->Ndof));
     TRT_TRY(h->alloc(&h->Tn, h->Ndof));
+    TRT_TRY(h->alloc(&h->Tk, h->Ndof));
     TRT_TRY(h->alloc(&h->resid, h->Ndof));
     TRT_TRY(h->alloc(&h->Binv, (size_t)h->Ne * N * N));
     TRT_TRY(h->alloc(&h->dpw, (size_t)h->Ne * P.nq3 * h->G));
@@ -503,53 +528,86 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
     hipLaunchKernelGGL(k_time_source, dim3((unsigned)((cells * N + 255) / 256)), dim3(256), 0, s, cells,
                        1.0 / h->d.dt, P.Mm, h->qin, h->psin, h->qpsi);
     TRT_TRY(hipMemsetAsync(h->flags, 0, sizeof(int) * 4, s));
-    // linearize at T = T^n
-    hipLaunchKernelGGL(k_linearize, dim3(eb), dim3(64), 0, s, P, h->T, h->Tn, h->resid, h->Binv, h->em, h->dpw,
-                       h->flags);
-    // em += dB Bt'^{-1} resid
-    hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 0, h->resid, h->Binv, h->dpw, 1.0, 1, h->em);
-    TRT_TRY(hipGetLastError());
-    int bad = 0;
-    TRT_TRY(hipMemcpyAsync(&bad, h->flags, sizeof(int), hipMemcpyDeviceToHost, s));
-    TRT_TRY(hipStreamSynchronize(s));
-    if (bad) return tfail(ALLIUM_ERR_NUMERIC, "singular energy-balance Jacobian");
+    // LinearizedTRTOperator::Mult turns a fixup off for the transport solve and
+    // back on for the final sweep (src/trt_linearized.cpp:236-238,284-289)
+    int fix_type = 0, fix_on = 0;
+    double fix_min = 0.0;
+    ABI_TRY(allium_sweep_get_fixup(h->L, &fix_type, &fix_min, &fix_on));
     const double inv_w = 1.0 / h->wsum;
-    // phi_src = D Linv (D^T em + q_psi)
-    TRT_TRY(allium::launch_axpby(h->phi_size, inv_w, h->em, 0.0, h->em, h->iso, s));
-    ABI_TRY(allium_sweep_mult_moments_device(h->L, h->qpsi, h->iso, h->psi, h->phisrc, s));
-    // source iteration on (I - D Linv D^T Ms) phi = phi_src (SLISolver, src/linalg.cpp:103-153)
-    if (h->d.si_warm_start) ABI_TRY(allium_moments_device(h->L, h->psin, h->phi, s));
-    else TRT_TRY(hipMemsetAsync(h->phi, 0, sizeof(double) * h->phi_size, s));
     const int nb = allium::si_update_blocks(h->phi_size);
     std::vector<double> hp(nb);
-    double r0 = 0.0, nrm = 0.0;
-    int it = 0;
-    for (it = 1;; it++) {
-        hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, inv_w, 0, h->iso);
+    const int outer_max = std::max(1, h->d.outer_maxit);
+    double nrm = 0.0, onrm = 0.0, or0 = 0.0;
+    int it = 0, k = 0;
+    // FixedPointIterationSolver on the temperature (src/fixed_point.cpp:134-172,
+    // ComponentReductionOperator on T, exe/scallion.cpp:830): each pass is one
+    // LinearizedTRTOperator::Mult linearized at the current T with the sources
+    // of the previous time level; one pass without an outer solver
+    for (k = 1;; k++) {
+        if (outer_max > 1)
+            TRT_TRY(hipMemcpyAsync(h->Tk, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToDevice, s));
+        if (fix_on) ABI_TRY(allium_sweep_use_fixup(h->L, 0));
+        // linearize at the current T
+        hipLaunchKernelGGL(k_linearize, dim3(eb), dim3(64), 0, s, P, h->T, h->Tn, h->resid, h->Binv, h->em, h->dpw,
+                           h->flags);
+        // em += dB Bt'^{-1} resid
+        hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 0, h->resid, h->Binv, h->dpw, 1.0, 1, h->em);
         TRT_TRY(hipGetLastError());
-        ABI_TRY(allium_sweep_mult_moments_device(h->L, nullptr, h->iso, h->psi, h->phit, s));
-        TRT_TRY(allium::launch_si_update(h->phi_size, h->phisrc, h->phit, h->phi, h->part, s));
-        TRT_TRY(hipMemcpyAsync(hp.data(), h->part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+        int bad = 0;
+        TRT_TRY(hipMemcpyAsync(&bad, h->flags, sizeof(int), hipMemcpyDeviceToHost, s));
         TRT_TRY(hipStreamSynchronize(s));
-        nrm = 0.0;
-        for (double v : hp) nrm += v;
-        nrm = std::sqrt(nrm);
-        if (!std::isfinite(nrm)) return tfail(ALLIUM_ERR_NUMERIC, "non-finite transport residual");
-        if (it == 1) r0 = std::max(nrm * h->d.si_reltol, h->d.si_abstol);
-        if (nrm < r0 || it >= h->d.si_maxit) break;
+        if (bad) return tfail(ALLIUM_ERR_NUMERIC, "singular energy-balance Jacobian");
+        // phi_src = D Linv (D^T em + q_psi)
+        TRT_TRY(allium::launch_axpby(h->phi_size, inv_w, h->em, 0.0, h->em, h->iso, s));
+        ABI_TRY(allium_sweep_mult_moments_device(h->L, h->qpsi, h->iso, h->psi, h->phisrc, s));
+        // source iteration on (I - D Linv D^T Ms) phi = phi_src (SLISolver, src/linalg.cpp:103-153);
+        // a warm start begins from D psi of the current iterate
+        if (h->d.si_warm_start) ABI_TRY(allium_moments_device(h->L, k == 1 ? h->psin : h->psi, h->phi, s));
+        else TRT_TRY(hipMemsetAsync(h->phi, 0, sizeof(double) * h->phi_size, s));
+        double r0 = 0.0;
+        for (it = 1;; it++) {
+            hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, inv_w, 0,
+                               h->iso);
+            TRT_TRY(hipGetLastError());
+            ABI_TRY(allium_sweep_mult_moments_device(h->L, nullptr, h->iso, h->psi, h->phit, s));
+            TRT_TRY(allium::launch_si_update(h->phi_size, h->phisrc, h->phit, h->phi, h->part, s));
+            TRT_TRY(hipMemcpyAsync(hp.data(), h->part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+            TRT_TRY(hipStreamSynchronize(s));
+            nrm = 0.0;
+            for (double v : hp) nrm += v;
+            nrm = std::sqrt(nrm);
+            if (!std::isfinite(nrm)) return tfail(ALLIUM_ERR_NUMERIC, "non-finite transport residual");
+            if (it == 1) r0 = std::max(nrm * h->d.si_reltol, h->d.si_abstol);
+            if (nrm < r0 || it >= h->d.si_maxit) break;
+        }
+        // em += Ms phi ; psi = Linv(D^T em + q_psi), with the fixup back on
+        hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, 1.0, 1, h->em);
+        TRT_TRY(allium::launch_axpby(h->phi_size, inv_w, h->em, 0.0, h->em, h->iso, s));
+        if (fix_on) ABI_TRY(allium_sweep_use_fixup(h->L, 1));
+        ABI_TRY(allium_sweep_mult_iso_device(h->L, h->qpsi, h->iso, h->psi, s));
+        // temperature update
+        hipLaunchKernelGGL(k_update_T, dim3((h->Ne + 255) / 256), dim3(256), 0, s, P, h->phi, h->resid, h->Binv,
+                           h->T, h->flags + 1);
+        TRT_TRY(hipGetLastError());
+        // implicit opacity: OpacityUpdate::Update after every pass (src/trt_linearized.cpp:311)
+        if (h->d.implicit_opacity) ABI_TRY(update_opacity(h));
+        if (outer_max == 1) break;
+        hipLaunchKernelGGL(k_diff2, dim3(kPart), dim3(256), 0, s, (long long)h->Ndof, h->T, h->Tk, h->part);
+        TRT_TRY(hipGetLastError());
+        std::vector<double> dp(kPart);
+        TRT_TRY(hipMemcpyAsync(dp.data(), h->part, sizeof(double) * kPart, hipMemcpyDeviceToHost, s));
+        TRT_TRY(hipStreamSynchronize(s));
+        onrm = 0.0;
+        for (double v : dp) onrm += v;
+        onrm = std::sqrt(onrm);
+        if (!std::isfinite(onrm)) return tfail(ALLIUM_ERR_NUMERIC, "non-finite temperature change");
+        if (k == 1) or0 = std::max(h->d.outer_abstol, onrm * h->d.outer_reltol);
+        if (onrm < or0 || k >= outer_max) break;
     }
-    // em += Ms phi ; psi = Linv(D^T em + q_psi)
-    hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, 1.0, 1, h->em);
-    TRT_TRY(allium::launch_axpby(h->phi_size, inv_w, h->em, 0.0, h->em, h->iso, s));
-    ABI_TRY(allium_sweep_mult_iso_device(h->L, h->qpsi, h->iso, h->psi, s));
-    // temperature update
-    hipLaunchKernelGGL(k_update_T, dim3((h->Ne + 255) / 256), dim3(256), 0, s, P, h->phi, h->resid, h->Binv, h->T,
-                       h->flags + 1);
-    TRT_TRY(hipGetLastError());
     h->step++;
     h->time += h->d.dt;
     // explicit opacity for the next step
-    ABI_TRY(update_opacity(h));
+    if (!h->d.implicit_opacity) ABI_TRY(update_opacity(h));
     if (info) {
         int fl = 0;
         TRT_TRY(hipMemcpyAsync(&fl, h->flags + 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -563,6 +621,8 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
         info->time = h->time;
         info->transport_iterations = it;
         info->transport_residual = nrm;
+        info->outer_iterations = k;
+        info->outer_residual = onrm;
         info->floored = fl;
         info->T_min = 1e300;
         info->T_max = -1e300;

@@ -17,14 +17,16 @@ class TRTDesc(ctypes.Structure):
                 ("opacity_coef", ctypes.c_double), ("cv", ctypes.c_double), ("dt", ctypes.c_double),
                 ("T0", ctypes.c_double), ("T_inflow", ctypes.c_double), ("inflow_sides", ctypes.c_int),
                 ("si_reltol", ctypes.c_double), ("si_abstol", ctypes.c_double), ("si_maxit", ctypes.c_int),
-                ("si_warm_start", ctypes.c_int)]
+                ("si_warm_start", ctypes.c_int), ("outer_maxit", ctypes.c_int), ("outer_abstol", ctypes.c_double),
+                ("outer_reltol", ctypes.c_double), ("implicit_opacity", ctypes.c_int)]
 
 
 class TRTInfo(ctypes.Structure):
     _fields_ = [("step", ctypes.c_int), ("time", ctypes.c_double), ("transport_iterations", ctypes.c_int),
                 ("transport_residual", ctypes.c_double), ("floored", ctypes.c_long), ("T_min", ctypes.c_double),
                 ("T_max", ctypes.c_double), ("radiation_energy", ctypes.c_double),
-                ("sweep_seconds", ctypes.c_double), ("sweeps", ctypes.c_long)]
+                ("sweep_seconds", ctypes.c_double), ("sweeps", ctypes.c_long), ("outer_iterations", ctypes.c_int),
+                ("outer_residual", ctypes.c_double)]
 
 
 def planck_integrals(x):
@@ -45,13 +47,17 @@ class TRT:
 
     def __init__(self, nel, sn=6, groups=16, emin=1e-2, emax=1e1, opacity_coef=10.0, cv=1.0, dt=1e-3, T0=0.01,
                  T_inflow=1.0, inflow_sides=1, si_reltol=1e-8, si_abstol=0.0, si_maxit=1000, warm_start=False,
-                 quad=None):
+                 quad=None, outer_maxit=1, outer_abstol=0.0, outer_reltol=0.0, implicit_opacity=False):
+        """outer_*: scallion's fixed-point iteration on T around the linearized
+        step (one pass by default); implicit_opacity re-evaluates the opacities
+        after every pass."""
         from . import Mesh, _quad_desc
         self._lib = _lib.load()
         self.mesh = Mesh(nel)
         q, self._keep = _quad_desc(sn if quad is None else quad, 3)
         d = TRTDesc(groups, emin, emax, opacity_coef, cv, dt, T0, T_inflow, inflow_sides, si_reltol, si_abstol,
-                    si_maxit, int(warm_start))
+                    si_maxit, int(warm_start), int(outer_maxit), float(outer_abstol), float(outer_reltol),
+                    int(bool(implicit_opacity)))
         h = ctypes.c_void_p()
         m = self.mesh.desc()
         check(self._lib.allium_trt_create(ctypes.byref(m), 1, ctypes.byref(q), ctypes.addressof(d), ctypes.byref(h)))

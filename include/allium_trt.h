@@ -44,7 +44,17 @@ typedef struct {
     double si_abstol;     /*   max(si_reltol * first norm, si_abstol) or after si_maxit */
     int si_maxit;
     int si_warm_start;    /* 0: start from phi = 0 (SLISolver without iterative_mode);
-                             1: from D psi^n */
+                             1: from D psi of the current iterate (psi^n on the first pass) */
+    /* scallion's outer FixedPointIterationSolver on the temperature
+     * (src/fixed_point.cpp:134-172, exe/scallion.cpp:830-861): each pass
+     * relinearizes at the current T with the previous time level's sources, until
+     * ||T_k - T_{k-1}||_2 < max(outer_reltol * first change, outer_abstol) or
+     * outer_maxit passes.  0 or 1: one pass, the linearized solver alone. */
+    int outer_maxit;
+    double outer_abstol, outer_reltol;
+    int implicit_opacity; /* 1: opacities re-evaluated at T after every pass (OpacityUpdate,
+                             src/opacity_update.hpp, src/trt_linearized.cpp:311); needs
+                             outer_maxit > 1.  0: once per time step (explicit) */
 } allium_trt_desc;
 
 typedef struct {
@@ -57,6 +67,8 @@ typedef struct {
     double radiation_energy;    /* sum_g int phi_g (dimensionless E = phi / c) */
     double sweep_seconds; /* GPU time in sweeps so far */
     long sweeps;
+    int outer_iterations;       /* fixed-point passes of the last step */
+    double outer_residual;      /* last ||T_k - T_{k-1}||_2 (0 with one pass) */
 } allium_trt_info;
 
 int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,

@@ -94,8 +94,14 @@ class TRTOracle:
     """Linearized backward-Euler TRT on a uniform unit-box hex mesh, p = 1."""
 
     def __init__(self, nel, omega, weights, G=4, emin=1e-2, emax=1e1, coef=10.0, cv=1.0, dt=1e-3, T0=0.01,
-                 T_inflow=1.0, inflow_sides=1, si_reltol=1e-10, si_abstol=0.0, si_maxit=500, warm_start=False):
+                 T_inflow=1.0, inflow_sides=1, si_reltol=1e-10, si_abstol=0.0, si_maxit=500, warm_start=False,
+                 outer_maxit=1, outer_abstol=0.0, outer_reltol=0.0, implicit_opacity=False, fixup=None):
         self.nel = tuple(nel)
+        # (kind, psi_min): the sweep fixup, applied in the final sweep of each pass only
+        self.fixup = fixup
+        # FixedPointIterationSolver on T around the linearized step (src/fixed_point.cpp:134-172)
+        self.outer = (max(1, int(outer_maxit)), outer_abstol, outer_reltol, bool(implicit_opacity))
+        self.outer_iterations = []
         self.Ne = int(np.prod(nel))
         self.G, self.cv, self.dt, self.coef = G, cv, dt, coef
         self.si = (si_reltol, si_abstol, si_maxit, warm_start)
@@ -143,11 +149,40 @@ class TRTOracle:
         return self._apply_dB(u, dpw)
 
     def step(self):
-        pr, G, Ne = self.pr, self.G, self.Ne
+        """One time step: passes of LinearizedTRTOperator::Mult (src/trt_linearized.cpp:230-312)
+        until the temperature change meets the outer tolerance (one pass by
+        default); opacities after every pass (implicit) or once (explicit)."""
+        pr = self.pr
         Tn, psin = self.T.copy(), self.psi.copy()
         qpsi = self.qin + (1.0 / self.dt) * (psin.reshape(-1, 8) @ self.Mm.T).ravel()
+        maxit, abstol, reltol, implicit = self.outer
+        r0 = 0.0
+        for k in range(1, maxit + 1):
+            Tk = self.T.copy()
+            self._pass(Tn, qpsi)
+            if implicit:
+                self.sigma = self.opacity()
+                pr.set_sigma(self.sigma)
+            if maxit == 1:
+                break
+            nrm = float(np.linalg.norm(self.T - Tk))
+            if k == 1:
+                r0 = max(abstol, nrm * reltol)
+            if nrm < r0:
+                break
+        self.outer_iterations.append(k)
+        if not implicit:
+            self.sigma = self.opacity()
+            pr.set_sigma(self.sigma)
+
+    def _pass(self, Tn, qpsi):
+        """LinearizedTRTOperator::Mult at the current (psi, T) with the previous
+        time level's sources (Tn, qpsi)."""
+        pr, G, Ne = self.pr, self.G, self.Ne
         cvdt = self.cv / self.dt
         T = self.T
+        # the fixup is off for the transport solve (src/trt_linearized.cpp:236-238)
+        pr.set_fixup("none")
         # linearization (rule of order 4 for the gray balance, 3 for the groups)
         Tq4 = T @ self.phi4.T
         sp4, sr4 = spectrum(self.bounds, Tq4), spectrum(self.bounds, Tq4, True)
@@ -166,7 +201,7 @@ class TRTOracle:
         psi = pr.sweep(qpsi + pr.DT(em.ravel()))
         phisrc = pr.D(psi)
         reltol, abstol, maxit, warm = self.si
-        phi = pr.D(psin) if warm else np.zeros_like(phisrc)
+        phi = pr.D(self.psi) if warm else np.zeros_like(phisrc)
         r0 = None
         for it in range(1, maxit + 1):
             y = pr.D(pr.sweep(pr.DT(self._Ms(phi, Binv, dpw).ravel())))
@@ -179,9 +214,9 @@ class TRTOracle:
                 break
         self.iterations.append(it)
         em = em + self._Ms(phi, Binv, dpw)
+        if self.fixup is not None:
+            pr.set_fixup(*self.fixup)
         self.psi = pr.sweep(qpsi + pr.DT(em.ravel()))
         t = resid + np.einsum("ge,gei->ei", self.sigma, phi.reshape(G, Ne, 8) @ self.Mu.T)
         self.T = np.maximum(T + np.einsum("eij,ej->ei", Binv, t), 1e-8)
         self.phi = phi
-        self.sigma = self.opacity()
-        pr.set_sigma(self.sigma)

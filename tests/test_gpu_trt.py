@@ -61,3 +61,62 @@ def test_trt_config4_steps():
     T = g.temperature().reshape(32, 32, 32, 8)
     assert np.isfinite(T).all() and T.min() > 0.0
     assert T[:, :, 0].mean() > T[:, :, -1].mean()
+
+
+@pytest.mark.parametrize("implicit", [False, True])
+def test_trt_outer_fixed_point_matches_oracle(implicit):
+    """scallion's outer FixedPointIterationSolver on T (src/fixed_point.cpp:134-172)
+    around the linearized step, with explicit or implicit opacities
+    (OpacityUpdate, src/opacity_update.hpp): same passes as the oracle, same
+    T and phi; the converged step differs from the single linearized pass."""
+    print(f"trt outer fixed point, implicit opacity {implicit}", flush=True)
+    nel, G = (4, 3, 3), 4
+    om, w = ls_fixture(4, 3)
+    kw = dict(G=G, emin=1e-2, emax=1e1, coef=10.0, cv=1.0, dt=1e-2, T0=0.01, T_inflow=1.0, inflow_sides=1,
+              si_reltol=1e-12)
+    outer = dict(outer_maxit=40, outer_abstol=1e-11, implicit_opacity=implicit)
+    o = O.TRTOracle(nel, om, w, **kw, **outer)
+    one = O.TRTOracle(nel, om, w, **kw)
+    g = trt.TRT(nel, sn=4, groups=G, emin=1e-2, emax=1e1, opacity_coef=10.0, cv=1.0, dt=1e-2, T0=0.01, T_inflow=1.0,
+                inflow_sides=1, si_reltol=1e-12, **outer)
+    for step in range(3):
+        o.step()
+        one.step()
+        info = g.step()
+        eT = rel_err(g.temperature(), o.T.ravel())
+        ep = rel_err(g.scalar_flux(), o.phi)
+        print(f"  step {step + 1}: passes gpu {info['outer_iterations']} oracle {o.outer_iterations[-1]}, "
+              f"last change {info['outer_residual']:.2e}; T err {eT:.2e}, phi err {ep:.2e}; "
+              f"vs one pass {rel_err(o.T, one.T):.2e}")
+        assert eT < 1e-9 and ep < 1e-9
+        assert abs(info["outer_iterations"] - o.outer_iterations[-1]) <= 1 and info["outer_iterations"] > 2
+        assert info["outer_residual"] < 1e-11
+    assert rel_err(o.T, one.T) > 1e-6
+
+
+def test_trt_fixup_only_in_the_final_sweep():
+    """A sweep fixup set on the TRT handle is off for the transport solve and
+    on for the final sweep of each pass (src/trt_linearized.cpp:236-238,284-289)."""
+    print("trt fixup", flush=True)
+    import ctypes
+    from allium_amd import _lib
+    nel, G = (4, 3, 3), 2
+    om, w = ls_fixture(4, 3)
+    kw = dict(G=G, emin=1e-2, emax=1e1, coef=10.0, cv=1.0, dt=1e-3, T0=0.01, T_inflow=1.0, inflow_sides=1,
+              si_reltol=1e-12)
+    o = O.TRTOracle(nel, om, w, **kw, fixup=("zero and scale", 0.0))
+    g = trt.TRT(nel, sn=4, groups=G, emin=1e-2, emax=1e1, opacity_coef=10.0, cv=1.0, dt=1e-3, T0=0.01, T_inflow=1.0,
+                inflow_sides=1, si_reltol=1e-12)
+    lib = _lib.load()
+    L = lib.allium_trt_sweep(g._h)
+    _lib.check(lib.allium_sweep_set_fixup(L, 2, ctypes.c_double(0.0)))
+    applied = ctypes.c_longlong()
+    for step in range(2):
+        o.step()
+        g.step()
+        assert rel_err(g.temperature(), o.T.ravel()) < 1e-9 and rel_err(g.scalar_flux(), o.phi) < 1e-9
+    _lib.check(lib.allium_sweep_fixup_counts(L, ctypes.byref(applied), None))
+    print(f"  fixup applications on the GPU: {applied.value}")
+    on = ctypes.c_int()
+    _lib.check(lib.allium_sweep_get_fixup(L, None, None, ctypes.byref(on)))
+    assert on.value == 1
