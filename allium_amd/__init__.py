@@ -420,6 +420,25 @@ def host_mip_matrix(mesh, fe_order, quad, sigma_t, sigma_s, **desc):
     return rp, col, val
 
 
+def host_p1_matrix(mesh, fe_order, sigma_t, sigma_s):
+    """chive's "p1sa" 2x2 block operator as CSR arrays (row_ptr, col, val) over
+    the element-block layout [J_0, .., J_{dim-1}, phi] x n per element, host only."""
+    lib = _lib.load()
+    st = _f64(np.broadcast_to(sigma_t, (mesh.Ne,)))
+    ss = _f64(np.broadcast_to(sigma_s, (mesh.Ne,)))
+    nnz = ctypes.c_longlong(0)
+    md = mesh.desc()
+    check(lib.allium_host_p1_matrix(ctypes.byref(md), fe_order, _d(st), _d(ss), ctypes.byref(nnz), None, None, None))
+    N = (mesh.dim + 1) * (fe_order + 1) ** mesh.dim
+    rp = np.zeros(mesh.Ne * N + 1, dtype=np.int64)
+    col = np.zeros(nnz.value, dtype=np.int32)
+    val = np.zeros(nnz.value)
+    check(lib.allium_host_p1_matrix(ctypes.byref(md), fe_order, _d(st), _d(ss), ctypes.byref(nnz),
+                                    rp.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                    col.ctypes.data_as(_lib.c_ip), _d(val)))
+    return rp, col, val
+
+
 class DSA:
     """Diffusion synthetic acceleration with the scalar MIP operator on the GPU
     (DiffusionSyntheticAccelerationOperator, src/transport_op.hpp:38-59, as

@@ -142,6 +142,9 @@ SIGNATURES = {
     "allium_host_mip_matrix": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc), c_dp,
                                               c_dp, ctypes.POINTER(DSADesc), ctypes.POINTER(ctypes.c_longlong),
                                               ctypes.POINTER(ctypes.c_longlong), c_ip, c_dp]),
+    "allium_host_p1_matrix": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, c_dp, c_dp,
+                                             ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                             c_ip, c_dp]),
     # allium_trt.h (the descriptor/info structs are declared in allium_amd/trt.py)
     "allium_trt_create": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc), c_vp,
                                          ctypes.POINTER(c_vp)]),

@@ -683,6 +683,147 @@ bool assemble_mip(const Mesh &m, int p, const double *sigma_t, const double *sig
     return true;
 }
 
+bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigma_a, double alpha, MipMatrix &K,
+                 std::string &err)
+{
+    if (p < 1 || p > 2) { err = "P1SA needs fe_order 1 or 2"; return false; }
+    const int dim = m.dim;
+    int n = 1;
+    for (int d = 0; d < dim; d++) n *= p + 1;
+    const int N = (dim + 1) * n, ns = 2 * dim;
+    if (N > 32) { err = "P1SA blocks are implemented up to 32 unknowns per element (2D p<=2, 3D p=1)"; return false; }
+    if (!(alpha > 0)) { err = "P1SA needs alpha > 0"; return false; }
+    for (int e = 0; e < m.Ne; e++)
+        if (!(sigma_t[e] > 0) || !std::isfinite(sigma_a[e])) { err = "P1SA needs sigma_t > 0 and finite sigma_a"; return false; }
+    K.n = N; K.nside = ns; K.nc = 0;
+    K.Kd.assign((size_t)m.Ne * N * N, 0.0);
+    K.Ko.assign((size_t)m.Ne * ns * N * N, 0.0);
+    K.nb.assign((size_t)m.Ne * ns, -1);
+    K.P.clear();
+    Basis B{dim, p, n, gll_nodes(p)};
+    for (int e = 0; e < m.Ne; e++)
+        for (int sd = 0; sd < ns; sd++) {
+            const int f = m.face_of(e, sd);
+            const int o = (m.f_e1[f] == e) ? m.f_e2[f] : m.f_e1[f];
+            K.nb[(size_t)e * ns + sd] = (m.f_e2[f] >= 0) ? o : -1;
+        }
+    // row (c, i) / column (d, j) of an element block: c n + i; component dim is phi
+    auto at = [&](double *blk, int c, int i, int d, int j) -> double & { return blk[(size_t)(c * n + i) * N + d * n + j]; };
+    const TensorRule r_v = tensor_rule(dim, gauss(npts(2 * p + dim - 1)));
+    const TensorRule r_f = tensor_rule(dim - 1, gauss(npts(2 * p)));
+    // VectorMass(3 sigma_t), Mass(sigma_a), D = -(grad psi, J), G = -D^T
+    parallel_for(m.Ne, [&](long b, long eend) {
+        std::vector<double> phi(n), dphi(3 * n), g(3 * n);
+        double J[9], adj[9];
+        for (long e = b; e < eend; e++) {
+            double *Kd = &K.Kd[(size_t)e * N * N];
+            for (size_t q = 0; q < r_v.w.size(); q++) {
+                elem_jacobian(m, (int)e, r_v.x[q].data(), J);
+                const double det = det_adj(dim, J, adj);
+                B.eval(r_v.x[q].data(), phi.data(), dphi.data());
+                for (int i = 0; i < n; i++)
+                    for (int d = 0; d < dim; d++) {
+                        double v = 0.0;
+                        for (int r = 0; r < dim; r++) v += dphi[3 * i + r] * adj[r * 3 + d];
+                        g[3 * i + d] = v;   // det * physical gradient
+                    }
+                const double w = r_v.w[q] * std::fabs(det), wg = r_v.w[q] * (det > 0 ? 1.0 : -1.0);
+                for (int i = 0; i < n; i++)
+                    for (int j = 0; j < n; j++) {
+                        const double mm = w * phi[i] * phi[j];
+                        for (int c = 0; c < dim; c++) at(Kd, c, i, c, j) += 3.0 * sigma_t[e] * mm;
+                        at(Kd, dim, i, dim, j) += sigma_a[e] * mm;
+                        for (int c = 0; c < dim; c++) {
+                            const double dv = -wg * g[3 * i + c] * phi[j];   // row psi_i, column J_c,j
+                            at(Kd, dim, i, c, j) += dv;
+                            at(Kd, c, j, dim, i) -= dv;
+                        }
+                    }
+            }
+        }
+    });
+    // faces, one axis and parity at a time (no two faces of a pass share an element)
+    for (int ax = 0; ax < dim; ax++)
+        for (int par = 0; par < 2; par++) {
+            const int f0 = m.foff[ax], nfa = m.foff[ax + 1] - m.foff[ax];
+            parallel_for(nfa, [&](long b, long fend) {
+                std::vector<double> ph[2] = {std::vector<double>(n), std::vector<double>(n)}, dph(3 * n);
+                double xi[2][3], Nv[3], nh[3];
+                for (long fl = b; fl < fend; fl++) {
+                    const int f = f0 + (int)fl;
+                    const int *ext = m.fext[ax];
+                    const int c3[3] = {(int)(fl % ext[0]), (int)((fl / ext[0]) % ext[1]),
+                                       (int)(fl / ((long)ext[0] * ext[1]))};
+                    if ((c3[ax] & 1) != par) continue;
+                    const int e1 = m.f_e1[f], e2 = m.f_e2[f];
+                    if (e2 < 0) {
+                        const int side = m.f_bside[f], hi = side & 1;
+                        const bool marshak = m.bc[side] == 0;
+                        const double beta = marshak ? 1.0 / (2.0 * alpha) : 1.0 / alpha;
+                        double *Kd = &K.Kd[(size_t)e1 * N * N];
+                        for (size_t q = 0; q < r_f.w.size(); q++) {
+                            side_point(dim, ax, hi, r_f.x[q].data(), xi[0]);
+                            const double area = side_ortho(m, e1, ax, hi, xi[0], Nv);
+                            for (int d = 0; d < dim; d++) nh[d] = Nv[d] / area;
+                            B.eval(xi[0], ph[0].data(), dph.data());
+                            const double w = r_f.w[q];
+                            for (int i = 0; i < n; i++)
+                                for (int j = 0; j < n; j++) {
+                                    const double pp = ph[0][i] * ph[0][j];
+                                    for (int c = 0; c < dim; c++)
+                                        for (int d = 0; d < dim; d++)
+                                            at(Kd, c, i, d, j) += beta * w * area * nh[c] * nh[d] * pp;
+                                    if (!marshak) continue;
+                                    at(Kd, dim, i, dim, j) += 0.5 * alpha * w * area * pp;
+                                    for (int d = 0; d < dim; d++) {
+                                        const double dv = 0.5 * w * Nv[d] * pp;   // 0.5 psi J.n
+                                        at(Kd, dim, i, d, j) += dv;
+                                        at(Kd, d, j, dim, i) -= dv;
+                                    }
+                                }
+                        }
+                        continue;
+                    }
+                    // blocks (a, b) of the face: 0 = e1, 1 = e2
+                    double *blk[2][2] = {{&K.Kd[(size_t)e1 * N * N], &K.Ko[((size_t)e1 * ns + 2 * ax + 1) * N * N]},
+                                         {&K.Ko[((size_t)e2 * ns + 2 * ax) * N * N], &K.Kd[(size_t)e2 * N * N]}};
+                    for (size_t q = 0; q < r_f.w.size(); q++) {
+                        side_point(dim, ax, 1, r_f.x[q].data(), xi[0]);
+                        side_point(dim, ax, 0, r_f.x[q].data(), xi[1]);
+                        const double area = side_ortho(m, e1, ax, 1, xi[0], Nv);   // out of e1
+                        for (int d = 0; d < dim; d++) nh[d] = Nv[d] / area;
+                        B.eval(xi[0], ph[0].data(), dph.data());
+                        B.eval(xi[1], ph[1].data(), dph.data());
+                        const double w = r_f.w[q];
+                        for (int a = 0; a < 2; a++)
+                            for (int bb = 0; bb < 2; bb++) {
+                                const double sab = (a == bb) ? 1.0 : -1.0, sa = a == 0 ? 1.0 : -1.0;
+                                double *X = blk[a][bb];
+                                // the transpose block of G: rows of element bb, columns of element a
+                                double *XT = blk[bb][a];
+                                for (int i = 0; i < n; i++)
+                                    for (int j = 0; j < n; j++) {
+                                        const double pp = ph[a][i] * ph[bb][j];
+                                        at(X, dim, i, dim, j) += sab * 0.5 * alpha * w * area * pp;
+                                        for (int c = 0; c < dim; c++)
+                                            for (int d = 0; d < dim; d++)
+                                                at(X, c, i, d, j) += sab * (0.5 / alpha) * w * area * nh[c] * nh[d] * pp;
+                                        for (int d = 0; d < dim; d++) {
+                                            const double dv = 0.5 * sa * w * Nv[d] * pp;   // [psi]{J.n}
+                                            at(X, dim, i, d, j) += dv;
+                                            at(XT, d, j, dim, i) -= dv;
+                                        }
+                                    }
+                            }
+                    }
+                }
+            });
+        }
+    for (double v : K.Kd)
+        if (!std::isfinite(v)) { err = "non-finite P1 matrix"; return false; }
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // sweep graph levels (src/sweep.cpp:155-208 edges; 308-322, 477-512 Kahn)
 // ---------------------------------------------------------------------------

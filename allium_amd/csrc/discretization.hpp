@@ -88,6 +88,11 @@ struct MipMatrix {
 };
 bool assemble_mip(const Mesh &m, int p, const double *sigma_t, const double *sigma_a, const MipParams &prm,
                   MipMatrix &K, std::string &err);
+// chive's "p1sa" operator (P1Discretization::GetOperator, src/moment_discretization.cpp:400-490)
+// in the same element-block storage: per element the unknowns [J_0 (n), .., J_{dim-1} (n), phi (n)],
+// block size K.n = (dim + 1) n; K.P stays empty
+bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigma_a, double alpha, MipMatrix &K,
+                 std::string &err);
 
 // Sweep plan: level of every node e*Nomega+a and the cells grouped by level.
 struct SweepPlan {

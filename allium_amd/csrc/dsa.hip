@@ -81,6 +81,8 @@ hipError_t launch_dg(const DgLevel &L, const double *x, const double *b, double 
     case 4: hipLaunchKernelGGL((k_dg<4, MODE>), dim3(grid), dim3(kBlock), 0, s, L, x, b, out, partial); break;
     case 8: hipLaunchKernelGGL((k_dg<8, MODE>), dim3(grid), dim3(kBlock), 0, s, L, x, b, out, partial); break;
     case 9: hipLaunchKernelGGL((k_dg<9, MODE>), dim3(grid), dim3(kBlock), 0, s, L, x, b, out, partial); break;
+    case 12: hipLaunchKernelGGL((k_dg<12, MODE>), dim3(grid), dim3(kBlock), 0, s, L, x, b, out, partial); break;
+    case 32: hipLaunchKernelGGL((k_dg<32, MODE>), dim3(grid), dim3(kBlock), 0, s, L, x, b, out, partial); break;
     case 27: hipLaunchKernelGGL((k_dg<27, MODE>), dim3(grid), dim3(kBlock), 0, s, L, x, b, out, partial); break;
     default: return hipErrorInvalidValue;
     }

@@ -32,6 +32,43 @@ int fail(int code, const std::string &msg)
             return fail(ALLIUM_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
+// P1Discretization's alpha (BlockMomentDiscretization default; chive sets no other for p1sa)
+constexpr double kP1Alpha = 0.5;
+
+// K in CSR, rows and columns in its element-block layout; row_ptr == NULL
+// returns the entry count only
+int export_csr(const allium::MipMatrix &K, int Ne, long long *nnz, long long *row_ptr, int *col, double *val)
+{
+    const int n = K.n, ns = K.nside;
+    long long cnt = 0;
+    for (int e = 0; e < Ne; e++) {
+        int blocks = 1;
+        for (int s = 0; s < ns; s++) blocks += K.nb[(size_t)e * ns + s] >= 0;
+        cnt += (long long)blocks * n * n;
+    }
+    if (!row_ptr) { *nnz = cnt; return ALLIUM_OK; }
+    if (*nnz < cnt || !col || !val) return fail(ALLIUM_ERR_ARG, "CSR capacity too small");
+    long long k = 0;
+    for (int e = 0; e < Ne; e++)
+        for (int i = 0; i < n; i++) {
+            row_ptr[(long long)e * n + i] = k;
+            // columns ascending: blocks ordered by element id
+            std::vector<std::pair<int, const double *>> blk;
+            blk.push_back({e, &K.Kd[((size_t)e * n + i) * n]});
+            for (int s = 0; s < ns; s++) {
+                const int o = K.nb[(size_t)e * ns + s];
+                if (o >= 0) blk.push_back({o, &K.Ko[(((size_t)e * ns + s) * n + i) * n]});
+            }
+            std::sort(blk.begin(), blk.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+            for (auto &bb : blk)
+                for (int j = 0; j < n; j++) { col[k] = bb.first * n + j; val[k] = bb.second[j]; k++; }
+        }
+    row_ptr[(long long)Ne * n] = k;
+    *nnz = k;
+    return ALLIUM_OK;
+}
+
+
 // the coarsest vertex grid is inverted densely up to this size; coarsening
 // stops once a grid is at most kCoarseTarget vertices
 constexpr long long kDenseMax = 1331;
@@ -680,33 +717,22 @@ int allium_host_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const all
     allium::MipMatrix K;
     if (!allium::assemble_mip(m, fe_order, sigma_t, siga.data(), mip_params(dd, fe_order, q), K, why))
         return fail(ALLIUM_ERR_ARG, why);
-    const int n = K.n, ns = K.nside;
-    long long cnt = 0;
-    for (int e = 0; e < m.Ne; e++) {
-        int blocks = 1;
-        for (int s = 0; s < ns; s++) blocks += K.nb[(size_t)e * ns + s] >= 0;
-        cnt += (long long)blocks * n * n;
-    }
-    if (!row_ptr) { *nnz = cnt; return ALLIUM_OK; }
-    if (*nnz < cnt || !col || !val) return fail(ALLIUM_ERR_ARG, "CSR capacity too small");
-    long long k = 0;
-    for (int e = 0; e < m.Ne; e++)
-        for (int i = 0; i < n; i++) {
-            row_ptr[(long long)e * n + i] = k;
-            // columns ascending: blocks ordered by element id
-            std::vector<std::pair<int, const double *>> blk;
-            blk.push_back({e, &K.Kd[((size_t)e * n + i) * n]});
-            for (int s = 0; s < ns; s++) {
-                const int o = K.nb[(size_t)e * ns + s];
-                if (o >= 0) blk.push_back({o, &K.Ko[(((size_t)e * ns + s) * n + i) * n]});
-            }
-            std::sort(blk.begin(), blk.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
-            for (auto &bb : blk)
-                for (int j = 0; j < n; j++) { col[k] = bb.first * n + j; val[k] = bb.second[j]; k++; }
-        }
-    row_ptr[(long long)m.Ne * n] = k;
-    *nnz = k;
-    return ALLIUM_OK;
+    return export_csr(K, m.Ne, nnz, row_ptr, col, val);
+}
+
+int allium_host_p1_matrix(const allium_mesh_desc *mesh, int fe_order, const double *sigma_t, const double *sigma_s,
+                          long long *nnz, long long *row_ptr, int *col, double *val)
+{
+    if (!mesh || !sigma_t || !sigma_s || !nnz) return fail(ALLIUM_ERR_ARG, "null argument");
+    allium::Mesh m;
+    std::string why;
+    if (!allium::build_mesh(mesh->dim, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, m, why))
+        return fail(ALLIUM_ERR_ARG, why);
+    std::vector<double> siga(m.Ne);
+    for (int e = 0; e < m.Ne; e++) siga[e] = sigma_t[e] - sigma_s[e];
+    allium::MipMatrix K;
+    if (!allium::assemble_p1(m, fe_order, sigma_t, siga.data(), kP1Alpha, K, why)) return fail(ALLIUM_ERR_ARG, why);
+    return export_csr(K, m.Ne, nnz, row_ptr, col, val);
 }
 
 }  // extern "C"

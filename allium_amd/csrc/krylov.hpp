@@ -3,7 +3,34 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 namespace allium {
+
+// Givens rotation (cs, sn) that zeroes dy against dx, and its application
+// (mfem's GeneratePlaneRotation / ApplyPlaneRotation, linalg/solvers.cpp)
+inline void givens(double dx, double dy, double &cs, double &sn)
+{
+    if (dy == 0.0) {
+        cs = 1.0;
+        sn = 0.0;
+    } else if (std::fabs(dy) > std::fabs(dx)) {
+        const double t = dx / dy;
+        sn = 1.0 / std::sqrt(1.0 + t * t);
+        cs = t * sn;
+    } else {
+        const double t = dy / dx;
+        cs = 1.0 / std::sqrt(1.0 + t * t);
+        sn = t * cs;
+    }
+}
+
+inline void apply_givens(double &dx, double &dy, double cs, double sn)
+{
+    const double t = cs * dx + sn * dy;
+    dy = -sn * dx + cs * dy;
+    dx = t;
+}
 
 // Reductions write one partial sum per block; the consumer of a partial array
 // re-sums it in a fixed order, so results do not depend on scheduling.

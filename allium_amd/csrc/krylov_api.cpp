@@ -52,29 +52,6 @@ int fail(int code, const std::string &msg)
 constexpr int kMaxKdim = 200;
 constexpr int kUpdateChunk = 64;
 
-void rotation(double dx, double dy, double &cs, double &sn)
-{
-    if (dy == 0.0) {
-        cs = 1.0;
-        sn = 0.0;
-    } else if (std::fabs(dy) > std::fabs(dx)) {
-        const double t = dx / dy;
-        sn = 1.0 / std::sqrt(1.0 + t * t);
-        cs = t * sn;
-    } else {
-        const double t = dy / dx;
-        cs = 1.0 / std::sqrt(1.0 + t * t);
-        sn = t * cs;
-    }
-}
-
-void apply_rotation(double &dx, double &dy, double cs, double sn)
-{
-    const double t = cs * dx + sn * dy;
-    dy = -sn * dx + cs * dy;
-    dx = t;
-}
-
 struct Krylov {
     allium_sweep *h;
     hipStream_t s;
@@ -346,10 +323,10 @@ int krylov_solve(allium_sweep_t h, const double *sigma_s, const double *source, 
             double *hc = Hh.data() + (size_t)i * (m + 1);
             HIP_TRY(hipMemcpyAsync(hc, Hcol, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
-            for (int k = 0; k < i; k++) apply_rotation(hc[k], hc[k + 1], cs[k], sn[k]);
-            rotation(hc[i], hc[i + 1], cs[i], sn[i]);
-            apply_rotation(hc[i], hc[i + 1], cs[i], sn[i]);
-            apply_rotation(sv[i], sv[i + 1], cs[i], sn[i]);
+            for (int k = 0; k < i; k++) allium::apply_givens(hc[k], hc[k + 1], cs[k], sn[k]);
+            allium::givens(hc[i], hc[i + 1], cs[i], sn[i]);
+            allium::apply_givens(hc[i], hc[i + 1], cs[i], sn[i]);
+            allium::apply_givens(sv[i], sv[i + 1], cs[i], sn[i]);
             const double res = std::fabs(sv[i + 1]);
             if (norms) norms[j] = res;
             if (!std::isfinite(res)) return fail(ALLIUM_ERR_NUMERIC, "non-finite residual estimate");

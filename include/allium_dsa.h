@@ -80,6 +80,12 @@ int allium_dsa_get_info(allium_dsa_t d, allium_dsa_info *info);
 int allium_host_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
                            const double *sigma_t, const double *sigma_s, const allium_dsa_desc *desc,
                            long long *nnz, long long *row_ptr, int *col, double *val);
+/* Host only: chive's "p1sa" operator, P1Discretization::GetOperator
+ * (src/moment_discretization.cpp:400-490) with alpha 0.5, in CSR over the
+ * element-block layout [J_0 (n), .., J_{dim-1} (n), phi (n)] per element
+ * ((dim + 1) Ndof rows).  2D p = 1, 2 and 3D p = 1. */
+int allium_host_p1_matrix(const allium_mesh_desc *mesh, int fe_order, const double *sigma_t, const double *sigma_s,
+                          long long *nnz, long long *row_ptr, int *col, double *val);
 
 #ifdef __cplusplus
 }

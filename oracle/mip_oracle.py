@@ -7,6 +7,8 @@ Restates, in FP64 numpy/scipy:
     penalty factor and lower bound as exe/chive.cpp:663-682 sets them;
     ComputeAlpha, src/angular_quadrature.cpp:266-276) on structured quad/hex
     meshes, returned as a scipy CSR matrix in the phi layout;
+  * chive's "p1sa" 2x2 block operator (p1_matrix, P1Discretization,
+    src/moment_discretization.cpp:400-490);
   * DiffusionSyntheticAccelerationOperator (src/transport_op.hpp:38-59) inside
     SLISolver (src/linalg.cpp:103-153): z = r + K^{-1} Ms r, x += z, norm ||z||,
     with K solved directly (scipy's sparse LU), as chive's "direct" inner solver.
@@ -205,10 +207,12 @@ def mip_matrix(dim, nel, vertices, p, sigma_t, sigma_s, omega, weights, bc=None,
     return K.tocsr()
 
 
-def dsa_source_iteration(pr, sigma_s, source, K, abstol=1e-8, reltol=0.0, maxit=1000):
+def dsa_source_iteration(pr, sigma_s, source, K, abstol=1e-8, reltol=0.0, maxit=1000, solve=None):
     """SLISolver on T = I - D L^{-1} D^T Ms with the DSA preconditioner
-    z = r + K^{-1} Ms r; returns (phi, iterations, norms)."""
-    solve = spla.factorized(sp.csc_matrix(K))
+    z = r + K^{-1} Ms r; returns (phi, iterations, norms).  `solve` replaces
+    the direct solve of K (e.g. p1_solver)."""
+    if solve is None:
+        solve = spla.factorized(sp.csc_matrix(K))
     ms = lambda x: pr.mass(sigma_s, x)
     b = pr.D(pr.sweep(source))
     x = np.zeros_like(b)
@@ -225,3 +229,121 @@ def dsa_source_iteration(pr, sigma_s, source, K, abstol=1e-8, reltol=0.0, maxit=
         if nrm < r0:
             break
     return x, it, np.array(norms)
+
+
+def p1_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc=None, alpha=0.5):
+    """chive's "p1sa" operator, the 2x2 block system of P1Discretization::GetOperator
+    (src/moment_discretization.cpp:400-490) as a CSR matrix, unknowns per element
+    [J_0 (n), ..., J_{dim-1} (n), phi (n)] (element e, component c, node i at
+    (e (dim+1) + c) n + i):
+      Mt = VectorMass(3 sigma_t) + DGVectorJumpJump(1/(2 alpha)) on interior and
+           inflow (Marshak) sides, 1/alpha on reflective sides (src/moment_integrators.cpp:495-580);
+      Ma = Mass(sigma_a) + Penalty(alpha/2, unscaled) on interior faces (:206-287)
+           + BoundaryMass(alpha/2) on Marshak sides;
+      D  = -(grad psi, J) + DGJumpAverage [psi]{J.n} on interior faces and 0.5 psi J.n
+           on Marshak sides (:289-388);  G = -D^T.
+    alpha is P1Discretization's 0.5 (chive does not set it for p1sa,
+    exe/chive.cpp:608-619).  Volume rule order 2p + dim - 1, face rule 2p."""
+    geo = _Geometry(dim, nel, vertices)
+    sigma_t = np.asarray(sigma_t, float).reshape(-1)
+    sigma_a = sigma_t - np.asarray(sigma_s, float).reshape(-1)
+    bc = list(bc) if bc is not None else [0] * (2 * dim)
+    n = (p + 1) ** dim
+    N = (dim + 1) * n
+    rows, cols, vals = [], [], []
+
+    def add(r, c, v):
+        rows.append(np.repeat(r, len(c)))
+        cols.append(np.tile(c, len(r)))
+        vals.append(v.ravel())
+
+    def idx(e, c):
+        return np.arange(n) + (e * (dim + 1) + c) * n
+
+    xv, wv = _rule(dim, 2 * p + dim - 1)
+    phi_v, g_v = _basis(dim, p, xv)
+    for e in range(geo.Ne):
+        J = geo.jac(e, xv)
+        det = np.linalg.det(J)
+        grad = np.einsum("qnr,qrd->qnd", g_v, np.linalg.inv(J))
+        w = wv * np.abs(det)
+        M = np.einsum("q,qi,qj->ij", w, phi_v, phi_v)
+        for c in range(dim):
+            add(idx(e, c), idx(e, c), 3.0 * sigma_t[e] * M)
+        add(idx(e, dim), idx(e, dim), sigma_a[e] * M)
+        for c in range(dim):
+            Dv = -np.einsum("q,qi,qj->ij", w, grad[:, :, c], phi_v)     # row psi_i, column J_c,j
+            add(idx(e, dim), idx(e, c), Dv)
+            add(idx(e, c), idx(e, dim), -Dv.T)
+    uf, wf = _rule(dim - 1, 2 * p)
+    stride = [1, geo.nel[0], geo.nel[0] * (geo.nel[1] if dim > 1 else 1)]
+    for e1 in range(geo.Ne):
+        cc = geo.coords(e1)
+        for ax in range(dim):
+            def side_pts(hi):
+                xi = np.zeros((len(wf), dim))
+                others = [d for d in range(dim) if d != ax]
+                for k, d in enumerate(others):
+                    xi[:, d] = uf[:, k]
+                xi[:, ax] = hi
+                return xi
+            for hi in (0, 1):
+                at_bdr = (cc[ax] == 0 and hi == 0) or (cc[ax] == geo.nel[ax] - 1 and hi == 1)
+                if at_bdr:
+                    xi = side_pts(hi)
+                    Nn = _face_normal(geo.jac(e1, xi), ax, hi)
+                    area = np.linalg.norm(Nn, axis=1)
+                    nh = Nn / area[:, None]
+                    ph, _ = _basis(dim, p, xi)
+                    beta = 1.0 / alpha if bc[2 * ax + hi] != 0 else 1.0 / (2 * alpha)
+                    for c in range(dim):
+                        for d in range(dim):
+                            add(idx(e1, c), idx(e1, d),
+                                beta * np.einsum("q,qi,qj->ij", wf * area * nh[:, c] * nh[:, d], ph, ph))
+                    if bc[2 * ax + hi] == 0:
+                        add(idx(e1, dim), idx(e1, dim), 0.5 * alpha * np.einsum("q,qi,qj->ij", wf * area, ph, ph))
+                        for d in range(dim):
+                            Db = 0.5 * np.einsum("q,qi,qj->ij", wf * Nn[:, d], ph, ph)
+                            add(idx(e1, dim), idx(e1, d), Db)
+                            add(idx(e1, d), idx(e1, dim), -Db.T)
+                    continue
+                if hi == 0:
+                    continue
+                e2 = e1 + stride[ax]
+                xi1, xi2 = side_pts(1), side_pts(0)
+                Nn = _face_normal(geo.jac(e1, xi1), ax, 1)
+                area = np.linalg.norm(Nn, axis=1)
+                nh = Nn / area[:, None]
+                ph1, _ = _basis(dim, p, xi1)
+                ph2, _ = _basis(dim, p, xi2)
+                jmp = [(e1, ph1, 1.0), (e2, ph2, -1.0)]
+                for ea, pa, sa in jmp:
+                    for eb, pb, sb in jmp:
+                        base = sa * sb * np.einsum("q,qi,qj->ij", wf * area, pa, pb)
+                        add(idx(ea, dim), idx(eb, dim), 0.5 * alpha * base)
+                        for c in range(dim):
+                            for d in range(dim):
+                                add(idx(ea, c), idx(eb, d), (0.5 / alpha) * sa * sb *
+                                    np.einsum("q,qi,qj->ij", wf * area * nh[:, c] * nh[:, d], pa, pb))
+                        # [psi]{J.n}: test side sign sa, trial average 1/2
+                        for d in range(dim):
+                            Df = 0.5 * sa * np.einsum("q,qi,qj->ij", wf * Nn[:, d], pa, pb)
+                            add(idx(ea, dim), idx(eb, d), Df)
+                            add(idx(eb, d), idx(ea, dim), -Df.T)
+    Ntot = geo.Ne * N
+    P = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(Ntot, Ntot))
+    return P.tocsr()
+
+
+def p1_solver(P, dim, n):
+    """phi-part of P^{-1} [0; b] (the TripleProductOperator of
+    ComponentExtractionOperator, exe/chive.cpp:612-617), P solved directly."""
+    solve = spla.factorized(sp.csc_matrix(P))
+    N = (dim + 1) * n
+
+    def apply(b):
+        Ne = b.size // n
+        full = np.zeros(Ne * N)
+        full.reshape(Ne, dim + 1, n)[:, dim, :] = b.reshape(Ne, n)
+        return solve(full).reshape(Ne, dim + 1, n)[:, dim, :].ravel()
+    return apply

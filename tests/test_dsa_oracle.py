@@ -175,3 +175,111 @@ def test_tdl_deck_converges_in_under_12_iterations():
     b = pr.D(pr.sweep(q))
     r = b - phi + pr.D(pr.sweep(pr.DT(pr.mass(ss, phi))))
     assert np.linalg.norm(r) < 1e-5 * np.linalg.norm(phi)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_p1_blocks_against_closed_forms(dim):
+    """chive's "p1sa" operator (oracle p1_matrix) on a uniform mesh with one
+    reflective side, p = 1 (node integrals: vol / 2^dim, side area / 2^(dim-1)):
+      D c      = -sum_{all sides} int psi c.n + 1/2 sum_{Marshak sides} int psi c.n
+                 for a constant current c (integration by parts; interior jumps vanish);
+      Ma 1     = sigma_a int psi + alpha/2 int_{Marshak} psi;
+      Mt c     = 3 sigma_t int psi c + beta int_side psi n (c.n), beta = 1/(2 alpha) on
+                 Marshak and 1/alpha on reflective sides;
+      G        = -D^T."""
+    n_el = 3
+    nel = (n_el,) * dim
+    vert = P.cartesian_vertices(nel)
+    rng = np.random.default_rng(4)
+    ne = n_el ** dim
+    st = rng.uniform(0.5, 2.0, ne)
+    ss = st * rng.uniform(0.1, 0.9, ne)
+    bc = [0, 1] + [0] * (2 * dim - 2)        # x-high side reflective
+    alpha = 0.5
+    Pm = M.p1_matrix(dim, nel, vert, 1, st, ss, bc=bc, alpha=alpha).toarray()
+    n, N = 2 ** dim, (dim + 1) * 2 ** dim
+    h = 1.0 / n_el
+    vol, area = h ** dim / n, h ** (dim - 1) / 2 ** (dim - 1)
+
+    def blk(e, c):
+        return slice((e * (dim + 1) + c) * n, (e * (dim + 1) + c + 1) * n)
+
+    def side_nodes(e):
+        """(axis, hi, node mask) of each boundary side the element touches"""
+        co = [(e // n_el ** d) % n_el for d in range(dim)]
+        out = []
+        for ax in range(dim):
+            for hi in (0, 1):
+                if co[ax] == (n_el - 1) * hi:
+                    out.append((ax, hi, np.array([((i >> ax) & 1) == hi for i in range(n)])))
+        return out
+
+    c = rng.normal(size=dim)
+    x = np.zeros(ne * N)
+    for e in range(ne):
+        for d in range(dim):
+            x[blk(e, d)] = c[d]
+    y = Pm @ x
+    for e in range(ne):
+        Dc = np.zeros(n)
+        Mtc = np.array([3 * st[e] * vol * c[d] * np.ones(n) for d in range(dim)])
+        for ax, hi, mask in side_nodes(e):
+            cn = c[ax] * (1 if hi else -1)
+            marshak = bc[2 * ax + hi] == 0
+            Dc[mask] += (-1.0 + (0.5 if marshak else 0.0)) * area * cn
+            beta = 1 / (2 * alpha) if marshak else 1 / alpha
+            Mtc[ax][mask] += beta * area * c[ax]
+        np.testing.assert_allclose(y[blk(e, dim)], Dc, atol=1e-14)
+        for d in range(dim):
+            np.testing.assert_allclose(y[blk(e, d)], Mtc[d], atol=1e-14)
+    # Ma on constants (the phi block only)
+    x = np.zeros(ne * N)
+    for e in range(ne):
+        x[blk(e, dim)] = 1.0
+    y = Pm @ x
+    for e in range(ne):
+        expect = (st[e] - ss[e]) * vol * np.ones(n)
+        for ax, hi, mask in side_nodes(e):
+            if bc[2 * ax + hi] == 0:
+                expect[mask] += 0.5 * alpha * area
+        phi_rows = y[blk(e, dim)]
+        # the G columns act on phi: D^T's contribution shows in the current rows, not here
+        np.testing.assert_allclose(phi_rows, expect, atol=1e-14)
+    # G = -D^T
+    jdx = np.concatenate([np.arange(blk(e, 0).start, blk(e, dim).start) for e in range(ne)])
+    pdx = np.concatenate([np.arange(blk(e, dim).start, blk(e, dim).stop) for e in range(ne)])
+    np.testing.assert_allclose(Pm[np.ix_(jdx, pdx)], -Pm[np.ix_(pdx, jdx)].T, atol=1e-16)
+
+
+def test_p1sa_accelerates_the_tdl_deck():
+    """SLISolver with the P1SA preconditioner (direct inner solve, as chive
+    requires for p1sa, exe/chive.cpp:610) on test/chive/tdl.lua: converged, the
+    transport fixed point, and far fewer iterations than plain SI's 50."""
+    nel, vert, om, w, st, ss, eps = _tdl()
+    pr = oracle.Problem(2, nel, vert, 1, om, w, st)
+    q = pr.form_source(Q=eps)
+    Pm = M.p1_matrix(2, nel, vert, 1, st, ss)
+    phi, it, norms = M.dsa_source_iteration(pr, ss, q, None, abstol=1e-6, maxit=50,
+                                            solve=M.p1_solver(Pm, 2, 4))
+    assert norms[-1] < 1e-6 and it < 12
+    b = pr.D(pr.sweep(q))
+    r = b - phi + pr.D(pr.sweep(pr.DT(pr.mass(ss, phi))))
+    assert np.linalg.norm(r) < 1e-5 * np.linalg.norm(phi)
+
+
+@pytest.mark.parametrize("dim,p,jitter", [(2, 1, 0.0), (2, 1, 0.2), (2, 2, 0.1), (3, 1, 0.0), (3, 1, 0.15)])
+def test_host_p1_assembly_matches_oracle(dim, p, jitter):
+    """allium_host_p1_matrix (C++) against the numpy restatement, with one
+    reflective side."""
+    n = 4 if dim == 2 else 3
+    nel = (n,) * dim
+    rng = np.random.default_rng(11)
+    vert = P.jittered_vertices(nel, jitter, rng) if jitter else P.cartesian_vertices(nel)
+    st = P.lognormal_sigma(n ** dim, 1.0, rng)
+    ss = st * rng.uniform(0.5, 0.999, n ** dim)
+    bc = [0, 1] + [0] * (2 * dim - 2)
+    mesh = A.Mesh(nel, vertices=vert, bc=bc)
+    rp, col, val = A.host_p1_matrix(mesh, p, st, ss)
+    Ph = sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2).toarray()
+    Po = M.p1_matrix(dim, nel, vert, p, st, ss, bc=bc).toarray()
+    assert np.abs(Ph - Po).max() < 1e-12 * np.abs(Po).max()
