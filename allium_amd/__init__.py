@@ -393,10 +393,13 @@ class DiscreteToMoment:
         return psi
 
 
+DSA_METHODS = {"mip": 0, "p1sa": 1}
+
+
 def _dsa_desc(kappa=-1.0, bound_below=True, alpha=0.0, sym=-1.0, cg_reltol=1e-6, cg_abstol=0.0, cg_maxit=100,
-              smooth=2):
+              smooth=2, method="mip"):
     return _lib.DSADesc(float(kappa), int(bool(bound_below)), float(alpha), float(sym), float(cg_reltol),
-                        float(cg_abstol), int(cg_maxit), int(smooth))
+                        float(cg_abstol), int(cg_maxit), int(smooth), DSA_METHODS[method])
 
 
 def host_mip_matrix(mesh, fe_order, quad, sigma_t, sigma_s, **desc):
@@ -440,14 +443,17 @@ def host_p1_matrix(mesh, fe_order, sigma_t, sigma_s):
 
 
 class DSA:
-    """Diffusion synthetic acceleration with the scalar MIP operator on the GPU
-    (DiffusionSyntheticAccelerationOperator, src/transport_op.hpp:38-59, as
-    chive's "scalar mip" preconditioner, exe/chive.cpp:663-682).
+    """Diffusion synthetic acceleration on the GPU
+    (DiffusionSyntheticAccelerationOperator, src/transport_op.hpp:38-59) with
+    chive's "scalar mip" operator (method "mip", exe/chive.cpp:663-682) or its
+    "p1sa" one (method "p1sa", exe/chive.cpp:608-619).
 
     ``Mult(x, y)``: y = x + K^{-1} Ms x on device vectors; ``solve(b)``: K^{-1} b
-    for host arrays.  Options: kappa (< 0: |kappa| (p+1)^2), bound_below,
-    alpha (<= 0: from the quadrature), sym, cg_reltol, cg_abstol, cg_maxit,
-    smooth (damped Jacobi sweeps per multigrid level and direction)."""
+    for host arrays (for "p1sa": the phi-part of P^{-1} [0; b]).  Options of
+    "mip": kappa (< 0: |kappa| (p+1)^2), bound_below, alpha (<= 0: from the
+    quadrature), sym, smooth (damped Jacobi sweeps per multigrid level and
+    direction); cg_reltol, cg_abstol, cg_maxit are the CG ("mip") or GMRES
+    ("p1sa") stopping rule."""
 
     def __init__(self, Linv, sigma_s, **desc):
         self.Linv = Linv

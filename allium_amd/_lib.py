@@ -49,7 +49,7 @@ class KrylovOptions(ctypes.Structure):
 class DSADesc(ctypes.Structure):
     _fields_ = [("kappa", ctypes.c_double), ("bound_below", ctypes.c_int), ("alpha", ctypes.c_double),
                 ("sym", ctypes.c_double), ("cg_reltol", ctypes.c_double), ("cg_abstol", ctypes.c_double),
-                ("cg_maxit", ctypes.c_int), ("smooth", ctypes.c_int)]
+                ("cg_maxit", ctypes.c_int), ("smooth", ctypes.c_int), ("method", ctypes.c_int)]
 
 
 class DSAInfo(ctypes.Structure):

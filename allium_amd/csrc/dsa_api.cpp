@@ -15,6 +15,7 @@
 #include "cnn_internal.hpp"
 #include "dsa.hpp"
 #include "dsa_internal.hpp"
+#include "krylov.hpp"
 #include "sweep_handle.hpp"
 
 namespace {
@@ -342,6 +343,13 @@ struct allium_dsa {
     double *hsc = nullptr;                    // pinned host copy of sc
     int npartial = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // P1SA: dg holds the 2x2 P1 blocks ((dim + 1) n unknowns per element) and
+    // the P1 system is solved by GMRES in the vectors below
+    int method = ALLIUM_DSA_MIP;
+    long long pn = 0;                         // P1 unknowns
+    double *pb = nullptr, *px = nullptr, *pw = nullptr, *pt = nullptr, *pV = nullptr;
+    double *ppart = nullptr, *pH = nullptr;
+    int pnb = 0;
     std::vector<void *> owned;
     allium_dsa_info info{};
     ~allium_dsa()
@@ -460,6 +468,178 @@ int pcg(allium_dsa *d, const double *b, double *x, hipStream_t s)
     return ALLIUM_OK;
 }
 
+// GMRES restart length of the P1 solve
+constexpr int kP1Kdim = 40;
+constexpr int kP1Chunk = 64;
+
+// DG blocks go to the device column-major
+void to_colmajor(int n, const std::vector<double> &src, std::vector<double> &dst)
+{
+    dst.resize(src.size());
+    const size_t nb = src.size() / ((size_t)n * n);
+    allium::parallel_for((long)nb, [&](long b, long bend) {
+        for (long k = b; k < bend; k++)
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < n; j++) dst[(size_t)k * n * n + (size_t)j * n + i] = src[(size_t)k * n * n + (size_t)i * n + j];
+    });
+}
+
+// the P1 level, its GMRES vectors and the buffers every method shares; takes
+// ownership of d
+int create_p1(allium_dsa *raw, const allium::MipMatrix &K, const std::vector<double> &Dinv, const double *sigma_s,
+              allium_dsa_t *out)
+{
+    std::unique_ptr<allium_dsa> d(raw);
+    const allium::Mesh &m = d->h->mesh;
+    const int n = K.n, Ne = m.Ne, mk = kP1Kdim;
+    {
+        std::vector<double> cm;
+        double *dp;
+        int *ip;
+        to_colmajor(n, K.Kd, cm);
+        HIP_TRY(upload(d->owned, &dp, cm.data(), cm.size()));
+        d->dg.Kd = dp;
+        to_colmajor(n, K.Ko, cm);
+        HIP_TRY(upload(d->owned, &dp, cm.data(), cm.size()));
+        d->dg.Ko = dp;
+        to_colmajor(n, Dinv, cm);
+        HIP_TRY(upload(d->owned, &dp, cm.data(), cm.size()));
+        d->dg.Dinv = dp;
+        HIP_TRY(upload(d->owned, &ip, K.nb.data(), K.nb.size()));
+        d->dg.nb = ip;
+    }
+    d->dg.dim = m.dim;
+    d->dg.n = n;
+    d->dg.ns = K.nside;
+    d->dg.nc = 0;
+    d->dg.P = nullptr;
+    d->dg.Ne = Ne;
+    for (int k = 0; k < 3; k++) d->dg.nel[k] = m.nel[k];
+    d->dg.omega = 1.0;
+    d->pn = (long long)Ne * n;
+    d->info.levels = 1;
+    d->info.level_size[0] = d->pn;
+    d->info.omega[0] = 1.0;
+    d->info.dense_coarsest = 0;
+    const long long np = d->h->phi_size;
+    HIP_TRY(upload(d->owned, &d->sigs, sigma_s, (size_t)Ne));
+    for (double **q : {&d->bv, &d->xv}) HIP_TRY(upload<double>(d->owned, q, nullptr, (size_t)np));
+    for (double **q : {&d->pb, &d->px, &d->pw, &d->pt}) HIP_TRY(upload<double>(d->owned, q, nullptr, (size_t)d->pn));
+    HIP_TRY(upload<double>(d->owned, &d->pV, nullptr, (size_t)(mk + 1) * d->pn));
+    d->pnb = allium::krylov_blocks(d->pn);
+    HIP_TRY(upload<double>(d->owned, &d->ppart, nullptr, (size_t)2 * d->pnb));
+    HIP_TRY(upload<double>(d->owned, &d->pH, nullptr, (size_t)(mk + 1) * (mk + 1) + mk + 1));
+    d->npartial = std::max(allium::dot_blocks(np), allium::dg_blocks(d->dg));
+    HIP_TRY(upload<double>(d->owned, &d->partial, nullptr, (size_t)d->npartial));
+    HIP_TRY(upload<double>(d->owned, &d->sc, nullptr, 8));
+    HIP_TRY(hipMemset(d->sc, 0, sizeof(double) * 8));
+    HIP_TRY(hipHostMalloc((void **)&d->hsc, sizeof(double) * 8, hipHostMallocDefault));
+    HIP_TRY(hipEventCreate(&d->ev0));
+    HIP_TRY(hipEventCreate(&d->ev1));
+    *out = d.release();
+    return ALLIUM_OK;
+}
+
+// phi-part of P^{-1} [0; b] (exe/chive.cpp:608-617: the TripleProductOperator
+// of ComponentExtractionOperator around the P1 solve).  The reference solves P
+// with a direct solver; here GMRES(kP1Kdim), preconditioned from the right by
+// the inverse diagonal element blocks, from x = 0 until the true residual
+// ||[0; b] - P x|| <= max(cg_reltol ||b||, cg_abstol) or cg_maxit steps.
+int p1_solve(allium_dsa *d, const double *b, double *x, hipStream_t s)
+{
+    const allium::DgLevel &L = d->dg;
+    const long long np = d->pn;
+    const int m = kP1Kdim, nb = d->pnb, n = L.n / (L.dim + 1), Ne = L.Ne;
+    const size_t rowb = sizeof(double) * n, pitch = sizeof(double) * L.n;
+    double *part[2] = {d->ppart, d->ppart + nb};
+    double *Hd = d->pH, *yd = d->pH + (size_t)(m + 1) * (m + 1);
+    auto V = [&](int i) { return d->pV + (size_t)i * np; };
+    // pb = [0; b] element by element
+    HIP_TRY(hipMemsetAsync(d->pb, 0, sizeof(double) * np, s));
+    HIP_TRY(hipMemcpy2DAsync(d->pb + (size_t)L.dim * n, pitch, b, rowb, rowb, Ne, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemsetAsync(d->px, 0, sizeof(double) * np, s));
+    HIP_TRY(hipMemcpyAsync(d->pw, d->pb, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    double beta = 0.0;
+    auto start_cycle = [&]() -> int {
+        HIP_TRY(allium::launch_mgs(np, nullptr, nb, nullptr, nullptr, nullptr, d->pw, part[0], s));
+        HIP_TRY(allium::launch_normalize(np, part[0], nb, Hd + (size_t)m * (m + 1), d->pw, V(0), s));
+        HIP_TRY(hipMemcpyAsync(d->hsc, Hd + (size_t)m * (m + 1), sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        beta = d->hsc[0];
+        return std::isfinite(beta) ? ALLIUM_OK : fail(ALLIUM_ERR_NUMERIC, "P1SA: non-finite residual");
+    };
+    std::vector<double> Hh((size_t)(m + 1) * m, 0.0), sv(m + 1), cs(m), sn(m), y;
+    // px += Dinv (sum_{j<k} y_j V_j)
+    auto update = [&](int k) -> int {
+        if (k < 1) return ALLIUM_OK;
+        y.assign(sv.begin(), sv.begin() + k);
+        for (int i = k - 1; i >= 0; i--) {
+            y[i] /= Hh[(size_t)i * (m + 1) + i];
+            for (int j = i - 1; j >= 0; j--) y[j] -= Hh[(size_t)i * (m + 1) + j] * y[i];
+        }
+        HIP_TRY(hipMemcpyAsync(yd, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemsetAsync(d->pt, 0, sizeof(double) * np, s));
+        for (int j0 = 0; j0 < k; j0 += kP1Chunk)
+            HIP_TRY(allium::launch_basis_update(np, std::min(kP1Chunk, k - j0), V(j0), np, yd + j0, d->pt, s));
+        HIP_TRY(allium::dg_jacobi(L, nullptr, d->pt, d->pw, s));
+        HIP_TRY(allium::launch_axpby(np, 1.0, d->px, 1.0, d->pw, d->px, s));
+        // y lives in pageable host memory
+        HIP_TRY(hipStreamSynchronize(s));
+        return ALLIUM_OK;
+    };
+    int rc = start_cycle();
+    if (rc) return rc;
+    const double stop = std::max(d->desc.cg_reltol * beta, d->desc.cg_abstol);
+    int its = 0;
+    double res = beta;
+    bool done = beta <= stop;
+    while (!done && its < d->desc.cg_maxit) {
+        std::fill(sv.begin(), sv.end(), 0.0);
+        sv[0] = beta;
+        int i = 0;
+        for (; i < m && its < d->desc.cg_maxit; i++) {
+            its++;
+            // w = P Dinv v_i
+            HIP_TRY(allium::dg_jacobi(L, nullptr, V(i), d->pt, s));
+            HIP_TRY(allium::dg_spmv_dot(L, d->pt, d->pw, d->partial, s));
+            double *Hcol = Hd + (size_t)i * (m + 1);
+            for (int k = 0; k <= i + 1; k++)
+                HIP_TRY(allium::launch_mgs(np, part[(k + 1) & 1], nb, k > 0 ? Hcol + (k - 1) : nullptr,
+                                           k > 0 ? V(k - 1) : nullptr, k <= i ? V(k) : nullptr, d->pw,
+                                           part[k & 1], s));
+            HIP_TRY(allium::launch_normalize(np, part[(i + 1) & 1], nb, Hcol + i + 1, d->pw, V(i + 1), s));
+            double *hc = Hh.data() + (size_t)i * (m + 1);
+            HIP_TRY(hipMemcpyAsync(hc, Hcol, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            for (int k = 0; k < i; k++) allium::apply_givens(hc[k], hc[k + 1], cs[k], sn[k]);
+            allium::givens(hc[i], hc[i + 1], cs[i], sn[i]);
+            allium::apply_givens(hc[i], hc[i + 1], cs[i], sn[i]);
+            allium::apply_givens(sv[i], sv[i + 1], cs[i], sn[i]);
+            res = std::fabs(sv[i + 1]);
+            if (!std::isfinite(res)) return fail(ALLIUM_ERR_NUMERIC, "P1SA: GMRES breakdown");
+            if (res <= stop) { i++; done = true; break; }
+        }
+        if ((rc = update(i))) return rc;
+        if (done || its >= d->desc.cg_maxit) break;
+        // restart from the true residual
+        HIP_TRY(allium::dg_resid(L, d->px, d->pb, d->pw, s));
+        if ((rc = start_cycle())) return rc;
+        res = beta;
+        done = beta <= stop;
+    }
+    d->info.last_cg_iterations = its;
+    d->info.last_cg_norm = res;
+    d->info.cg_iterations += its;
+    HIP_TRY(hipMemcpy2DAsync(x, rowb, d->px + (size_t)L.dim * n, pitch, rowb, Ne, hipMemcpyDeviceToDevice, s));
+    return ALLIUM_OK;
+}
+
+// x = K^{-1} b (MIP) or the phi-part of P^{-1} [0; b] (P1SA)
+int inner_solve(allium_dsa *d, const double *b, double *x, hipStream_t s)
+{
+    return d->method == ALLIUM_DSA_P1 ? p1_solve(d, b, x, s) : pcg(d, b, x, s);
+}
+
 }  // namespace
 
 namespace allium_dsa_internal {
@@ -470,7 +650,7 @@ int correct(allium_dsa *d, const double *d_r, double *d_x, hipStream_t s, double
     const long long N = h->phi_size;
     HIP_TRY(hipEventRecord(d->ev0, s));
     HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, d->sigs, 1.0, d_r, d->bv, s));
-    int rc = pcg(d, d->bv, d->xv, s);
+    int rc = inner_solve(d, d->bv, d->xv, s);
     if (rc) return rc;
     HIP_TRY(allium::dsa_finish(N, d_r, d->xv, d_x, d->partial, s));
     HIP_TRY(allium::reduce_to(d->partial, allium::dot_blocks(N), d->sc, 4, s));
@@ -504,6 +684,7 @@ void allium_dsa_default_desc(allium_dsa_desc *desc)
     desc->cg_abstol = 0.0;
     desc->cg_maxit = 100;
     desc->smooth = 2;
+    desc->method = ALLIUM_DSA_MIP;
 }
 
 int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_desc *desc, allium_dsa_t *out)
@@ -518,6 +699,10 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
     d->h = h;
     if (desc) d->desc = *desc; else allium_dsa_default_desc(&d->desc);
     if (d->desc.cg_maxit < 1 || d->desc.smooth < 1) return fail(ALLIUM_ERR_ARG, "cg_maxit and smooth must be >= 1");
+    if (d->desc.method != ALLIUM_DSA_MIP && d->desc.method != ALLIUM_DSA_P1)
+        return fail(ALLIUM_ERR_ARG, "method must be ALLIUM_DSA_MIP or ALLIUM_DSA_P1");
+    d->method = d->desc.method;
+    const bool p1 = d->method == ALLIUM_DSA_P1;
     const allium::Mesh &m = h->mesh;
     const int Ne = m.Ne, p = h->ed.p;
     std::vector<double> siga(Ne);
@@ -527,7 +712,8 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
     }
     allium::MipMatrix K;
     std::string why;
-    if (!allium::assemble_mip(m, p, h->sigma.data(), siga.data(), mip_params(d->desc, p, h->quad), K, why))
+    if (p1 ? !allium::assemble_p1(m, p, h->sigma.data(), siga.data(), kP1Alpha, K, why)
+           : !allium::assemble_mip(m, p, h->sigma.data(), siga.data(), mip_params(d->desc, p, h->quad), K, why))
         return fail(ALLIUM_ERR_ARG, why);
     const int n = K.n, ns = K.nside;
     // block-Jacobi inverses and the smoother damping of the DG level
@@ -537,8 +723,9 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
         for (long e = b; e < eend; e++)
             if (!invert(n, &Dinv[(size_t)e * n * n])) ok = false;
     });
-    if (!ok) return fail(ALLIUM_ERR_NUMERIC, "DSA: singular diagonal block of the MIP matrix");
+    if (!ok) return fail(ALLIUM_ERR_NUMERIC, "DSA: singular diagonal block of the diffusion matrix");
     const long long N0 = (long long)Ne * n;
+    if (p1) return create_p1(d.release(), K, Dinv, sigma_s, out);
     std::vector<double> tmp(N0);
     const double lam0 = power_max(N0, [&](const double *v, double *u) {
         host_mv(K, Ne, v, tmp.data());
@@ -586,15 +773,7 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
         if (!invert(Nc, Ainv.data())) return fail(ALLIUM_ERR_NUMERIC, "DSA: singular coarsest operator");
     }
     // upload: DG blocks column-major
-    auto colmajor = [&](const std::vector<double> &src, std::vector<double> &dst) {
-        dst.resize(src.size());
-        const size_t nb = src.size() / ((size_t)n * n);
-        allium::parallel_for((long)nb, [&](long b, long bend) {
-            for (long k = b; k < bend; k++)
-                for (int i = 0; i < n; i++)
-                    for (int j = 0; j < n; j++) dst[(size_t)k * n * n + (size_t)j * n + i] = src[(size_t)k * n * n + (size_t)i * n + j];
-        });
-    };
+    auto colmajor = [&](const std::vector<double> &src, std::vector<double> &dst) { to_colmajor(n, src, dst); };
     {
         std::vector<double> cm;
         double *dp;
@@ -665,7 +844,7 @@ int allium_dsa_mult_device(allium_dsa_t d, const double *d_x, double *d_y, void 
     hipStream_t s = (hipStream_t)stream;
     allium_sweep *h = d->h;
     HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, d->sigs, 1.0, d_x, d->bv, s));
-    int rc = pcg(d, d->bv, d->xv, s);
+    int rc = inner_solve(d, d->bv, d->xv, s);
     if (rc) return rc;
     HIP_TRY(allium::launch_axpby(h->phi_size, 1.0, d_x, 1.0, d->xv, d_y, s));
     d->info.applications++;
@@ -675,7 +854,7 @@ int allium_dsa_mult_device(allium_dsa_t d, const double *d_x, double *d_y, void 
 int allium_dsa_solve_device(allium_dsa_t d, const double *d_b, double *d_x, void *stream)
 {
     if (!d || !d_b || !d_x) return fail(ALLIUM_ERR_ARG, "null argument");
-    return pcg(d, d_b, d_x, (hipStream_t)stream);
+    return inner_solve(d, d_b, d_x, (hipStream_t)stream);
 }
 
 int allium_dsa_solve(allium_dsa_t d, const double *b, double *x)
@@ -684,7 +863,7 @@ int allium_dsa_solve(allium_dsa_t d, const double *b, double *x)
     hipStream_t s = d->h->stream;
     const size_t bytes = sizeof(double) * d->h->phi_size;
     HIP_TRY(hipMemcpyAsync(d->bv, b, bytes, hipMemcpyHostToDevice, s));
-    int rc = pcg(d, d->bv, d->xv, s);
+    int rc = inner_solve(d, d->bv, d->xv, s);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(x, d->xv, bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -42,7 +42,14 @@ typedef struct {
     double cg_abstol;
     int cg_maxit;
     int smooth;         /* damped Jacobi sweeps before and after each coarse correction */
+    int method;         /* ALLIUM_DSA_MIP, or ALLIUM_DSA_P1: chive's "p1sa" (exe/chive.cpp:608-619),
+                           the 2x2 P1 system of allium_host_p1_matrix solved by GMRES(40) with
+                           element block-Jacobi from the right, to ||r|| <= max(cg_reltol ||b||,
+                           cg_abstol) in at most cg_maxit steps; kappa, alpha, sym, smooth unused */
 } allium_dsa_desc;
+
+#define ALLIUM_DSA_MIP 0
+#define ALLIUM_DSA_P1 1
 
 typedef struct {
     int levels;                 /* DG level + vertex-grid levels */
@@ -57,7 +64,7 @@ typedef struct {
 } allium_dsa_info;
 
 /* defaults: kappa -1 ((p+1)^2), bound_below 1, alpha 0 (ComputeAlpha), sym -1,
- * cg_reltol 1e-6, cg_abstol 0, cg_maxit 100, smooth 2 */
+ * cg_reltol 1e-6, cg_abstol 0, cg_maxit 100, smooth 2, method ALLIUM_DSA_MIP */
 void allium_dsa_default_desc(allium_dsa_desc *desc);
 
 /* Build K on the mesh, quadrature and sigma_t of the sweep handle (G == 1)
