@@ -343,12 +343,15 @@ struct allium_dsa {
     double *hsc = nullptr;                    // pinned host copy of sc
     int npartial = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // P1SA: dg holds the 2x2 P1 blocks ((dim + 1) n unknowns per element) and
-    // the P1 system is solved by GMRES in the vectors below
+    // P1SA: p1 holds the 2x2 P1 blocks ((dim + 1) n unknowns per element); the
+    // P1 system is solved by GMRES in the vectors below, preconditioned with
+    // the help of the MIP hierarchy above
     int method = ALLIUM_DSA_MIP;
     long long pn = 0;                         // P1 unknowns
-    double *pb = nullptr, *px = nullptr, *pw = nullptr, *pt = nullptr, *pV = nullptr;
-    double *ppart = nullptr, *pH = nullptr;
+    allium::DgLevel p1{};
+    double *pb = nullptr, *px = nullptr, *pw = nullptr, *pt = nullptr, *pr = nullptr, *pz = nullptr;
+    double *pe = nullptr, *pV = nullptr;
+    double *ppart = nullptr, *pH = nullptr, *ppart_spmv = nullptr;
     int pnb = 0;
     std::vector<void *> owned;
     allium_dsa_info info{};
@@ -484,70 +487,99 @@ void to_colmajor(int n, const std::vector<double> &src, std::vector<double> &dst
     });
 }
 
-// the P1 level, its GMRES vectors and the buffers every method shares; takes
-// ownership of d
-int create_p1(allium_dsa *raw, const allium::MipMatrix &K, const std::vector<double> &Dinv, const double *sigma_s,
-              allium_dsa_t *out)
+// P1SA: the P1 level and its GMRES vectors, added to a complete MIP object
+// (whose V-cycle preconditions the P1 solve)
+int add_p1(allium_dsa *d, const double *siga)
 {
-    std::unique_ptr<allium_dsa> d(raw);
     const allium::Mesh &m = d->h->mesh;
-    const int n = K.n, Ne = m.Ne, mk = kP1Kdim;
+    const int Ne = m.Ne, mk = kP1Kdim;
+    allium::MipMatrix K;
+    std::string why;
+    if (!allium::assemble_p1(m, d->h->ed.p, d->h->sigma.data(), siga, kP1Alpha, K, why)) return fail(ALLIUM_ERR_ARG, why);
+    const int n = K.n;
+    std::vector<double> Dinv(K.Kd);
+    std::atomic<bool> ok{true};
+    allium::parallel_for(Ne, [&](long b, long eend) {
+        for (long e = b; e < eend; e++)
+            if (!invert(n, &Dinv[(size_t)e * n * n])) ok = false;
+    });
+    if (!ok) return fail(ALLIUM_ERR_NUMERIC, "DSA: singular diagonal block of the P1 matrix");
     {
         std::vector<double> cm;
         double *dp;
         int *ip;
         to_colmajor(n, K.Kd, cm);
         HIP_TRY(upload(d->owned, &dp, cm.data(), cm.size()));
-        d->dg.Kd = dp;
+        d->p1.Kd = dp;
         to_colmajor(n, K.Ko, cm);
         HIP_TRY(upload(d->owned, &dp, cm.data(), cm.size()));
-        d->dg.Ko = dp;
+        d->p1.Ko = dp;
         to_colmajor(n, Dinv, cm);
         HIP_TRY(upload(d->owned, &dp, cm.data(), cm.size()));
-        d->dg.Dinv = dp;
+        d->p1.Dinv = dp;
         HIP_TRY(upload(d->owned, &ip, K.nb.data(), K.nb.size()));
-        d->dg.nb = ip;
+        d->p1.nb = ip;
     }
-    d->dg.dim = m.dim;
-    d->dg.n = n;
-    d->dg.ns = K.nside;
-    d->dg.nc = 0;
-    d->dg.P = nullptr;
-    d->dg.Ne = Ne;
-    for (int k = 0; k < 3; k++) d->dg.nel[k] = m.nel[k];
-    d->dg.omega = 1.0;
+    d->p1.dim = m.dim;
+    d->p1.n = n;
+    d->p1.ns = K.nside;
+    d->p1.nc = 0;
+    d->p1.P = nullptr;
+    d->p1.Ne = Ne;
+    for (int k = 0; k < 3; k++) d->p1.nel[k] = m.nel[k];
+    d->p1.omega = 1.0;
     d->pn = (long long)Ne * n;
-    d->info.levels = 1;
+    // the P1 level goes first in the info, the MIP levels follow
+    if (d->info.levels >= 16) return fail(ALLIUM_ERR_UNSUPPORTED, "DSA: too many levels");
+    for (int l = d->info.levels; l > 0; l--) {
+        d->info.level_size[l] = d->info.level_size[l - 1];
+        d->info.omega[l] = d->info.omega[l - 1];
+    }
+    d->info.levels++;
     d->info.level_size[0] = d->pn;
     d->info.omega[0] = 1.0;
-    d->info.dense_coarsest = 0;
-    const long long np = d->h->phi_size;
-    HIP_TRY(upload(d->owned, &d->sigs, sigma_s, (size_t)Ne));
-    for (double **q : {&d->bv, &d->xv}) HIP_TRY(upload<double>(d->owned, q, nullptr, (size_t)np));
-    for (double **q : {&d->pb, &d->px, &d->pw, &d->pt}) HIP_TRY(upload<double>(d->owned, q, nullptr, (size_t)d->pn));
+    for (double **q : {&d->pb, &d->px, &d->pw, &d->pt, &d->pr, &d->pz})
+        HIP_TRY(upload<double>(d->owned, q, nullptr, (size_t)d->pn));
+    // pe: phi-part embedding, zero outside the phi slots
+    HIP_TRY(upload<double>(d->owned, &d->pe, nullptr, (size_t)d->pn));
+    HIP_TRY(hipMemset(d->pe, 0, sizeof(double) * d->pn));
     HIP_TRY(upload<double>(d->owned, &d->pV, nullptr, (size_t)(mk + 1) * d->pn));
     d->pnb = allium::krylov_blocks(d->pn);
     HIP_TRY(upload<double>(d->owned, &d->ppart, nullptr, (size_t)2 * d->pnb));
     HIP_TRY(upload<double>(d->owned, &d->pH, nullptr, (size_t)(mk + 1) * (mk + 1) + mk + 1));
-    d->npartial = std::max(allium::dot_blocks(np), allium::dg_blocks(d->dg));
-    HIP_TRY(upload<double>(d->owned, &d->partial, nullptr, (size_t)d->npartial));
-    HIP_TRY(upload<double>(d->owned, &d->sc, nullptr, 8));
-    HIP_TRY(hipMemset(d->sc, 0, sizeof(double) * 8));
-    HIP_TRY(hipHostMalloc((void **)&d->hsc, sizeof(double) * 8, hipHostMallocDefault));
-    HIP_TRY(hipEventCreate(&d->ev0));
-    HIP_TRY(hipEventCreate(&d->ev1));
-    *out = d.release();
+    // dg_spmv_dot on the P1 level writes its own partial sums
+    HIP_TRY(upload<double>(d->owned, &d->ppart_spmv, nullptr, (size_t)allium::dg_blocks(d->p1)));
+    return ALLIUM_OK;
+}
+
+// z = B r, the P1 preconditioner: block-Jacobi, a MIP V-cycle on the phi-part
+// of the remaining residual (the MIP operator stands in for the Schur
+// complement of the P1 system on phi), block-Jacobi again
+int p1_prec(allium_dsa *d, const double *r, double *z, hipStream_t s)
+{
+    const allium::DgLevel &L = d->p1;
+    const int n = L.n / (L.dim + 1);
+    const size_t rowb = sizeof(double) * n, pitch = sizeof(double) * L.n;
+    const size_t off = (size_t)L.dim * n;
+    HIP_TRY(allium::dg_jacobi(L, nullptr, r, d->pz, s));
+    HIP_TRY(allium::dg_resid(L, d->pz, r, d->pr, s));
+    // rv, zv: the MIP CG vectors, free in this mode
+    HIP_TRY(hipMemcpy2DAsync(d->rv, rowb, d->pr + off, pitch, rowb, L.Ne, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(vcycle(d, d->rv, d->zv, s));
+    HIP_TRY(hipMemcpy2DAsync(d->pe + off, pitch, d->zv, rowb, rowb, L.Ne, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(allium::launch_axpby(d->pn, 1.0, d->pz, 1.0, d->pe, d->pz, s));
+    HIP_TRY(allium::dg_jacobi(L, d->pz, r, z, s));
     return ALLIUM_OK;
 }
 
 // phi-part of P^{-1} [0; b] (exe/chive.cpp:608-617: the TripleProductOperator
 // of ComponentExtractionOperator around the P1 solve).  The reference solves P
 // with a direct solver; here GMRES(kP1Kdim), preconditioned from the right by
-// the inverse diagonal element blocks, from x = 0 until the true residual
+// p1_prec, from x = 0 until the true residual
 // ||[0; b] - P x|| <= max(cg_reltol ||b||, cg_abstol) or cg_maxit steps.
 int p1_solve(allium_dsa *d, const double *b, double *x, hipStream_t s)
 {
-    const allium::DgLevel &L = d->dg;
+    const allium::DgLevel &L = d->p1;
     const long long np = d->pn;
     const int m = kP1Kdim, nb = d->pnb, n = L.n / (L.dim + 1), Ne = L.Ne;
     const size_t rowb = sizeof(double) * n, pitch = sizeof(double) * L.n;
@@ -581,7 +613,8 @@ int p1_solve(allium_dsa *d, const double *b, double *x, hipStream_t s)
         HIP_TRY(hipMemsetAsync(d->pt, 0, sizeof(double) * np, s));
         for (int j0 = 0; j0 < k; j0 += kP1Chunk)
             HIP_TRY(allium::launch_basis_update(np, std::min(kP1Chunk, k - j0), V(j0), np, yd + j0, d->pt, s));
-        HIP_TRY(allium::dg_jacobi(L, nullptr, d->pt, d->pw, s));
+        int rc = p1_prec(d, d->pt, d->pw, s);
+        if (rc) return rc;
         HIP_TRY(allium::launch_axpby(np, 1.0, d->px, 1.0, d->pw, d->px, s));
         // y lives in pageable host memory
         HIP_TRY(hipStreamSynchronize(s));
@@ -599,9 +632,9 @@ int p1_solve(allium_dsa *d, const double *b, double *x, hipStream_t s)
         int i = 0;
         for (; i < m && its < d->desc.cg_maxit; i++) {
             its++;
-            // w = P Dinv v_i
-            HIP_TRY(allium::dg_jacobi(L, nullptr, V(i), d->pt, s));
-            HIP_TRY(allium::dg_spmv_dot(L, d->pt, d->pw, d->partial, s));
+            // w = P B v_i
+            if ((rc = p1_prec(d, V(i), d->pt, s))) return rc;
+            HIP_TRY(allium::dg_spmv_dot(L, d->pt, d->pw, d->ppart_spmv, s));
             double *Hcol = Hd + (size_t)i * (m + 1);
             for (int k = 0; k <= i + 1; k++)
                 HIP_TRY(allium::launch_mgs(np, part[(k + 1) & 1], nb, k > 0 ? Hcol + (k - 1) : nullptr,
@@ -712,8 +745,7 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
     }
     allium::MipMatrix K;
     std::string why;
-    if (p1 ? !allium::assemble_p1(m, p, h->sigma.data(), siga.data(), kP1Alpha, K, why)
-           : !allium::assemble_mip(m, p, h->sigma.data(), siga.data(), mip_params(d->desc, p, h->quad), K, why))
+    if (!allium::assemble_mip(m, p, h->sigma.data(), siga.data(), mip_params(d->desc, p, h->quad), K, why))
         return fail(ALLIUM_ERR_ARG, why);
     const int n = K.n, ns = K.nside;
     // block-Jacobi inverses and the smoother damping of the DG level
@@ -723,9 +755,8 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
         for (long e = b; e < eend; e++)
             if (!invert(n, &Dinv[(size_t)e * n * n])) ok = false;
     });
-    if (!ok) return fail(ALLIUM_ERR_NUMERIC, "DSA: singular diagonal block of the diffusion matrix");
+    if (!ok) return fail(ALLIUM_ERR_NUMERIC, "DSA: singular diagonal block of the MIP matrix");
     const long long N0 = (long long)Ne * n;
-    if (p1) return create_p1(d.release(), K, Dinv, sigma_s, out);
     std::vector<double> tmp(N0);
     const double lam0 = power_max(N0, [&](const double *v, double *u) {
         host_mv(K, Ne, v, tmp.data());
@@ -832,6 +863,10 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
     HIP_TRY(hipHostMalloc((void **)&d->hsc, sizeof(double) * 8, hipHostMallocDefault));
     HIP_TRY(hipEventCreate(&d->ev0));
     HIP_TRY(hipEventCreate(&d->ev1));
+    if (p1) {
+        int rc = add_p1(d.get(), siga.data());
+        if (rc) return rc;
+    }
     *out = d.release();
     return ALLIUM_OK;
 }

@@ -5,7 +5,7 @@ test/chive/tdl.lua) written as YAML or a dict, runs the outer-solve path of
 exe/chive.cpp:550-750 (FormTransportSource, InverseAdvectionOperator,
 SLISolver, GMRESSolver or FGMRESSolver on TransportOperator as
 io::CreateIterativeSolver builds them (src/io.cpp:179-230), optional
-"scalar mip" DSA preconditioner, final sweep) and reports with chive's YAML keys ("transport iterations" with
+"scalar mip" or "p1sa" DSA preconditioner, final sweep) and reports with chive's YAML keys ("transport iterations" with
 it / norm / inner solver, "outer iterations", "inner iteration" max / min /
 avg / total; exe/chive.cpp:40-75,738-746), so scripts that parse chive's
 output read this one too.
@@ -22,6 +22,8 @@ Deck keys (YAML):
       fe_order: 1
       solver: {type: sli, abstol: 1.0e-6, reltol: 0, max_iter: 50}   # or gmres / fgmres with kdim
       preconditioner: {type: scalar mip, kappa: 4, solver: {type: cg, reltol: 1.0e-10, max_iter: 200}}
+      # or {type: p1sa, solver: {type: direct}}; a "direct" inner solve is CG
+      # (scalar mip) or GMRES (p1sa) to a relative residual of 1e-12
 
 Material `source` and boundary `value` are per-steradian isotropic densities,
 as in chive (exe/chive.cpp:236-247,294-306).  Boundary type "reflective"
@@ -91,8 +93,13 @@ def run(deck):
     if stype != "sli" and not 1 <= kdim <= 200:
         raise ValueError("kdim must be in 1..200")
     prec = drv.get("preconditioner")
-    if prec and str(prec.get("type", "")).lower() not in ("scalar mip", "mip"):
-        raise ValueError(f"preconditioner {prec.get('type')} is not supported (scalar mip only)")
+    ptype = str(prec.get("type", "")).lower() if prec else None
+    if prec and ptype not in ("scalar mip", "mip", "p1sa"):
+        raise ValueError(f"preconditioner {prec.get('type')} is not supported (scalar mip, p1sa)")
+    # chive's inner solver defaults to "direct"; p1sa takes nothing else (exe/chive.cpp:502-524,609)
+    itype = str(prec.get("solver", {}).get("type", "direct")).lower() if prec else None
+    if ptype == "p1sa" and itype not in ("direct", "superlu"):
+        raise ValueError("only direct available for P1")
 
     out = {}
     t0 = time.perf_counter()
@@ -102,8 +109,14 @@ def run(deck):
     q = Linv.form_source(Q=Q.reshape(1, -1), inflow=np.array(inflow).reshape(1, -1))
     accel = None
     out["preconditioner"] = "none"
-    if prec:
+    if ptype == "p1sa":
+        # the direct solve is emulated by GMRES to a relative residual of 1e-12
+        accel = A.DSA(Linv, sigma_s, method="p1sa", cg_reltol=1e-12, cg_maxit=20000)
+        out["preconditioner"] = {"type": "p1sa", "solver": {"type": itype}}
+    elif prec:
         inner = prec.get("solver", {})
+        if itype in ("direct", "superlu"):
+            inner = {"reltol": 1e-12, "max_iter": 1000}
         kappa = float(prec.get("kappa", (p + 1) ** 2))
         accel = A.DSA(Linv, sigma_s, kappa=kappa, bound_below=bool(prec.get("bound_stabilization_below", True)),
                       sym=float(prec.get("sigma", -1.0)), cg_reltol=float(inner.get("reltol", 1e-10)),

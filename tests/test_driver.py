@@ -26,7 +26,8 @@ def _deck():
     (lambda d: d.update(material_map=["mat"] * 3), "one entry per element"),
     (lambda d: d.update(boundary_map=["vacuum"] * 3), "boundary_map"),
     (lambda d: d["boundary_conditions"]["vacuum"].update(type="periodic"), "not supported"),
-    (lambda d: d["driver"].update(preconditioner={"type": "p1sa"}), "scalar mip only"),
+    (lambda d: d["driver"].update(preconditioner={"type": "ldgsa"}), "not supported"),
+    (lambda d: d["driver"].update(preconditioner={"type": "p1sa", "solver": {"type": "cg"}}), "only direct"),
 ])
 def test_deck_errors(edit, msg):
     d = _deck()
@@ -53,6 +54,21 @@ def test_tdl_deck_outer_iterations():
     assert float(rows[-1]["norm"]) < 1e-6 and rows[-1]["inner solver"]["it"] > 0
     assert out["inner iteration"]["total"] == sum(r["inner solver"]["it"] for r in rows)
     assert np.isfinite(out["balance"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["p1sa", "mip"])
+def test_tdl_deck_direct_inner_solver(prec):
+    """test/chive/CMakeLists.txt:117-121: the TDL deck with
+    `preconditioner = {type = "<dsa>", solver = {type = "direct"}}` for p1sa
+    and mip needs fewer than 12 outer iterations."""
+    print(f"driver tdl {prec} direct", flush=True)
+    d = _deck()
+    d["driver"]["preconditioner"] = {"type": prec, "solver": {"type": "direct"}}
+    out = driver.run(d)
+    print(yaml.safe_dump({k: out[k] for k in ("outer iterations", "inner iteration", "preconditioner")}))
+    assert out["converged"] and out["outer iterations"] < 12
+    assert out["preconditioner"]["type"] == ("p1sa" if prec == "p1sa" else "scalar mip")
 
 
 @pytest.mark.gpu
