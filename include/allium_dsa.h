@@ -20,6 +20,12 @@
  * d-linear vertex space as first coarse level (Galerkin P^T K P) and
  * vertex-grid coarsening by two down to a dense coarsest solve.
  * One group only (chive's steady one-group SI).
+ *
+ * desc.method = ALLIUM_DSA_P1 selects chive's "p1sa" preconditioner instead
+ * (exe/chive.cpp:608-619): y = x + (P^{-1} [0; Ms x])_phi with P the 2x2 P1
+ * system of allium_host_p1_matrix, which the reference solves directly.  Here
+ * GMRES solves it, preconditioned by block-Jacobi around one V-cycle of the
+ * MIP hierarchy above on the phi-part of the residual.
  */
 #ifndef ALLIUM_DSA_H
 #define ALLIUM_DSA_H
