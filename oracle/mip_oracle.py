@@ -16,6 +16,9 @@ Pinned in tests/test_dsa_oracle.py by closed forms (row sums: absorption plus
 Marshak boundary mass; the product with a linear function: the boundary flux
 D du/dn) and by the reference's thick-diffusion-limit test deck
 (test/chive/tdl.lua, test/chive/test_tdl.py: fewer than 12 iterations).
+p1_matrix is pinned there by closed forms on constant and element-wise
+constant fields (volume, boundary and interior face terms of every block,
+G = -D^T) and by the same deck with the p1sa preconditioner.
 """
 import numpy as np
 import scipy.sparse as sp

@@ -267,6 +267,65 @@ def test_p1sa_accelerates_the_tdl_deck():
     assert np.linalg.norm(r) < 1e-5 * np.linalg.norm(phi)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_p1_interior_face_terms_on_elementwise_constants(dim):
+    """The interior face terms of p1_matrix, which global constants do not see:
+    element-wise constant currents c_e and fluxes f_e on a uniform mesh, p = 1,
+    all sides Marshak.  For node i of element e on a side with outward normal n
+    (axis a) and node face integral A = side area / 2^(dim-1):
+      Mt row (component a):  beta A (c_e - c_nbr)_a on interior sides, beta = 1/(2 alpha)
+                             (DGVectorJumpJump on the normal component);
+      D row:                 A (c_e + c_nbr)/2 . n - A c_e . n = A (c_nbr - c_e) . n / 2
+                             (integration by parts plus the {J.n} average);
+      Ma row:                (alpha/2) A (f_e - f_nbr) (unscaled penalty),
+    plus the volume and boundary parts of the constant-field test."""
+    n_el = 3
+    nel = (n_el,) * dim
+    vert = P.cartesian_vertices(nel)
+    rng = np.random.default_rng(9)
+    ne = n_el ** dim
+    st = rng.uniform(0.5, 2.0, ne)
+    ss = st * rng.uniform(0.1, 0.9, ne)
+    alpha = 0.5
+    beta = 1 / (2 * alpha)
+    Pm = M.p1_matrix(dim, nel, vert, 1, st, ss, alpha=alpha).toarray()
+    n, N = 2 ** dim, (dim + 1) * 2 ** dim
+    h = 1.0 / n_el
+    vol, area = h ** dim / n, h ** (dim - 1) / 2 ** (dim - 1)
+    C = rng.normal(size=(ne, dim))
+    F = rng.normal(size=ne)
+    x = np.zeros((ne, dim + 1, n))
+    x[:, :dim, :] = C[:, :, None]
+    x[:, dim, :] = F[:, None]
+    y = (Pm @ x.ravel()).reshape(ne, dim + 1, n)
+    for e in range(ne):
+        co = [(e // n_el ** d) % n_el for d in range(dim)]
+        Mt = 3 * st[e] * vol * np.repeat(C[e][:, None], n, 1)
+        Dr = np.zeros(n)
+        Ma = (st[e] - ss[e]) * vol * F[e] * np.ones(n)
+        for ax in range(dim):
+            for hi in (0, 1):
+                mask = np.array([((i >> ax) & 1) == hi for i in range(n)])
+                sgn = 1.0 if hi else -1.0
+                if co[ax] == (n_el - 1) * hi:
+                    # Marshak boundary side
+                    Mt[ax][mask] += beta * area * C[e][ax]
+                    Dr[mask] += -0.5 * area * sgn * C[e][ax]
+                    Ma[mask] += 0.5 * alpha * area * F[e]
+                else:
+                    o = e + (1 if hi else -1) * n_el ** ax
+                    Mt[ax][mask] += beta * area * (C[e][ax] - C[o][ax])
+                    Dr[mask] += 0.5 * area * sgn * (C[o][ax] - C[e][ax])
+                    Ma[mask] += 0.5 * alpha * area * (F[e] - F[o])
+        # the phi rows hold D c + Ma f; the current rows Mt c + G f
+        np.testing.assert_allclose(y[e, dim], Dr + Ma, atol=1e-14)
+        Gf = y[e, :dim] - Mt
+        # G = -D^T: pinned separately; here only that the current rows differ from Mt c by G f
+        xf = np.zeros((ne, dim + 1, n))
+        xf[:, dim, :] = F[:, None]
+        np.testing.assert_allclose(Gf, (Pm @ xf.ravel()).reshape(ne, dim + 1, n)[e, :dim], atol=1e-13)
+
+
 @pytest.mark.parametrize("dim,p,jitter", [(2, 1, 0.0), (2, 1, 0.2), (2, 2, 0.1), (3, 1, 0.0), (3, 1, 0.15)])
 def test_host_p1_assembly_matches_oracle(dim, p, jitter):
     """allium_host_p1_matrix (C++) against the numpy restatement, with one

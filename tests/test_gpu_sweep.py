@@ -1,6 +1,6 @@
 """GPU parity of the HIP sweep against the CPU oracle, through the C ABI.
 
-Bar: FP64 sweeps within 1e-12 relative of the oracle (north-star tolerance);
+Bar: FP64 sweeps within 1e-12 relative of the oracle;
 restated reference tests (psi == 1, L(L^-1 x) = x) on the GPU path; the
 FP32-refined path within its stated bound.  Sizes are those the oracle runs in
 seconds; the full 64^3 workload is covered by size-independent properties.
