@@ -29,6 +29,15 @@ __device__ __forceinline__ void cell_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// every entry of r is computed before any code after this point that the
+// compiler would otherwise hoist (empty asm statements keep their order)
+template <int N>
+__device__ __forceinline__ void fence_row(double (&r)[N])
+{
+#pragma unroll
+    for (int j = 0; j < N; ++j) asm volatile("" : "+v"(r[j]));
+}
+
 template <int DIM, int P, bool UNI, typename T>
 __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepParams *__restrict__ prms,
                                                     CoopLevel cl)
@@ -71,14 +80,31 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
         const double *Mu = p.Mu + (size_t)e * N * N + (size_t)rr * N;
         const double *Mt = p.Mt + (size_t)e * N * N + (size_t)rr * N;
         const double tau = p.timedep ? p.tau : 0.0;
+        // one matrix row at a time: each phase's row pointer passes through an
+        // empty asm after fences on every entry of the running sum, so the
+        // compiler cannot hoist all 5 N loads ahead of the sums (that took
+        // 256 VGPRs + AGPRs, one wave per SIMD)
+        const double *q = Gd;
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            double v = om[0] * Gd[j];
+        for (int j = 0; j < N; ++j) Ar[j] = om[0] * q[j];
 #pragma unroll
-            for (int d = 1; d < DIM; ++d) v = fma(om[d], Gd[(size_t)d * N * N + j], v);
-            v = fma(sig, Mu[j], v);
-            Ar[j] = fma(tau, Mt[j], v);
+        for (int d = 1; d < DIM; ++d) {
+            q = Gd + (size_t)d * N * N;
+            fence_row<N>(Ar);
+            asm volatile("" : "+v"(q));
+#pragma unroll
+            for (int j = 0; j < N; ++j) Ar[j] = fma(om[d], q[j], Ar[j]);
         }
+        q = Mu;
+        fence_row<N>(Ar);
+        asm volatile("" : "+v"(q));
+#pragma unroll
+        for (int j = 0; j < N; ++j) Ar[j] = fma(sig, q[j], Ar[j]);
+        q = Mt;
+        fence_row<N>(Ar);
+        asm volatile("" : "+v"(q));
+#pragma unroll
+        for (int j = 0; j < N; ++j) Ar[j] = fma(tau, q[j], Ar[j]);
     }
     if (p.src) b = p.src[off + rr];
     if (p.iso) b += p.iso[(long long)g * p.Ndof + (long long)e * N + rr];
