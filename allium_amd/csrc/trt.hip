@@ -296,6 +296,23 @@ __global__ void k_minmax(long long n, const double *__restrict__ x, double *__re
     if (threadIdx.x == 0) { part[2 * blockIdx.x] = lo[0]; part[2 * blockIdx.x + 1] = hi[0]; }
 }
 
+// part[blk] = sum of x[i] w[i % N] over a grid-stride share (summed on the host in block order)
+__global__ void k_wsum(long long n, const double *__restrict__ x, const double *__restrict__ w,
+                       double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        acc = fma(x[i], w[i % N], acc);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 thread_local std::string g_err;
 int tfail(int code, const std::string &m)
 {
@@ -326,7 +343,8 @@ struct allium_trt {
     TrtParams P{};
     std::vector<void *> owned;
     double *Tk = nullptr;   // temperature before the current outer pass
-    double *T = nullptr, *Tn = nullptr, *psi = nullptr, *psin = nullptr, *qin = nullptr, *qpsi = nullptr;
+    double *T = nullptr, *Tn = nullptr, *psi = nullptr, *qin = nullptr, *qpsi = nullptr;
+    double *volw = nullptr;   // element-mass row sums: int phi = sum_k phi[k] volw[k % N]
     double *sigma = nullptr, *em = nullptr, *phi = nullptr, *phisrc = nullptr, *phit = nullptr, *iso = nullptr;
     double *resid = nullptr, *Binv = nullptr, *dpw = nullptr, *part = nullptr, *dmid = nullptr;
     int *flags = nullptr;
@@ -461,6 +479,12 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     TRT_TRY(h->alloc(&dp, w4.size(), w4.data())); P.w4 = dp;
     TRT_TRY(h->alloc(&dp, Mm.size(), Mm.data())); P.Mm = dp;
     TRT_TRY(h->alloc(&dp, Mu.size(), Mu.data())); P.Mu = dp;
+    {
+        std::vector<double> vw(N, 0.0);
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) vw[i] += Mu[(size_t)i * N + j];
+        TRT_TRY(h->alloc(&h->volw, vw.size(), vw.data()));
+    }
     TRT_TRY(h->alloc(&dp, h->bounds.size(), h->bounds.data())); P.bounds = dp;
     TRT_TRY(h->alloc(&h->dmid, h->mid.size(), h->mid.data()));
     TRT_TRY(h->alloc(&h->sigma, sig0.size(), sig0.data()));
@@ -473,7 +497,6 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     TRT_TRY(h->alloc(&h->Binv, (size_t)h->Ne * N * N));
     TRT_TRY(h->alloc(&h->dpw, (size_t)h->Ne * P.nq3 * h->G));
     TRT_TRY(h->alloc(&h->psi, h->psi_size));
-    TRT_TRY(h->alloc(&h->psin, h->psi_size));
     TRT_TRY(h->alloc(&h->qin, h->psi_size));
     TRT_TRY(h->alloc(&h->qpsi, h->psi_size));
     TRT_TRY(h->alloc(&h->em, h->phi_size));
@@ -481,7 +504,7 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     TRT_TRY(h->alloc(&h->phisrc, h->phi_size));
     TRT_TRY(h->alloc(&h->phit, h->phi_size));
     TRT_TRY(h->alloc(&h->iso, h->phi_size));
-    TRT_TRY(h->alloc(&h->part, 2 * kPart));
+    TRT_TRY(h->alloc(&h->part, 3 * kPart));
     TRT_TRY(h->alloc(&h->flags, 4));
     // initial state: T = T0, psi in equilibrium with T0 (PlanckEmissionPSCoefficient,
     // src/phase_coefficient.cpp:72-76: the Planck fraction between the group bounds)
@@ -523,10 +546,10 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
     const int eb = (h->Ne + 63) / 64;
     // previous time level
     TRT_TRY(hipMemcpyAsync(h->Tn, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToDevice, s));
-    TRT_TRY(hipMemcpyAsync(h->psin, h->psi, sizeof(double) * h->psi_size, hipMemcpyDeviceToDevice, s));
+    // the time source reads psi^n before the first sweep of the step overwrites it
     const long long cells = h->psi_size / N;
     hipLaunchKernelGGL(k_time_source, dim3((unsigned)((cells * N + 255) / 256)), dim3(256), 0, s, cells,
-                       1.0 / h->d.dt, P.Mm, h->qin, h->psin, h->qpsi);
+                       1.0 / h->d.dt, P.Mm, h->qin, h->psi, h->qpsi);
     TRT_TRY(hipMemsetAsync(h->flags, 0, sizeof(int) * 4, s));
     // LinearizedTRTOperator::Mult turns a fixup off for the transport solve and
     // back on for the final sweep (src/trt_linearized.cpp:236-238,284-289)
@@ -562,7 +585,8 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
         ABI_TRY(allium_sweep_mult_moments_device(h->L, h->qpsi, h->iso, h->psi, h->phisrc, s));
         // source iteration on (I - D Linv D^T Ms) phi = phi_src (SLISolver, src/linalg.cpp:103-153);
         // a warm start begins from D psi of the current iterate
-        if (h->d.si_warm_start) ABI_TRY(allium_moments_device(h->L, k == 1 ? h->psin : h->psi, h->phi, s));
+        // (in the first pass psi is still psi^n)
+        if (h->d.si_warm_start) ABI_TRY(allium_moments_device(h->L, h->psi, h->phi, s));
         else TRT_TRY(hipMemsetAsync(h->phi, 0, sizeof(double) * h->phi_size, s));
         double r0 = 0.0;
         for (it = 1;; it++) {
@@ -612,10 +636,12 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
         int fl = 0;
         TRT_TRY(hipMemcpyAsync(&fl, h->flags + 1, sizeof(int), hipMemcpyDeviceToHost, s));
         hipLaunchKernelGGL(k_minmax, dim3(kPart), dim3(256), 0, s, (long long)h->Ndof, h->T, h->part);
-        std::vector<double> mm(2 * kPart);
+        // sum_g int phi_g = sum over dofs of phi times the element-mass row sums
+        hipLaunchKernelGGL(k_wsum, dim3(kPart), dim3(256), 0, s, (long long)h->phi_size, h->phi, h->volw,
+                           h->part + 2 * kPart);
+        TRT_TRY(hipGetLastError());
+        std::vector<double> mm(3 * kPart);
         TRT_TRY(hipMemcpyAsync(mm.data(), h->part, sizeof(double) * mm.size(), hipMemcpyDeviceToHost, s));
-        std::vector<double> phih(h->phi_size);
-        TRT_TRY(hipMemcpyAsync(phih.data(), h->phi, sizeof(double) * h->phi_size, hipMemcpyDeviceToHost, s));
         TRT_TRY(hipStreamSynchronize(s));
         info->step = h->step;
         info->time = h->time;
@@ -630,16 +656,8 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
             info->T_min = std::min(info->T_min, mm[2 * k]);
             info->T_max = std::max(info->T_max, mm[2 * k + 1]);
         }
-        // sum_g int phi_g = sum over dofs of phi times the element-mass row sums
-        double vol_w[N];
-        std::vector<double> Mu(N * N);
-        TRT_TRY(hipMemcpy(Mu.data(), P.Mu, sizeof(double) * N * N, hipMemcpyDeviceToHost));
-        for (int i = 0; i < N; i++) {
-            vol_w[i] = 0.0;
-            for (int j = 0; j < N; j++) vol_w[i] += Mu[i * N + j];
-        }
         double E = 0.0;
-        for (long k = 0; k < h->phi_size; k++) E += phih[k] * vol_w[k % N];
+        for (int k = 0; k < kPart; k++) E += mm[2 * kPart + k];
         info->radiation_energy = E;
         ABI_TRY(allium_sweep_timing(h->L, &info->sweep_seconds, &info->sweeps));
     }
