@@ -95,17 +95,60 @@ __global__ void k_opacity(int Ne, int G, const double *__restrict__ T, const dou
     for (int g = 0; g < G; ++g) sigma[(long long)g * Ne + e] = coef * rs / (mid[g] * mid[g] * mid[g]);
 }
 
-// per element: resid, Bt'^{-1}, group emission em (phi-sized) and the
-// emission-gradient weights dpw[e][q][g] = w_q sigma_g 4 T_q^3 r_g(T_q)
+// per (element, quadrature point): the Planck sums, which dominate the
+// linearization.  A rule-4 point stores its gray emission and derivative
+// weights W4[e][q][2] (Rosseland spectrum); a rule-3 point stores the group
+// emission weights E3[e][q][g] and the emission-gradient weights
+// dpw[e][q][g] = w_q sigma_g 4 T_q^3 r_g(T_q)
+__global__ void __launch_bounds__(256) k_planck_points(TrtParams P, const double *__restrict__ T,
+                                                       double *__restrict__ W4, double *__restrict__ E3,
+                                                       double *__restrict__ dpw)
+{
+    const int nq = P.nq4 + P.nq3;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)P.Ne * nq) return;
+    const int e = (int)(t / nq), q = (int)(t - (long long)e * nq);
+    const double *Te = T + (long long)e * N;
+    double tv[N], spec[GMAX];
+#pragma unroll
+    for (int i = 0; i < N; ++i) tv[i] = Te[i];
+    if (q < P.nq4) {
+        const double *ph = P.phi4 + q * N;
+        const double Tq = dot_basis(ph, tv);
+        double sp = 0.0, sr = 0.0;
+        allium::planck_spectrum<0>(P.bounds, P.G, Tq, spec);
+        for (int g = 0; g < P.G; ++g) sp = fma(P.sigma[(long long)g * P.Ne + e], spec[g], sp);
+        allium::planck_spectrum<1>(P.bounds, P.G, Tq, spec);
+        for (int g = 0; g < P.G; ++g) sr = fma(P.sigma[(long long)g * P.Ne + e], spec[g], sr);
+        const double T3 = Tq * Tq * Tq;
+        W4[((long long)e * P.nq4 + q) * 2] = P.w4[q] * sp * T3 * Tq;     // emission
+        W4[((long long)e * P.nq4 + q) * 2 + 1] = P.w4[q] * 4.0 * sr * T3;  // its derivative
+    } else {
+        const int r = q - P.nq4;
+        const double *ph = P.phi3 + r * N;
+        const double Tq = dot_basis(ph, tv);
+        const double T3 = Tq * Tq * Tq;
+        allium::planck_spectrum<0>(P.bounds, P.G, Tq, spec);
+        for (int g = 0; g < P.G; ++g)
+            E3[((long long)e * P.nq3 + r) * P.G + g] = P.w3[r] * P.sigma[(long long)g * P.Ne + e] * spec[g] * T3 * Tq;
+        allium::planck_spectrum<1>(P.bounds, P.G, Tq, spec);
+        for (int g = 0; g < P.G; ++g)
+            dpw[((long long)e * P.nq3 + r) * P.G + g] = P.w3[r] * P.sigma[(long long)g * P.Ne + e] * 4.0 * spec[g] * T3;
+    }
+}
+
+// per element, from the point weights of k_planck_points: resid, Bt'^{-1}
+// and the group emission em (phi-sized)
 __global__ void __launch_bounds__(64) k_linearize(TrtParams P, const double *__restrict__ T,
-                                                  const double *__restrict__ Tn, double *__restrict__ resid,
+                                                  const double *__restrict__ Tn, const double *__restrict__ W4,
+                                                  const double *__restrict__ E3, double *__restrict__ resid,
                                                   double *__restrict__ Binv, double *__restrict__ em,
-                                                  double *__restrict__ dpw, int *__restrict__ bad)
+                                                  int *__restrict__ bad)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= P.Ne) return;
     const double *Te = T + (long long)e * N;
-    double tv[N], A[N][N], spec[GMAX];
+    double tv[N], A[N][N];
 #pragma unroll
     for (int i = 0; i < N; ++i) tv[i] = Te[i];
     // (c_v/dt) M and resid = (c_v/dt) M (T^n - T)
@@ -122,15 +165,8 @@ __global__ void __launch_bounds__(64) k_linearize(TrtParams P, const double *__r
     // gray emission and its gradient (Rosseland spectrum), rule 4
     for (int q = 0; q < P.nq4; ++q) {
         const double *ph = P.phi4 + q * N;
-        const double Tq = dot_basis(ph, tv);
-        double sp = 0.0, sr = 0.0;
-        allium::planck_spectrum<0>(P.bounds, P.G, Tq, spec);
-        for (int g = 0; g < P.G; ++g) sp = fma(P.sigma[(long long)g * P.Ne + e], spec[g], sp);
-        allium::planck_spectrum<1>(P.bounds, P.G, Tq, spec);
-        for (int g = 0; g < P.G; ++g) sr = fma(P.sigma[(long long)g * P.Ne + e], spec[g], sr);
-        const double T3 = Tq * Tq * Tq;
-        const double we = P.w4[q] * sp * T3 * Tq;      // emission
-        const double wg = P.w4[q] * 4.0 * sr * T3;     // its derivative
+        const double we = W4[((long long)e * P.nq4 + q) * 2];
+        const double wg = W4[((long long)e * P.nq4 + q) * 2 + 1];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             resid[(long long)e * N + i] -= we * ph[i];
@@ -144,22 +180,16 @@ __global__ void __launch_bounds__(64) k_linearize(TrtParams P, const double *__r
     for (int i = 0; i < N; ++i)
 #pragma unroll
         for (int j = 0; j < N; ++j) Binv[((long long)e * N + i) * N + j] = inv[i][j];
-    // group emission and gradient weights, rule 3
+    // group emission, rule 3
     for (int i = 0; i < N; ++i)
         for (int g = 0; g < P.G; ++g) em[((long long)g * P.Ne + e) * N + i] = 0.0;
     for (int q = 0; q < P.nq3; ++q) {
         const double *ph = P.phi3 + q * N;
-        const double Tq = dot_basis(ph, tv);
-        const double T3 = Tq * Tq * Tq;
-        allium::planck_spectrum<0>(P.bounds, P.G, Tq, spec);
         for (int g = 0; g < P.G; ++g) {
-            const double s = P.w3[q] * P.sigma[(long long)g * P.Ne + e] * spec[g] * T3 * Tq;
+            const double s = E3[((long long)e * P.nq3 + q) * P.G + g];
             double *eg = em + ((long long)g * P.Ne + e) * N;
             for (int i = 0; i < N; ++i) eg[i] = fma(s, ph[i], eg[i]);
         }
-        allium::planck_spectrum<1>(P.bounds, P.G, Tq, spec);
-        for (int g = 0; g < P.G; ++g)
-            dpw[((long long)e * P.nq3 + q) * P.G + g] = P.w3[q] * P.sigma[(long long)g * P.Ne + e] * 4.0 * spec[g] * T3;
     }
 }
 
@@ -347,6 +377,7 @@ struct allium_trt {
     double *volw = nullptr;   // element-mass row sums: int phi = sum_k phi[k] volw[k % N]
     double *sigma = nullptr, *em = nullptr, *phi = nullptr, *phisrc = nullptr, *phit = nullptr, *iso = nullptr;
     double *resid = nullptr, *Binv = nullptr, *dpw = nullptr, *part = nullptr, *dmid = nullptr;
+    double *W4 = nullptr, *E3 = nullptr;   // Planck weights per quadrature point (k_planck_points)
     int *flags = nullptr;
     hipStream_t s = nullptr;
     ~allium_trt()
@@ -496,6 +527,8 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     TRT_TRY(h->alloc(&h->resid, h->Ndof));
     TRT_TRY(h->alloc(&h->Binv, (size_t)h->Ne * N * N));
     TRT_TRY(h->alloc(&h->dpw, (size_t)h->Ne * P.nq3 * h->G));
+    TRT_TRY(h->alloc(&h->E3, (size_t)h->Ne * P.nq3 * h->G));
+    TRT_TRY(h->alloc(&h->W4, (size_t)h->Ne * P.nq4 * 2));
     TRT_TRY(h->alloc(&h->psi, h->psi_size));
     TRT_TRY(h->alloc(&h->qin, h->psi_size));
     TRT_TRY(h->alloc(&h->qpsi, h->psi_size));
@@ -571,8 +604,13 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
             TRT_TRY(hipMemcpyAsync(h->Tk, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToDevice, s));
         if (fix_on) ABI_TRY(allium_sweep_use_fixup(h->L, 0));
         // linearize at the current T
-        hipLaunchKernelGGL(k_linearize, dim3(eb), dim3(64), 0, s, P, h->T, h->Tn, h->resid, h->Binv, h->em, h->dpw,
-                           h->flags);
+        {
+            const long long pts = (long long)h->Ne * (P.nq4 + P.nq3);
+            hipLaunchKernelGGL(k_planck_points, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, s, P, h->T, h->W4,
+                               h->E3, h->dpw);
+        }
+        hipLaunchKernelGGL(k_linearize, dim3(eb), dim3(64), 0, s, P, h->T, h->Tn, h->W4, h->E3, h->resid, h->Binv,
+                           h->em, h->flags);
         // em += dB Bt'^{-1} resid
         hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 0, h->resid, h->Binv, h->dpw, 1.0, 1, h->em);
         TRT_TRY(hipGetLastError());
