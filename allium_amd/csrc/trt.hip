@@ -272,18 +272,39 @@ __global__ void k_update_T(TrtParams P, const double *__restrict__ phi, const do
     if (fl) atomicAdd(floors, fl);
 }
 
-// q_psi = inflow + (1/dt) Mt psi^n, one thread per (cell, dof)
+// q_psi = (1/dt) Mt psi^n, one thread per cell (8 consecutive doubles in, 8 out)
 __global__ void k_time_source(long long cells, double tau, const double *__restrict__ Mm,
-                              const double *__restrict__ qin, const double *__restrict__ psin, double *__restrict__ out)
+                              const double *__restrict__ psin, double *__restrict__ out)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cells * N) return;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    double x[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) x[j] = psin[c * N + j];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) a = fma(Mm[i * N + j], x[j], a);
+        out[c * N + i] = tau * a;
+    }
+}
+
+// the inflow source, nonzero on boundary cells only: q_psi[k] = q_in[k] + (1/dt) (Mt psi^n)[k]
+// for the listed entries, with the same fused rounding as a dense pass
+__global__ void k_time_source_inflow(long long nnz, double tau, const double *__restrict__ Mm,
+                                     const long long *__restrict__ idx, const double *__restrict__ val,
+                                     const double *__restrict__ psin, double *__restrict__ out)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const long long t = idx[k];
     const int i = (int)(t % N);
     const double *x = psin + (t - i);
     double a = 0.0;
 #pragma unroll
     for (int j = 0; j < N; ++j) a = fma(Mm[i * N + j], x[j], a);
-    out[t] = fma(tau, a, qin ? qin[t] : 0.0);
+    out[t] = fma(tau, a, val[k]);
 }
 
 // part[blk] = sum (a - b)^2 over a grid-stride share (summed on the host in block order)
@@ -373,7 +394,10 @@ struct allium_trt {
     TrtParams P{};
     std::vector<void *> owned;
     double *Tk = nullptr;   // temperature before the current outer pass
-    double *T = nullptr, *Tn = nullptr, *psi = nullptr, *qin = nullptr, *qpsi = nullptr;
+    double *T = nullptr, *Tn = nullptr, *psi = nullptr, *qpsi = nullptr;
+    long long qin_nnz = 0;                  // inflow source as its nonzero entries
+    long long *qin_idx = nullptr;
+    double *qin_val = nullptr;
     double *volw = nullptr;   // element-mass row sums: int phi = sum_k phi[k] volw[k % N]
     double *sigma = nullptr, *em = nullptr, *phi = nullptr, *phisrc = nullptr, *phit = nullptr, *iso = nullptr;
     double *resid = nullptr, *Binv = nullptr, *dpw = nullptr, *part = nullptr, *dmid = nullptr;
@@ -530,7 +554,6 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     TRT_TRY(h->alloc(&h->E3, (size_t)h->Ne * P.nq3 * h->G));
     TRT_TRY(h->alloc(&h->W4, (size_t)h->Ne * P.nq4 * 2));
     TRT_TRY(h->alloc(&h->psi, h->psi_size));
-    TRT_TRY(h->alloc(&h->qin, h->psi_size));
     TRT_TRY(h->alloc(&h->qpsi, h->psi_size));
     TRT_TRY(h->alloc(&h->em, h->phi_size));
     TRT_TRY(h->alloc(&h->phi, h->phi_size));
@@ -561,7 +584,17 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
             if ((d.inflow_sides >> s) & 1) inflow[(size_t)g * 6 + s] = b * std::pow(Ti, 4) / fourpi;
     }
     ABI_TRY(allium_form_source(h->L, nullptr, inflow.data(), psi0.data()));
-    TRT_TRY(hipMemcpy(h->qin, psi0.data(), sizeof(double) * h->psi_size, hipMemcpyHostToDevice));
+    {
+        std::vector<long long> idx;
+        std::vector<double> val;
+        for (long long k = 0; k < h->psi_size; k++)
+            if (psi0[k] != 0.0) { idx.push_back(k); val.push_back(psi0[k]); }
+        h->qin_nnz = (long long)idx.size();
+        if (h->qin_nnz) {
+            TRT_TRY(h->alloc(&h->qin_idx, idx.size(), idx.data()));
+            TRT_TRY(h->alloc(&h->qin_val, val.size(), val.data()));
+        }
+    }
     ABI_TRY(update_opacity(h.get()));
     *out = h.release();
     return ALLIUM_OK;
@@ -581,8 +614,11 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
     TRT_TRY(hipMemcpyAsync(h->Tn, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToDevice, s));
     // the time source reads psi^n before the first sweep of the step overwrites it
     const long long cells = h->psi_size / N;
-    hipLaunchKernelGGL(k_time_source, dim3((unsigned)((cells * N + 255) / 256)), dim3(256), 0, s, cells,
-                       1.0 / h->d.dt, P.Mm, h->qin, h->psi, h->qpsi);
+    hipLaunchKernelGGL(k_time_source, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, cells, 1.0 / h->d.dt,
+                       P.Mm, h->psi, h->qpsi);
+    if (h->qin_nnz)
+        hipLaunchKernelGGL(k_time_source_inflow, dim3((unsigned)((h->qin_nnz + 255) / 256)), dim3(256), 0, s,
+                           h->qin_nnz, 1.0 / h->d.dt, P.Mm, h->qin_idx, h->qin_val, h->psi, h->qpsi);
     TRT_TRY(hipMemsetAsync(h->flags, 0, sizeof(int) * 4, s));
     // LinearizedTRTOperator::Mult turns a fixup off for the transport solve and
     // back on for the final sweep (src/trt_linearized.cpp:236-238,284-289)
