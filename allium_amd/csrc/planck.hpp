@@ -97,4 +97,19 @@ __host__ __device__ inline void planck_spectrum(const double *bounds, int G, dou
     out[G - 1] = 1.0 - prev;
 }
 
+// the same fractions handed to f(g, fraction) in group order, without an
+// array (a per-thread array indexed by a runtime group count lives in scratch)
+template <int R, typename F>
+__host__ __device__ inline void planck_spectrum_each(const double *bounds, int G, double T, F &&f)
+{
+    double prev = 0.0;
+    for (int g = 0; g < G - 1; ++g) {
+        const double x = bounds[g + 1] / T;
+        const double next = R ? rosseland_integral(x) : planck_integral(x);
+        f(g, next - prev);
+        prev = next;
+    }
+    f(G - 1, 1.0 - prev);
+}
+
 }  // namespace allium

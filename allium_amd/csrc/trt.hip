@@ -109,17 +109,17 @@ __global__ void __launch_bounds__(256) k_planck_points(TrtParams P, const double
     if (t >= (long long)P.Ne * nq) return;
     const int e = (int)(t / nq), q = (int)(t - (long long)e * nq);
     const double *Te = T + (long long)e * N;
-    double tv[N], spec[GMAX];
+    double tv[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) tv[i] = Te[i];
+    const double *sig = P.sigma + e;
+    const long long gs = P.Ne;
     if (q < P.nq4) {
         const double *ph = P.phi4 + q * N;
         const double Tq = dot_basis(ph, tv);
         double sp = 0.0, sr = 0.0;
-        allium::planck_spectrum<0>(P.bounds, P.G, Tq, spec);
-        for (int g = 0; g < P.G; ++g) sp = fma(P.sigma[(long long)g * P.Ne + e], spec[g], sp);
-        allium::planck_spectrum<1>(P.bounds, P.G, Tq, spec);
-        for (int g = 0; g < P.G; ++g) sr = fma(P.sigma[(long long)g * P.Ne + e], spec[g], sr);
+        allium::planck_spectrum_each<0>(P.bounds, P.G, Tq, [&](int g, double v) { sp = fma(sig[g * gs], v, sp); });
+        allium::planck_spectrum_each<1>(P.bounds, P.G, Tq, [&](int g, double v) { sr = fma(sig[g * gs], v, sr); });
         const double T3 = Tq * Tq * Tq;
         W4[((long long)e * P.nq4 + q) * 2] = P.w4[q] * sp * T3 * Tq;     // emission
         W4[((long long)e * P.nq4 + q) * 2 + 1] = P.w4[q] * 4.0 * sr * T3;  // its derivative
@@ -128,12 +128,13 @@ __global__ void __launch_bounds__(256) k_planck_points(TrtParams P, const double
         const double *ph = P.phi3 + r * N;
         const double Tq = dot_basis(ph, tv);
         const double T3 = Tq * Tq * Tq;
-        allium::planck_spectrum<0>(P.bounds, P.G, Tq, spec);
-        for (int g = 0; g < P.G; ++g)
-            E3[((long long)e * P.nq3 + r) * P.G + g] = P.w3[r] * P.sigma[(long long)g * P.Ne + e] * spec[g] * T3 * Tq;
-        allium::planck_spectrum<1>(P.bounds, P.G, Tq, spec);
-        for (int g = 0; g < P.G; ++g)
-            dpw[((long long)e * P.nq3 + r) * P.G + g] = P.w3[r] * P.sigma[(long long)g * P.Ne + e] * 4.0 * spec[g] * T3;
+        double *e3 = E3 + ((long long)e * P.nq3 + r) * P.G;
+        double *dp = dpw + ((long long)e * P.nq3 + r) * P.G;
+        const double w = P.w3[r];
+        allium::planck_spectrum_each<0>(P.bounds, P.G, Tq,
+                                        [&](int g, double v) { e3[g] = w * sig[g * gs] * v * T3 * Tq; });
+        allium::planck_spectrum_each<1>(P.bounds, P.G, Tq,
+                                        [&](int g, double v) { dp[g] = w * sig[g * gs] * 4.0 * v * T3; });
     }
 }
 
