@@ -49,7 +49,8 @@ class KrylovOptions(ctypes.Structure):
 class DSADesc(ctypes.Structure):
     _fields_ = [("kappa", ctypes.c_double), ("bound_below", ctypes.c_int), ("alpha", ctypes.c_double),
                 ("sym", ctypes.c_double), ("cg_reltol", ctypes.c_double), ("cg_abstol", ctypes.c_double),
-                ("cg_maxit", ctypes.c_int), ("smooth", ctypes.c_int), ("method", ctypes.c_int)]
+                ("cg_maxit", ctypes.c_int), ("smooth", ctypes.c_int), ("method", ctypes.c_int),
+                ("scale_penalty", ctypes.c_int)]
 
 
 class DSAInfo(ctypes.Structure):
@@ -142,6 +143,10 @@ SIGNATURES = {
     "allium_host_mip_matrix": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc), c_dp,
                                               c_dp, ctypes.POINTER(DSADesc), ctypes.POINTER(ctypes.c_longlong),
                                               ctypes.POINTER(ctypes.c_longlong), c_ip, c_dp]),
+    "allium_host_block_mip_matrix": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc),
+                                                    c_dp, c_dp, ctypes.POINTER(DSADesc),
+                                                    ctypes.POINTER(ctypes.c_longlong),
+                                                    ctypes.POINTER(ctypes.c_longlong), c_ip, c_dp]),
     "allium_host_p1_matrix": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, c_dp, c_dp,
                                              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                                              c_ip, c_dp]),

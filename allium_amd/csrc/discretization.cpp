@@ -684,14 +684,14 @@ bool assemble_mip(const Mesh &m, int p, const double *sigma_t, const double *sig
 }
 
 bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigma_a, double alpha, MipMatrix &K,
-                 std::string &err)
+                 std::string &err, const BlockIpParams *ip)
 {
-    if (p < 1 || p > 2) { err = "P1SA needs fe_order 1 or 2"; return false; }
+    if (p < 1 || p > 2) { err = "block DSA operators need fe_order 1 or 2"; return false; }
     const int dim = m.dim;
     int n = 1;
     for (int d = 0; d < dim; d++) n *= p + 1;
     const int N = (dim + 1) * n, ns = 2 * dim;
-    if (N > 32) { err = "P1SA blocks are implemented up to 32 unknowns per element (2D p<=2, 3D p=1)"; return false; }
+    if (N > 32) { err = "block DSA operators are implemented up to 32 unknowns per element (2D p<=2, 3D p=1)"; return false; }
     if (!(alpha > 0)) { err = "P1SA needs alpha > 0"; return false; }
     for (int e = 0; e < m.Ne; e++)
         if (!(sigma_t[e] > 0) || !std::isfinite(sigma_a[e])) { err = "P1SA needs sigma_t > 0 and finite sigma_a"; return false; }
@@ -759,7 +759,8 @@ bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigm
                     if (e2 < 0) {
                         const int side = m.f_bside[f], hi = side & 1;
                         const bool marshak = m.bc[side] == 0;
-                        const double beta = marshak ? 1.0 / (2.0 * alpha) : 1.0 / alpha;
+                        if (ip && !marshak) continue;   // the IP blocks have no reflective-side terms
+                        const double beta = ip ? 0.0 : (marshak ? 1.0 / (2.0 * alpha) : 1.0 / alpha);
                         double *Kd = &K.Kd[(size_t)e1 * N * N];
                         for (size_t q = 0; q < r_f.w.size(); q++) {
                             side_point(dim, ax, hi, r_f.x[q].data(), xi[0]);
@@ -774,7 +775,9 @@ bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigm
                                         for (int d = 0; d < dim; d++)
                                             at(Kd, c, i, d, j) += beta * w * area * nh[c] * nh[d] * pp;
                                     if (!marshak) continue;
-                                    at(Kd, dim, i, dim, j) += 0.5 * alpha * w * area * pp;
+                                    // P1: BoundaryMass(alpha/2) and 0.5 psi J.n; IP: BoundaryMass(alpha) only
+                                    at(Kd, dim, i, dim, j) += (ip ? 1.0 : 0.5) * alpha * w * area * pp;
+                                    if (ip) continue;
                                     for (int d = 0; d < dim; d++) {
                                         const double dv = 0.5 * w * Nv[d] * pp;   // 0.5 psi J.n
                                         at(Kd, dim, i, d, j) += dv;
@@ -795,6 +798,17 @@ bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigm
                         B.eval(xi[0], ph[0].data(), dph.data());
                         B.eval(xi[1], ph[1].data(), dph.data());
                         const double w = r_f.w[q];
+                        // interior penalty on phi: P1 alpha/2; IP max(kappa {D} area / det_1, limit)
+                        // (PenaltyIntegrator, src/moment_integrators.cpp:206-287)
+                        double pen = 0.5 * alpha;
+                        if (ip) {
+                            double J1[9], adj1[9];
+                            elem_jacobian(m, e1, xi[0], J1);
+                            const double det1 = det_adj(dim, J1, adj1);
+                            const double Davg = ip->scale_D ? 0.5 * (1.0 / (3.0 * sigma_t[e1]) + 1.0 / (3.0 * sigma_t[e2])) : 1.0;
+                            pen = std::max(Davg * ip->kappa * area / det1, ip->limit);
+                        }
+                        const double jj = ip ? 0.0 : 0.5 / alpha;   // P1's DGVectorJumpJump
                         for (int a = 0; a < 2; a++)
                             for (int bb = 0; bb < 2; bb++) {
                                 const double sab = (a == bb) ? 1.0 : -1.0, sa = a == 0 ? 1.0 : -1.0;
@@ -804,10 +818,11 @@ bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigm
                                 for (int i = 0; i < n; i++)
                                     for (int j = 0; j < n; j++) {
                                         const double pp = ph[a][i] * ph[bb][j];
-                                        at(X, dim, i, dim, j) += sab * 0.5 * alpha * w * area * pp;
-                                        for (int c = 0; c < dim; c++)
-                                            for (int d = 0; d < dim; d++)
-                                                at(X, c, i, d, j) += sab * (0.5 / alpha) * w * area * nh[c] * nh[d] * pp;
+                                        at(X, dim, i, dim, j) += sab * pen * w * area * pp;
+                                        if (jj != 0.0)
+                                            for (int c = 0; c < dim; c++)
+                                                for (int d = 0; d < dim; d++)
+                                                    at(X, c, i, d, j) += sab * jj * w * area * nh[c] * nh[d] * pp;
                                         for (int d = 0; d < dim; d++) {
                                             const double dv = 0.5 * sa * w * Nv[d] * pp;   // [psi]{J.n}
                                             at(X, dim, i, d, j) += dv;
@@ -820,7 +835,7 @@ bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigm
             });
         }
     for (double v : K.Kd)
-        if (!std::isfinite(v)) { err = "non-finite P1 matrix"; return false; }
+        if (!std::isfinite(v)) { err = "non-finite block moment matrix"; return false; }
     return true;
 }
 

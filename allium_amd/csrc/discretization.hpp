@@ -88,11 +88,19 @@ struct MipMatrix {
 };
 bool assemble_mip(const Mesh &m, int p, const double *sigma_t, const double *sigma_a, const MipParams &prm,
                   MipMatrix &K, std::string &err);
+// chive's "mip" block operator (BlockIPDiscretization::GetOperator, src/moment_discretization.cpp:342-398):
+// interior penalty max(kappa {D} area / det_1, limit) with {D} the mean of 1/(3 sigma_t) (scale_D)
+// or 1, the vector mass alone in Mt, BoundaryMass(alpha) and no boundary term in D
+struct BlockIpParams {
+    double kappa;
+    double limit;
+    bool scale_D;
+};
 // chive's "p1sa" operator (P1Discretization::GetOperator, src/moment_discretization.cpp:400-490)
 // in the same element-block storage: per element the unknowns [J_0 (n), .., J_{dim-1} (n), phi (n)],
-// block size K.n = (dim + 1) n; K.P stays empty
+// block size K.n = (dim + 1) n; K.P stays empty.  With ip, the "mip" block operator instead.
 bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigma_a, double alpha, MipMatrix &K,
-                 std::string &err);
+                 std::string &err, const BlockIpParams *ip = nullptr);
 
 // Sweep plan: level of every node e*Nomega+a and the cells grouped by level.
 struct SweepPlan {

@@ -108,6 +108,16 @@ allium::MipParams mip_params(const allium_dsa_desc &d, int p, const allium::Quad
     return m;
 }
 
+// chive's "mip" penalty options (exe/chive.cpp:635-650) from the descriptor
+allium::BlockIpParams block_ip_params(const allium_dsa_desc &d, int p, double alpha)
+{
+    allium::BlockIpParams ip;
+    ip.kappa = d.kappa < 0 ? -d.kappa * (p + 1) * (p + 1) : d.kappa;
+    ip.limit = d.bound_below ? alpha / 2 : 0.0;
+    ip.scale_D = d.scale_penalty != 0;
+    return ip;
+}
+
 // in-place Gauss-Jordan inverse with partial pivoting of a row-major n x n matrix
 bool invert(int n, double *A)
 {
@@ -495,7 +505,15 @@ int add_p1(allium_dsa *d, const double *siga)
     const int Ne = m.Ne, mk = kP1Kdim;
     allium::MipMatrix K;
     std::string why;
-    if (!allium::assemble_p1(m, d->h->ed.p, d->h->sigma.data(), siga, kP1Alpha, K, why)) return fail(ALLIUM_ERR_ARG, why);
+    bool ok_asm;
+    if (d->method == ALLIUM_DSA_BLOCK_MIP) {
+        const double alpha = mip_params(d->desc, d->h->ed.p, d->h->quad).alpha;
+        const allium::BlockIpParams ip = block_ip_params(d->desc, d->h->ed.p, alpha);
+        ok_asm = allium::assemble_p1(m, d->h->ed.p, d->h->sigma.data(), siga, alpha, K, why, &ip);
+    } else {
+        ok_asm = allium::assemble_p1(m, d->h->ed.p, d->h->sigma.data(), siga, kP1Alpha, K, why);
+    }
+    if (!ok_asm) return fail(ALLIUM_ERR_ARG, why);
     const int n = K.n;
     std::vector<double> Dinv(K.Kd);
     std::atomic<bool> ok{true};
@@ -670,7 +688,7 @@ int p1_solve(allium_dsa *d, const double *b, double *x, hipStream_t s)
 // x = K^{-1} b (MIP) or the phi-part of P^{-1} [0; b] (P1SA)
 int inner_solve(allium_dsa *d, const double *b, double *x, hipStream_t s)
 {
-    return d->method == ALLIUM_DSA_P1 ? p1_solve(d, b, x, s) : pcg(d, b, x, s);
+    return d->method == ALLIUM_DSA_MIP ? pcg(d, b, x, s) : p1_solve(d, b, x, s);
 }
 
 }  // namespace
@@ -718,6 +736,7 @@ void allium_dsa_default_desc(allium_dsa_desc *desc)
     desc->cg_maxit = 100;
     desc->smooth = 2;
     desc->method = ALLIUM_DSA_MIP;
+    desc->scale_penalty = 1;
 }
 
 int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_desc *desc, allium_dsa_t *out)
@@ -732,10 +751,10 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
     d->h = h;
     if (desc) d->desc = *desc; else allium_dsa_default_desc(&d->desc);
     if (d->desc.cg_maxit < 1 || d->desc.smooth < 1) return fail(ALLIUM_ERR_ARG, "cg_maxit and smooth must be >= 1");
-    if (d->desc.method != ALLIUM_DSA_MIP && d->desc.method != ALLIUM_DSA_P1)
-        return fail(ALLIUM_ERR_ARG, "method must be ALLIUM_DSA_MIP or ALLIUM_DSA_P1");
+    if (d->desc.method != ALLIUM_DSA_MIP && d->desc.method != ALLIUM_DSA_P1 && d->desc.method != ALLIUM_DSA_BLOCK_MIP)
+        return fail(ALLIUM_ERR_ARG, "method must be ALLIUM_DSA_MIP, ALLIUM_DSA_P1 or ALLIUM_DSA_BLOCK_MIP");
     d->method = d->desc.method;
-    const bool p1 = d->method == ALLIUM_DSA_P1;
+    const bool p1 = d->method != ALLIUM_DSA_MIP;
     const allium::Mesh &m = h->mesh;
     const int Ne = m.Ne, p = h->ed.p;
     std::vector<double> siga(Ne);
@@ -931,6 +950,29 @@ int allium_host_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const all
     allium::MipMatrix K;
     if (!allium::assemble_mip(m, fe_order, sigma_t, siga.data(), mip_params(dd, fe_order, q), K, why))
         return fail(ALLIUM_ERR_ARG, why);
+    return export_csr(K, m.Ne, nnz, row_ptr, col, val);
+}
+
+int allium_host_block_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
+                                 const double *sigma_t, const double *sigma_s, const allium_dsa_desc *desc,
+                                 long long *nnz, long long *row_ptr, int *col, double *val)
+{
+    if (!mesh || !quad || !sigma_t || !sigma_s || !nnz) return fail(ALLIUM_ERR_ARG, "null argument");
+    allium::Mesh m;
+    allium::Quadrature q;
+    std::string why;
+    if (!allium::build_mesh(mesh->dim, mesh->nel, mesh->vertices, mesh->extents, mesh->bc, m, why))
+        return fail(ALLIUM_ERR_ARG, why);
+    int rc = allium_internal_quad(quad, mesh->dim, q);
+    if (rc) return rc;
+    allium_dsa_desc dd;
+    if (desc) dd = *desc; else allium_dsa_default_desc(&dd);
+    std::vector<double> siga(m.Ne);
+    for (int e = 0; e < m.Ne; e++) siga[e] = sigma_t[e] - sigma_s[e];
+    const double alpha = mip_params(dd, fe_order, q).alpha;
+    const allium::BlockIpParams ip = block_ip_params(dd, fe_order, alpha);
+    allium::MipMatrix K;
+    if (!allium::assemble_p1(m, fe_order, sigma_t, siga.data(), alpha, K, why, &ip)) return fail(ALLIUM_ERR_ARG, why);
     return export_csr(K, m.Ne, nnz, row_ptr, col, val);
 }
 

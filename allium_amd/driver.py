@@ -5,7 +5,7 @@ test/chive/tdl.lua) written as YAML or a dict, runs the outer-solve path of
 exe/chive.cpp:550-750 (FormTransportSource, InverseAdvectionOperator,
 SLISolver, GMRESSolver or FGMRESSolver on TransportOperator as
 io::CreateIterativeSolver builds them (src/io.cpp:179-230), optional
-"scalar mip" or "p1sa" DSA preconditioner, final sweep) and reports with chive's YAML keys ("transport iterations" with
+"scalar mip", block "mip" or "p1sa" DSA preconditioner, final sweep) and reports with chive's YAML keys ("transport iterations" with
 it / norm / inner solver, "outer iterations", "inner iteration" max / min /
 avg / total; exe/chive.cpp:40-75,738-746), so scripts that parse chive's
 output read this one too.
@@ -22,8 +22,8 @@ Deck keys (YAML):
       fe_order: 1
       solver: {type: sli, abstol: 1.0e-6, reltol: 0, max_iter: 50}   # or gmres / fgmres with kdim
       preconditioner: {type: scalar mip, kappa: 4, solver: {type: cg, reltol: 1.0e-10, max_iter: 200}}
-      # or {type: p1sa, solver: {type: direct}}; a "direct" inner solve is CG
-      # (scalar mip) or GMRES (p1sa) to a relative residual of 1e-12
+      # or {type: mip} (the block system) / {type: p1sa, solver: {type: direct}}; a "direct"
+      # inner solve is CG (scalar mip) or GMRES (mip, p1sa) to a relative residual of 1e-12
 
 Material `source` and boundary `value` are per-steradian isotropic densities,
 as in chive (exe/chive.cpp:236-247,294-306).  Boundary type "reflective"
@@ -95,7 +95,7 @@ def run(deck):
     prec = drv.get("preconditioner")
     ptype = str(prec.get("type", "")).lower() if prec else None
     if prec and ptype not in ("scalar mip", "mip", "p1sa"):
-        raise ValueError(f"preconditioner {prec.get('type')} is not supported (scalar mip, p1sa)")
+        raise ValueError(f"preconditioner {prec.get('type')} is not supported (scalar mip, mip, p1sa)")
     # chive's inner solver defaults to "direct"; p1sa takes nothing else (exe/chive.cpp:502-524,609)
     itype = str(prec.get("solver", {}).get("type", "direct")).lower() if prec else None
     if ptype == "p1sa" and itype not in ("direct", "superlu"):
@@ -113,6 +113,21 @@ def run(deck):
         # the direct solve is emulated by GMRES to a relative residual of 1e-12
         accel = A.DSA(Linv, sigma_s, method="p1sa", cg_reltol=1e-12, cg_maxit=20000)
         out["preconditioner"] = {"type": "p1sa", "solver": {"type": itype}}
+    elif ptype == "mip":
+        # chive's block interior-penalty system (exe/chive.cpp:635-662); the reference
+        # solves its Schur complement (CG + BoomerAMG or direct), here GMRES solves the
+        # block system, whose phi-part is that solution, to a relative residual of
+        # the inner reltol (1e-12 for "direct")
+        inner = prec.get("solver", {})
+        rt = 1e-12 if itype in ("direct", "superlu") else float(inner.get("reltol", 1e-10))
+        kappa = float(prec.get("kappa", (p + 1) ** 2))
+        scale = bool(prec.get("scale_stabilization", True))
+        lower = bool(prec.get("bound_stabilization_below", True))
+        accel = A.DSA(Linv, sigma_s, method="block mip", kappa=kappa, bound_below=lower, scale_penalty=scale,
+                      cg_reltol=rt, cg_abstol=float(inner.get("abstol", 0.0)),
+                      cg_maxit=int(inner.get("max_iter", 20000)) if itype not in ("direct", "superlu") else 20000)
+        out["preconditioner"] = {"type": "mip", "kappa": kappa, "scale stabilization": scale,
+                                 "bound stabilization from below": lower, "solver": {"type": itype}}
     elif prec:
         inner = prec.get("solver", {})
         if itype in ("direct", "superlu"):

@@ -21,7 +21,9 @@
  * vertex-grid coarsening by two down to a dense coarsest solve.
  * One group only (chive's steady one-group SI).
  *
- * desc.method = ALLIUM_DSA_P1 selects chive's "p1sa" preconditioner instead
+ * desc.method = ALLIUM_DSA_BLOCK_MIP selects chive's "mip" (the block interior-penalty
+ * system whose Schur complement the reference solves), and ALLIUM_DSA_P1 chive's
+ * "p1sa" preconditioner
  * (exe/chive.cpp:608-619): y = x + (P^{-1} [0; Ms x])_phi with P the 2x2 P1
  * system of allium_host_p1_matrix, which the reference solves directly.  Here
  * GMRES solves it, preconditioned by block-Jacobi around one V-cycle of the
@@ -51,11 +53,17 @@ typedef struct {
     int method;         /* ALLIUM_DSA_MIP, or ALLIUM_DSA_P1: chive's "p1sa" (exe/chive.cpp:608-619),
                            the 2x2 P1 system of allium_host_p1_matrix solved by GMRES(40) with
                            element block-Jacobi from the right, to ||r|| <= max(cg_reltol ||b||,
-                           cg_abstol) in at most cg_maxit steps; kappa, alpha, sym, smooth unused */
+                           cg_abstol) in at most cg_maxit steps; kappa, alpha, sym, smooth unused;
+                           or ALLIUM_DSA_BLOCK_MIP: chive's "mip" (exe/chive.cpp:635-662), the 2x2
+                           block system of allium_host_block_mip_matrix (kappa, bound_below, alpha,
+                           scale_penalty as for it), solved like ALLIUM_DSA_P1 */
+    int scale_penalty;  /* ALLIUM_DSA_BLOCK_MIP: penalty scaled by the mean of 1/(3 sigma_t)
+                           (chive's scale_stabilization, default 1) */
 } allium_dsa_desc;
 
 #define ALLIUM_DSA_MIP 0
 #define ALLIUM_DSA_P1 1
+#define ALLIUM_DSA_BLOCK_MIP 2
 
 typedef struct {
     int levels;                 /* DG level + vertex-grid levels */
@@ -70,7 +78,7 @@ typedef struct {
 } allium_dsa_info;
 
 /* defaults: kappa -1 ((p+1)^2), bound_below 1, alpha 0 (ComputeAlpha), sym -1,
- * cg_reltol 1e-6, cg_abstol 0, cg_maxit 100, smooth 2, method ALLIUM_DSA_MIP */
+ * cg_reltol 1e-6, cg_abstol 0, cg_maxit 100, smooth 2, method ALLIUM_DSA_MIP, scale_penalty 1 */
 void allium_dsa_default_desc(allium_dsa_desc *desc);
 
 /* Build K on the mesh, quadrature and sigma_t of the sweep handle (G == 1)
@@ -99,6 +107,13 @@ int allium_host_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const all
  * ((dim + 1) Ndof rows).  2D p = 1, 2 and 3D p = 1. */
 int allium_host_p1_matrix(const allium_mesh_desc *mesh, int fe_order, const double *sigma_t, const double *sigma_s,
                           long long *nnz, long long *row_ptr, int *col, double *val);
+/* Host only: chive's "mip" block operator, BlockIPDiscretization::GetOperator
+ * (src/moment_discretization.cpp:342-398) with the penalty options of desc
+ * (kappa, bound_below, alpha, scale_penalty), in allium_host_p1_matrix's
+ * layout.  Its Schur complement on phi is what the reference solves. */
+int allium_host_block_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
+                                 const double *sigma_t, const double *sigma_s, const allium_dsa_desc *desc,
+                                 long long *nnz, long long *row_ptr, int *col, double *val);
 
 #ifdef __cplusplus
 }

@@ -247,6 +247,45 @@ def p1_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc=None, alpha=0.5):
            on Marshak sides (:289-388);  G = -D^T.
     alpha is P1Discretization's 0.5 (chive does not set it for p1sa,
     exe/chive.cpp:608-619).  Volume rule order 2p + dim - 1, face rule 2p."""
+    return _block_moment_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc, alpha, None)
+
+
+def block_ip_matrix(dim, nel, vertices, p, sigma_t, sigma_s, omega, weights, bc=None, kappa=None,
+                    bound_below=True, scale=True):
+    """chive's "mip" operator, the 2x2 block system of BlockIPDiscretization::GetOperator
+    (src/moment_discretization.cpp:342-398) as exe/chive.cpp:635-650 sets it up, in
+    p1_matrix's layout:
+      Mt = VectorMass(3 sigma_t) (no face terms);
+      Ma = Mass(sigma_a) + PenaltyIntegrator on interior faces (src/moment_integrators.cpp:206-287):
+           weight w_q |N| max({D} kappa |N| / det J_1, limit), {D} the mean of 1/(3 sigma_t)
+           (scale) or 1, limit alpha/2 (bound_below) or 0, kappa (p+1)^2 by default;
+           + BoundaryMass(alpha) on Marshak sides, alpha = ComputeAlpha;
+      D  = -(grad psi, J) + DGJumpAverage on interior faces;  G = -D^T.
+    The reference solves its Schur complement Ma + D Mt^-1 D^T (FormSchurComplement,
+    :248-264), whose solution is the phi-part of the block solve with [0; b]."""
+    alpha = compute_alpha(omega, weights)
+    ip = {"kappa": float((p + 1) ** 2 if kappa is None else kappa),
+          "limit": alpha / 2 if bound_below else 0.0, "scale": bool(scale)}
+    return _block_moment_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc, alpha, ip)
+
+
+def schur_complement(Pb, dim, n):
+    """Ma + D Mt^-1 D^T of a block matrix whose Mt is block diagonal by element
+    (BlockMomentDiscretization::FormSchurComplement), over the phi unknowns."""
+    Pb = sp.csr_matrix(Pb)
+    N = (dim + 1) * n
+    Ne = Pb.shape[0] // N
+    cur = np.concatenate([np.arange(e * N, e * N + dim * n) for e in range(Ne)])
+    phi = np.concatenate([np.arange(e * N + dim * n, (e + 1) * N) for e in range(Ne)])
+    Mt = Pb[cur][:, cur].toarray()
+    G = Pb[cur][:, phi]
+    D = Pb[phi][:, cur]
+    Ma = Pb[phi][:, phi]
+    return (Ma - D @ sp.csr_matrix(np.linalg.inv(Mt)) @ G).tocsr()
+
+
+def _block_moment_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc, alpha, ip):
+    """p1_matrix (ip None) or block_ip_matrix (ip the penalty parameters)."""
     geo = _Geometry(dim, nel, vertices)
     sigma_t = np.asarray(sigma_t, float).reshape(-1)
     sigma_a = sigma_t - np.asarray(sigma_s, float).reshape(-1)
@@ -298,6 +337,10 @@ def p1_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc=None, alpha=0.5):
                     area = np.linalg.norm(Nn, axis=1)
                     nh = Nn / area[:, None]
                     ph, _ = _basis(dim, p, xi)
+                    if ip is not None:
+                        if bc[2 * ax + hi] == 0:
+                            add(idx(e1, dim), idx(e1, dim), alpha * np.einsum("q,qi,qj->ij", wf * area, ph, ph))
+                        continue
                     beta = 1.0 / alpha if bc[2 * ax + hi] != 0 else 1.0 / (2 * alpha)
                     for c in range(dim):
                         for d in range(dim):
@@ -320,14 +363,21 @@ def p1_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc=None, alpha=0.5):
                 ph1, _ = _basis(dim, p, xi1)
                 ph2, _ = _basis(dim, p, xi2)
                 jmp = [(e1, ph1, 1.0), (e2, ph2, -1.0)]
+                if ip is None:
+                    pen = np.full(len(wf), 0.5 * alpha)
+                else:
+                    det1 = np.linalg.det(geo.jac(e1, xi1))
+                    Davg = 0.5 * (1 / (3 * sigma_t[e1]) + 1 / (3 * sigma_t[e2])) if ip["scale"] else 1.0
+                    pen = np.maximum(Davg * ip["kappa"] * area / det1, ip["limit"])
                 for ea, pa, sa in jmp:
                     for eb, pb, sb in jmp:
-                        base = sa * sb * np.einsum("q,qi,qj->ij", wf * area, pa, pb)
-                        add(idx(ea, dim), idx(eb, dim), 0.5 * alpha * base)
-                        for c in range(dim):
-                            for d in range(dim):
-                                add(idx(ea, c), idx(eb, d), (0.5 / alpha) * sa * sb *
-                                    np.einsum("q,qi,qj->ij", wf * area * nh[:, c] * nh[:, d], pa, pb))
+                        base = sa * sb * np.einsum("q,qi,qj->ij", wf * area * pen, pa, pb)
+                        add(idx(ea, dim), idx(eb, dim), base)
+                        if ip is None:
+                            for c in range(dim):
+                                for d in range(dim):
+                                    add(idx(ea, c), idx(eb, d), (0.5 / alpha) * sa * sb *
+                                        np.einsum("q,qi,qj->ij", wf * area * nh[:, c] * nh[:, d], pa, pb))
                         # [psi]{J.n}: test side sign sa, trial average 1/2
                         for d in range(dim):
                             Df = 0.5 * sa * np.einsum("q,qi,qj->ij", wf * Nn[:, d], pa, pb)

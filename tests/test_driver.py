@@ -61,14 +61,15 @@ def test_tdl_deck_outer_iterations():
 def test_tdl_deck_direct_inner_solver(prec):
     """test/chive/CMakeLists.txt:117-121: the TDL deck with
     `preconditioner = {type = "<dsa>", solver = {type = "direct"}}` for p1sa
-    and mip needs fewer than 12 outer iterations."""
+    and mip (the block interior-penalty system) needs fewer than 12 outer
+    iterations."""
     print(f"driver tdl {prec} direct", flush=True)
     d = _deck()
     d["driver"]["preconditioner"] = {"type": prec, "solver": {"type": "direct"}}
     out = driver.run(d)
     print(yaml.safe_dump({k: out[k] for k in ("outer iterations", "inner iteration", "preconditioner")}))
     assert out["converged"] and out["outer iterations"] < 12
-    assert out["preconditioner"]["type"] == ("p1sa" if prec == "p1sa" else "scalar mip")
+    assert out["preconditioner"]["type"] == prec
 
 
 @pytest.mark.gpu

@@ -342,3 +342,123 @@ def test_host_p1_assembly_matches_oracle(dim, p, jitter):
     Ph = sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2).toarray()
     Po = M.p1_matrix(dim, nel, vert, p, st, ss, bc=bc).toarray()
     assert np.abs(Ph - Po).max() < 1e-12 * np.abs(Po).max()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_block_ip_blocks_on_elementwise_constants(dim):
+    """chive's "mip" block operator (oracle block_ip_matrix) on a uniform mesh,
+    p = 1, all sides Marshak, element-wise constant currents c_e and fluxes f_e.
+    For node i of element e on a side with outward normal n (axis a), node face
+    integral A and h the element width (area / det J = 1/h):
+      Mt row:  3 sigma_t vol c_e only (no face terms in BlockIPDiscretization);
+      D row:   A (c_nbr - c_e).n / 2 on interior sides, -A c_e.n on Marshak sides
+               (integration by parts; no boundary term in D);
+      Ma row:  sigma_a vol f_e + pen A (f_e - f_nbr) on interior sides with
+               pen = max(kappa (D_e + D_nbr)/2 / h, alpha/2), D = 1/(3 sigma_t),
+               + alpha A f_e on Marshak sides;
+      G = -D^T.  sigma_t spans both branches of the penalty's max."""
+    n_el = 3
+    nel = (n_el,) * dim
+    vert = P.cartesian_vertices(nel)
+    rng = np.random.default_rng(21)
+    ne = n_el ** dim
+    st = np.exp(rng.uniform(np.log(0.5), np.log(300.0), ne))
+    ss = st * rng.uniform(0.1, 0.9, ne)
+    om, w = ls_fixture(4, dim)
+    alpha = M.compute_alpha(om, w)
+    kappa = 4.0
+    Pm = M.block_ip_matrix(dim, nel, vert, 1, st, ss, om, w).toarray()
+    n, N = 2 ** dim, (dim + 1) * 2 ** dim
+    h = 1.0 / n_el
+    vol, area = h ** dim / n, h ** (dim - 1) / 2 ** (dim - 1)
+    Dc = 1.0 / (3.0 * st)
+    C = rng.normal(size=(ne, dim))
+    F = rng.normal(size=ne)
+    x = np.zeros((ne, dim + 1, n))
+    x[:, :dim, :] = C[:, :, None]
+    x[:, dim, :] = F[:, None]
+    y = (Pm @ x.ravel()).reshape(ne, dim + 1, n)
+    xf = np.zeros((ne, dim + 1, n))
+    xf[:, dim, :] = F[:, None]
+    yf = (Pm @ xf.ravel()).reshape(ne, dim + 1, n)
+    branches = set()
+    for e in range(ne):
+        co = [(e // n_el ** d) % n_el for d in range(dim)]
+        Dr = np.zeros(n)
+        Ma = (st[e] - ss[e]) * vol * F[e] * np.ones(n)
+        for ax in range(dim):
+            for hi in (0, 1):
+                mask = np.array([((i >> ax) & 1) == hi for i in range(n)])
+                sgn = 1.0 if hi else -1.0
+                if co[ax] == (n_el - 1) * hi:
+                    Dr[mask] += -area * sgn * C[e][ax]
+                    Ma[mask] += alpha * area * F[e]
+                else:
+                    o = e + (1 if hi else -1) * n_el ** ax
+                    Dr[mask] += 0.5 * area * sgn * (C[o][ax] - C[e][ax])
+                    val = kappa * 0.5 * (Dc[e] + Dc[o]) / h
+                    branches.add(val > alpha / 2)
+                    Ma[mask] += max(val, alpha / 2) * area * (F[e] - F[o])
+        np.testing.assert_allclose(y[e, dim], Dr + Ma, atol=1e-13)
+        Mt = 3 * st[e] * vol * np.repeat(C[e][:, None], n, 1)
+        np.testing.assert_allclose(y[e, :dim] - yf[e, :dim], Mt, rtol=1e-13, atol=1e-13)
+    assert branches == {True, False}
+    jdx = np.concatenate([np.arange(e * N, e * N + dim * n) for e in range(ne)])
+    pdx = np.concatenate([np.arange(e * N + dim * n, (e + 1) * N) for e in range(ne)])
+    np.testing.assert_allclose(Pm[np.ix_(jdx, pdx)], -Pm[np.ix_(pdx, jdx)].T, atol=1e-16)
+
+
+def test_block_ip_schur_complement_is_the_block_solve():
+    """FormSchurComplement (src/moment_discretization.cpp:248-264): the Schur
+    complement Ma + D Mt^-1 D^T is symmetric positive definite and its solve
+    equals the phi-part of the block solve with [0; b]; with scale_penalty off
+    the penalty is kappa / h (times 1) instead."""
+    nel = (5, 4)
+    rng = np.random.default_rng(2)
+    vert = P.jittered_vertices(nel, 0.15, rng)
+    st = P.lognormal_sigma(20, 1.0, rng) * 3.0
+    ss = st * 0.95
+    om, w = ls_fixture(4, 2)
+    Pm = M.block_ip_matrix(2, nel, vert, 1, st, ss, om, w)
+    S = M.schur_complement(Pm, 2, 4).toarray()
+    np.testing.assert_allclose(S, S.T, atol=1e-12 * np.abs(S).max())
+    assert np.linalg.eigvalsh(S).min() > 0
+    b = rng.normal(size=80)
+    np.testing.assert_allclose(M.p1_solver(Pm, 2, 4)(b), np.linalg.solve(S, b), rtol=1e-9)
+    P0 = M.block_ip_matrix(2, nel, vert, 1, st, ss, om, w, scale=False)
+    assert abs((P0 - Pm).toarray()).max() > 1e-3
+
+
+def test_block_ip_accelerates_the_tdl_deck():
+    """chive.tdl.mip (test/chive/CMakeLists.txt:117-121, test/chive/test_tdl.py):
+    under 12 iterations with the block "mip" preconditioner, at the transport
+    fixed point."""
+    nel, vert, om, w, st, ss, eps = _tdl()
+    pr = oracle.Problem(2, nel, vert, 1, om, w, st)
+    q = pr.form_source(Q=eps)
+    Pm = M.block_ip_matrix(2, nel, vert, 1, st, ss, om, w)
+    phi, it, norms = M.dsa_source_iteration(pr, ss, q, None, abstol=1e-6, maxit=50, solve=M.p1_solver(Pm, 2, 4))
+    assert norms[-1] < 1e-6 and it < 12
+    b = pr.D(pr.sweep(q))
+    r = b - phi + pr.D(pr.sweep(pr.DT(pr.mass(ss, phi))))
+    assert np.linalg.norm(r) < 1e-5 * np.linalg.norm(phi)
+
+
+@pytest.mark.parametrize("dim,p,jitter,scale", [(2, 1, 0.0, True), (2, 1, 0.2, True), (2, 2, 0.1, False),
+                                                (3, 1, 0.0, True), (3, 1, 0.15, True)])
+def test_host_block_mip_assembly_matches_oracle(dim, p, jitter, scale):
+    """allium_host_block_mip_matrix (C++) against the numpy restatement, with
+    one reflective side."""
+    n = 4 if dim == 2 else 3
+    nel = (n,) * dim
+    rng = np.random.default_rng(13)
+    vert = P.jittered_vertices(nel, jitter, rng) if jitter else P.cartesian_vertices(nel)
+    st = np.exp(rng.uniform(np.log(0.5), np.log(100.0), n ** dim))
+    ss = st * rng.uniform(0.5, 0.999, n ** dim)
+    bc = [0, 1] + [0] * (2 * dim - 2)
+    om, w = ls_fixture(4, dim)
+    mesh = A.Mesh(nel, vertices=vert, bc=bc)
+    rp, col, val = A.host_block_mip_matrix(mesh, p, (om, w), st, ss, scale_penalty=scale)
+    Ph = sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2).toarray()
+    Po = M.block_ip_matrix(dim, nel, vert, p, st, ss, om, w, bc=bc, scale=scale).toarray()
+    np.testing.assert_allclose(Ph, Po, rtol=1e-12, atol=1e-12 * np.abs(Po).max())

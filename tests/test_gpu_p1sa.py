@@ -1,7 +1,9 @@
-"""GPU checks of chive's "p1sa" preconditioner (DSA(method="p1sa"),
-exe/chive.cpp:608-619): the 2x2 P1 system of P1Discretization::GetOperator
-(src/moment_discretization.cpp:400-490), solved on the GPU by GMRES with
-element block-Jacobi where the reference solves it directly.
+"""GPU checks of chive's block DSA preconditioners: "p1sa" (DSA(method="p1sa"),
+exe/chive.cpp:608-619), the 2x2 P1 system of P1Discretization::GetOperator
+(src/moment_discretization.cpp:400-490), and "mip" (DSA(method="block mip"),
+exe/chive.cpp:635-662), the block interior-penalty system whose Schur
+complement the reference solves.  The GPU solves both block systems by GMRES
+preconditioned with element block-Jacobi around a MIP V-cycle.
 
 The solve is compared with a direct sparse solve of the host-assembled P1
 matrix (itself checked against the oracle's p1_matrix in test_dsa_oracle.py);
@@ -169,3 +171,50 @@ def test_p1sa_errors():
         A.DSA(A.InverseAdvectionOperator(mesh, 0, 4, st), st * 0.9, method="p1sa")
     with pytest.raises(KeyError):
         A.DSA(A.InverseAdvectionOperator(mesh, 1, 4, st), st * 0.9, method="ldgsa")
+
+
+BLOCK_CASES = [c for c in CASES if c[2] == 1]
+
+
+@pytest.mark.parametrize("case", BLOCK_CASES, ids=[f"{c[0]}d-{'x'.join(map(str, c[1]))}" for c in BLOCK_CASES])
+def test_block_mip_solve_matches_schur_direct(case):
+    """chive's block "mip" (DSA(method="block mip")): the phi-part of the GPU
+    block solve equals a direct solve of the Schur complement Ma + D Mt^-1 D^T
+    that the reference forms (FormSchurComplement, src/moment_discretization.cpp:248-264)."""
+    dim, nel, p = case[:3]
+    print(f"block mip solve {case}", flush=True)
+    mesh, vert, bc, st, ss, om, w = _setup(*case)
+    Linv = A.InverseAdvectionOperator(mesh, p, (om, w), st)
+    dsa = A.DSA(Linv, ss, method="block mip", cg_reltol=1e-12, cg_maxit=3000)
+    b = np.random.default_rng(6).normal(size=Linv.phi_size)
+    x = dsa.solve(b)
+    rp, col, val = A.host_block_mip_matrix(mesh, p, (om, w), st, ss)
+    S = M.schur_complement(sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2), dim, (p + 1) ** dim)
+    ref = spla.spsolve(sp.csc_matrix(S), b)
+    info = dsa.info()
+    print(f"  GMRES its {info['last_cg_iterations']} rel {_rel(x, ref):.2e}")
+    assert info["last_cg_iterations"] < 3000 and _rel(x, ref) < 1e-9
+
+
+def test_tdl_deck_with_block_mip():
+    """chive.tdl.mip (test/chive/CMakeLists.txt:117-121): under 12 iterations,
+    the oracle's iterations (Schur complement solved directly there) and flux."""
+    print("block mip tdl", flush=True)
+    eps = 1e-4
+    nel = (8, 8)
+    vert = P.cartesian_vertices(nel)
+    om, w = ls_fixture(4, 2)
+    st, ss = np.full(64, 1 / eps), np.full(64, 1 / eps - eps)
+    Linv = A.InverseAdvectionOperator(A.Mesh(nel, vertices=vert), 1, (om, w), st)
+    q = Linv.form_source(Q=eps)
+    dsa = A.DSA(Linv, ss, method="block mip", cg_reltol=1e-13, cg_maxit=3000)
+    phi, _, it, norms = A.source_iteration(Linv, ss, q, abstol=1e-6, maxit=50, want_psi=False, accel=dsa)
+    pr = oracle.Problem(2, nel, vert, 1, om, w, st)
+    Pm = M.block_ip_matrix(2, nel, vert, 1, st, ss, om, w)
+    phi_o, it_o, norms_o = M.dsa_source_iteration(pr, ss, pr.form_source(Q=eps), None, abstol=1e-6, maxit=50,
+                                                  solve=M.p1_solver(Pm, 2, 4))
+    print(f"  GPU {it} its, oracle {it_o}; GMRES its per application "
+          f"{dsa.info()['cg_iterations'] / max(1, dsa.info()['applications']):.1f}")
+    assert it < 12 and it == it_o
+    np.testing.assert_allclose(norms, norms_o, rtol=1e-3, atol=1e-10 * norms_o[0])
+    assert _rel(phi, phi_o) < 1e-8
