@@ -194,11 +194,12 @@ __global__ void __launch_bounds__(64) k_linearize(TrtParams P, const double *__r
     }
 }
 
-// out[g][e] (+)= scale * dB_g u_e with u_e = Binv_e (src_e), src in T space
-// mode 0: src = u directly; mode 1: src = sum_g sigma_g Mu phi_g (Ms phi)
-__global__ void __launch_bounds__(64) k_apply_dB(TrtParams P, int mode, const double *__restrict__ x,
-                                                 const double *__restrict__ Binv, const double *__restrict__ dpw,
-                                                 double scale, int accumulate, double *__restrict__ out)
+// out[g][e] (+)= scale * dB_g u_e with u_e = Binv_e (src_e), src in T space,
+// in two kernels.  Per element: u_e and its values at the rule-3 points,
+// UQ[e][q] = phi3_q . u_e; mode 0: src = x directly; mode 1: src = sum_g
+// sigma_g Mu phi_g (Ms phi)
+__global__ void __launch_bounds__(64) k_dB_points(TrtParams P, int mode, const double *__restrict__ x,
+                                                  const double *__restrict__ Binv, double *__restrict__ UQ)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= P.Ne) return;
@@ -224,22 +225,29 @@ __global__ void __launch_bounds__(64) k_apply_dB(TrtParams P, int mode, const do
     const double *B = Binv + (long long)e * N * N;
 #pragma unroll
     for (int i = 0; i < N; ++i) u[i] = dot_basis(B + i * N, t);
-    double uq[32];
-    for (int q = 0; q < P.nq3; ++q) uq[q] = dot_basis(P.phi3 + q * N, u);
-    for (int g = 0; g < P.G; ++g) {
-        double *og = out + ((long long)g * P.Ne + e) * N;
-        double v[N];
+    for (int q = 0; q < P.nq3; ++q) UQ[(long long)e * P.nq3 + q] = dot_basis(P.phi3 + q * N, u);
+}
+
+// per (element, group), group fastest: out[g][e][i] (+)= scale sum_q dpw[e][q][g] UQ[e][q] phi3_q[i]
+__global__ void __launch_bounds__(256) k_dB_groups(TrtParams P, const double *__restrict__ UQ,
+                                                   const double *__restrict__ dpw, double scale, int accumulate,
+                                                   double *__restrict__ out)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)P.Ne * P.G) return;
+    const int e = (int)(t / P.G), g = (int)(t - (long long)e * P.G);
+    double v[N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) v[i] = 0.0;
-        for (int q = 0; q < P.nq3; ++q) {
-            const double s = dpw[((long long)e * P.nq3 + q) * P.G + g] * uq[q];
-            const double *ph = P.phi3 + q * N;
+    for (int i = 0; i < N; ++i) v[i] = 0.0;
+    for (int q = 0; q < P.nq3; ++q) {
+        const double sq = dpw[((long long)e * P.nq3 + q) * P.G + g] * UQ[(long long)e * P.nq3 + q];
+        const double *ph = P.phi3 + q * N;
 #pragma unroll
-            for (int i = 0; i < N; ++i) v[i] = fma(s, ph[i], v[i]);
-        }
-#pragma unroll
-        for (int i = 0; i < N; ++i) og[i] = accumulate ? fma(scale, v[i], og[i]) : scale * v[i];
+        for (int i = 0; i < N; ++i) v[i] = fma(sq, ph[i], v[i]);
     }
+    double *og = out + ((long long)g * P.Ne + e) * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) og[i] = accumulate ? fma(scale, v[i], og[i]) : scale * v[i];
 }
 
 // T += Binv (sum_g sigma_g Mu phi_g + resid), floored at 1e-8
@@ -403,6 +411,7 @@ struct allium_trt {
     double *sigma = nullptr, *em = nullptr, *phi = nullptr, *phisrc = nullptr, *phit = nullptr, *iso = nullptr;
     double *resid = nullptr, *Binv = nullptr, *dpw = nullptr, *part = nullptr, *dmid = nullptr;
     double *W4 = nullptr, *E3 = nullptr;   // Planck weights per quadrature point (k_planck_points)
+    double *UQ = nullptr;                  // Binv-weighted source at the rule-3 points (k_dB_points)
     int *flags = nullptr;
     hipStream_t s = nullptr;
     ~allium_trt()
@@ -554,6 +563,7 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     TRT_TRY(h->alloc(&h->dpw, (size_t)h->Ne * P.nq3 * h->G));
     TRT_TRY(h->alloc(&h->E3, (size_t)h->Ne * P.nq3 * h->G));
     TRT_TRY(h->alloc(&h->W4, (size_t)h->Ne * P.nq4 * 2));
+    TRT_TRY(h->alloc(&h->UQ, (size_t)h->Ne * P.nq3));
     TRT_TRY(h->alloc(&h->psi, h->psi_size));
     TRT_TRY(h->alloc(&h->qpsi, h->psi_size));
     TRT_TRY(h->alloc(&h->em, h->phi_size));
@@ -605,6 +615,18 @@ void allium_trt_destroy(allium_trt_t h) { delete h; }
 
 allium_sweep_t allium_trt_sweep(allium_trt_t h) { return h ? h->L : nullptr; }
 
+namespace {
+// out (+)= scale dB Binv (src): k_dB_points then k_dB_groups on the handle's stream
+void apply_dB(allium_trt *h, int mode, const double *x, double scale, int accumulate, double *out)
+{
+    const TrtParams &P = h->P;
+    hipLaunchKernelGGL(k_dB_points, dim3((h->Ne + 63) / 64), dim3(64), 0, h->s, P, mode, x, h->Binv, h->UQ);
+    const long long items = (long long)h->Ne * P.G;
+    hipLaunchKernelGGL(k_dB_groups, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, h->s, P, h->UQ, h->dpw,
+                       scale, accumulate, out);
+}
+}  // namespace
+
 int allium_trt_step(allium_trt_t h, allium_trt_info *info)
 {
     if (!h) return tfail(ALLIUM_ERR_ARG, "null handle");
@@ -649,7 +671,7 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
         hipLaunchKernelGGL(k_linearize, dim3(eb), dim3(64), 0, s, P, h->T, h->Tn, h->W4, h->E3, h->resid, h->Binv,
                            h->em, h->flags);
         // em += dB Bt'^{-1} resid
-        hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 0, h->resid, h->Binv, h->dpw, 1.0, 1, h->em);
+        apply_dB(h, 0, h->resid, 1.0, 1, h->em);
         TRT_TRY(hipGetLastError());
         int bad = 0;
         TRT_TRY(hipMemcpyAsync(&bad, h->flags, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -665,8 +687,7 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
         else TRT_TRY(hipMemsetAsync(h->phi, 0, sizeof(double) * h->phi_size, s));
         double r0 = 0.0;
         for (it = 1;; it++) {
-            hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, inv_w, 0,
-                               h->iso);
+            apply_dB(h, 1, h->phi, inv_w, 0, h->iso);
             TRT_TRY(hipGetLastError());
             ABI_TRY(allium_sweep_mult_moments_device(h->L, nullptr, h->iso, h->psi, h->phit, s));
             TRT_TRY(allium::launch_si_update(h->phi_size, h->phisrc, h->phit, h->phi, h->part, s));
@@ -680,7 +701,7 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
             if (nrm < r0 || it >= h->d.si_maxit) break;
         }
         // em += Ms phi ; psi = Linv(D^T em + q_psi), with the fixup back on
-        hipLaunchKernelGGL(k_apply_dB, dim3(eb), dim3(64), 0, s, P, 1, h->phi, h->Binv, h->dpw, 1.0, 1, h->em);
+        apply_dB(h, 1, h->phi, 1.0, 1, h->em);
         TRT_TRY(allium::launch_axpby(h->phi_size, inv_w, h->em, 0.0, h->em, h->iso, s));
         if (fix_on) ABI_TRY(allium_sweep_use_fixup(h->L, 1));
         ABI_TRY(allium_sweep_mult_iso_device(h->L, h->qpsi, h->iso, h->psi, s));
