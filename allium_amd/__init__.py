@@ -269,6 +269,21 @@ class InverseAdvectionOperator:
     def SetTimeAbsorption(self, tau):
         check(self._lib.allium_sweep_set_time_absorption(self._h, float(tau)))
 
+    def GetTimeAbsorption(self):
+        tau = ctypes.c_double()
+        check(self._lib.allium_sweep_get_time_absorption(self._h, ctypes.byref(tau)))
+        return tau.value
+
+    def GetLumpingType(self):
+        lump, pbj = ctypes.c_int(), ctypes.c_int()
+        check(self._lib.allium_sweep_get_lumping(self._h, ctypes.byref(lump), ctypes.byref(pbj)))
+        return lump.value
+
+    def IsParallelBlockJacobi(self):
+        lump, pbj = ctypes.c_int(), ctypes.c_int()
+        check(self._lib.allium_sweep_get_lumping(self._h, ctypes.byref(lump), ctypes.byref(pbj)))
+        return bool(pbj.value)
+
     def set_precision(self, precision):
         check(self._lib.allium_sweep_set_precision(self._h, precision))
 

@@ -70,6 +70,8 @@ SIGNATURES = {
     "allium_sweep_quadrature": (ctypes.c_int, [c_vp, c_dp, c_dp]),
     "allium_sweep_set_sigma": (ctypes.c_int, [c_vp, c_dp]),
     "allium_sweep_set_time_absorption": (ctypes.c_int, [c_vp, ctypes.c_double]),
+    "allium_sweep_get_time_absorption": (ctypes.c_int, [c_vp, c_dp]),
+    "allium_sweep_get_lumping": (ctypes.c_int, [c_vp, c_ip, c_ip]),
     "allium_sweep_set_precision": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "allium_sweep_set_launch": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "allium_sweep_set_tuning": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),

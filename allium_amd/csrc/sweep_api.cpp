@@ -599,6 +599,21 @@ int allium_sweep_set_time_absorption(allium_sweep_t h, double tau)
     return ALLIUM_OK;
 }
 
+int allium_sweep_get_time_absorption(allium_sweep_t h, double *tau)
+{
+    if (!h || !tau) return fail(ALLIUM_ERR_ARG, "null argument");
+    *tau = h->ds.prm.timedep ? h->ds.prm.tau : 0.0;
+    return ALLIUM_OK;
+}
+
+int allium_sweep_get_lumping(allium_sweep_t h, int *lump, int *parallel_block_jacobi)
+{
+    if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
+    if (lump) *lump = 0;
+    if (parallel_block_jacobi) *parallel_block_jacobi = 0;
+    return ALLIUM_OK;
+}
+
 int allium_sweep_set_precision(allium_sweep_t h, int precision)
 {
     if (!h || (precision != ALLIUM_FP64 && precision != ALLIUM_FP32_REFINED))

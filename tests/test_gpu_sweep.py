@@ -496,3 +496,14 @@ def test_sweep_timing_log_counts_every_sweep():
     assert t1 > t0 and (t1 - t0) / 150 < 0.1
     t2, n2 = Linv.timing()
     assert (t2, n2) == (t1, n1)
+
+
+def test_operator_getters():
+    """GetTimeAbsorption / GetLumpingType / IsParallelBlockJacobi
+    (src/sweep.hpp:105,109,127): 0 until SetTimeAbsorption, no lumping, no PBJ."""
+    print("operator getters", flush=True)
+    Linv = A.InverseAdvectionOperator(A.Mesh((3, 3, 3)), 1, 4, np.ones(27))
+    assert Linv.GetTimeAbsorption() == 0.0
+    Linv.SetTimeAbsorption(2.5)
+    assert Linv.GetTimeAbsorption() == 2.5
+    assert Linv.GetLumpingType() == 0 and not Linv.IsParallelBlockJacobi()
