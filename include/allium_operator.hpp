@@ -83,9 +83,35 @@ public:
     /* ≙ SetTimeAbsorption(tau) */
     void SetTimeAbsorption(double tau) {
         check(allium_sweep_set_time_absorption(h_, tau));
-        tau_ = tau;
     }
-    double GetTimeAbsorption() const { return tau_; }
+    double GetTimeAbsorption() const {
+        double tau = 0.0;
+        check(allium_sweep_get_time_absorption(h_, &tau));
+        return tau;
+    }
+    /* ≙ GetLumpingType (always 0) and IsParallelBlockJacobi (always false) */
+    int GetLumpingType() const {
+        int lump = 0, pbj = 0;
+        check(allium_sweep_get_lumping(h_, &lump, &pbj));
+        return lump;
+    }
+    bool IsParallelBlockJacobi() const {
+        int lump = 0, pbj = 0;
+        check(allium_sweep_get_lumping(h_, &lump, &pbj));
+        return pbj != 0;
+    }
+    /* ≙ SetFixupOperator / UseFixup / IsFixupOn / SetFixupMonitorData (src/sweep.hpp:116-122):
+     * type ALLIUM_FIXUP_ZERO, _ZERO_AND_SCALE or _RYOSUKE; the monitor is a device array of
+     * G Ne doubles */
+    void SetFixupOperator(int type, double psi_min = 0.0) { check(allium_sweep_set_fixup(h_, type, psi_min)); }
+    void UseFixup(bool use) { check(allium_sweep_use_fixup(h_, use ? 1 : 0)); }
+    bool IsFixupOn() const {
+        int type = 0, on = 0;
+        double mn = 0.0;
+        check(allium_sweep_get_fixup(h_, &type, &mn, &on));
+        return on != 0;
+    }
+    void SetFixupMonitorData(double *d_monitor) { check(allium_sweep_set_fixup_monitor(h_, d_monitor)); }
     void SetPrecision(int precision) { check(allium_sweep_set_precision(h_, precision)); }
     void SetLaunch(int mode) { check(allium_sweep_set_launch(h_, mode)); }
 
@@ -103,7 +129,6 @@ public:
 private:
     allium_sweep_t h_ = nullptr;
     allium_sweep_info_t info_{};
-    double tau_ = 0.0;
 };
 
 }  // namespace allium
