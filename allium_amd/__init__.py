@@ -409,8 +409,9 @@ class DiscreteToMoment:
 
 
 # "mip": the scalar MIP operator (chive's "scalar mip"); "block mip": chive's "mip"
-# (block interior-penalty system, Schur complement solved); "p1sa": chive's "p1sa"
-DSA_METHODS = {"mip": 0, "p1sa": 1, "block mip": 2}
+# (block interior-penalty system, Schur complement solved); "p1sa": chive's "p1sa";
+# "ldgsa": chive's "ldgsa" (block LDG system)
+DSA_METHODS = {"mip": 0, "p1sa": 1, "block mip": 2, "ldgsa": 3}
 
 
 def _dsa_desc(kappa=-1.0, bound_below=True, alpha=0.0, sym=-1.0, cg_reltol=1e-6, cg_abstol=0.0, cg_maxit=100,
@@ -440,24 +441,32 @@ def host_mip_matrix(mesh, fe_order, quad, sigma_t, sigma_s, **desc):
     return rp, col, val
 
 
-def host_block_mip_matrix(mesh, fe_order, quad, sigma_t, sigma_s, **desc):
+def host_block_ldg_matrix(mesh, fe_order, quad, sigma_t, sigma_s, **desc):
+    """chive's "ldgsa" 2x2 block operator as CSR arrays in the layout of
+    host_p1_matrix, host only (option: alpha)."""
+    return host_block_mip_matrix(mesh, fe_order, quad, sigma_t, sigma_s, _entry="allium_host_block_ldg_matrix",
+                                 **desc)
+
+
+def host_block_mip_matrix(mesh, fe_order, quad, sigma_t, sigma_s, _entry="allium_host_block_mip_matrix", **desc):
     """chive's "mip" 2x2 block operator as CSR arrays (row_ptr, col, val) in the
     layout of host_p1_matrix, host only (options: kappa, bound_below, alpha,
     scale_penalty)."""
     lib = _lib.load()
+    fn = getattr(lib, _entry)
     qd, keep = _quad_desc(quad, mesh.dim)
     dd = _dsa_desc(**desc)
     st = _f64(np.broadcast_to(sigma_t, (mesh.Ne,)))
     ss = _f64(np.broadcast_to(sigma_s, (mesh.Ne,)))
     nnz = ctypes.c_longlong(0)
     md = mesh.desc()
-    check(lib.allium_host_block_mip_matrix(ctypes.byref(md), fe_order, ctypes.byref(qd), _d(st), _d(ss),
+    check(fn(ctypes.byref(md), fe_order, ctypes.byref(qd), _d(st), _d(ss),
                                            ctypes.byref(dd), ctypes.byref(nnz), None, None, None))
     N = (mesh.dim + 1) * (fe_order + 1) ** mesh.dim
     rp = np.zeros(mesh.Ne * N + 1, dtype=np.int64)
     col = np.zeros(nnz.value, dtype=np.int32)
     val = np.zeros(nnz.value)
-    check(lib.allium_host_block_mip_matrix(ctypes.byref(md), fe_order, ctypes.byref(qd), _d(st), _d(ss),
+    check(fn(ctypes.byref(md), fe_order, ctypes.byref(qd), _d(st), _d(ss),
                                            ctypes.byref(dd), ctypes.byref(nnz),
                                            rp.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
                                            col.ctypes.data_as(_lib.c_ip), _d(val)))
@@ -487,8 +496,9 @@ class DSA:
     """Diffusion synthetic acceleration on the GPU
     (DiffusionSyntheticAccelerationOperator, src/transport_op.hpp:38-59) with
     chive's "scalar mip" operator (method "mip", exe/chive.cpp:663-682), its
-    block "mip" one (method "block mip", exe/chive.cpp:635-662) or its "p1sa"
-    one (method "p1sa", exe/chive.cpp:608-619).
+    block "mip" one (method "block mip", exe/chive.cpp:635-662), its "p1sa"
+    one (method "p1sa", exe/chive.cpp:608-619) or its "ldgsa" one (method
+    "ldgsa", exe/chive.cpp:620-634).
 
     ``Mult(x, y)``: y = x + K^{-1} Ms x on device vectors; ``solve(b)``: K^{-1} b
     for host arrays (for "p1sa": the phi-part of P^{-1} [0; b]).  Options of

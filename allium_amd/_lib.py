@@ -149,6 +149,10 @@ SIGNATURES = {
                                                     c_dp, c_dp, ctypes.POINTER(DSADesc),
                                                     ctypes.POINTER(ctypes.c_longlong),
                                                     ctypes.POINTER(ctypes.c_longlong), c_ip, c_dp]),
+    "allium_host_block_ldg_matrix": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.POINTER(QuadDesc),
+                                                    c_dp, c_dp, ctypes.POINTER(DSADesc),
+                                                    ctypes.POINTER(ctypes.c_longlong),
+                                                    ctypes.POINTER(ctypes.c_longlong), c_ip, c_dp]),
     "allium_host_p1_matrix": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int, c_dp, c_dp,
                                              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                                              c_ip, c_dp]),

@@ -801,7 +801,7 @@ bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigm
                         // interior penalty on phi: P1 alpha/2; IP max(kappa {D} area / det_1, limit)
                         // (PenaltyIntegrator, src/moment_integrators.cpp:206-287)
                         double pen = 0.5 * alpha;
-                        if (ip) {
+                        if (ip && !ip->ldg) {
                             double J1[9], adj1[9];
                             elem_jacobian(m, e1, xi[0], J1);
                             const double det1 = det_adj(dim, J1, adj1);
@@ -823,11 +823,15 @@ bool assemble_p1(const Mesh &m, int p, const double *sigma_t, const double *sigm
                                             for (int c = 0; c < dim; c++)
                                                 for (int d = 0; d < dim; d++)
                                                     at(X, c, i, d, j) += sab * jj * w * area * nh[c] * nh[d] * pp;
-                                        for (int d = 0; d < dim; d++) {
-                                            const double dv = 0.5 * sa * w * Nv[d] * pp;   // [psi]{J.n}
-                                            at(X, dim, i, d, j) += dv;
-                                            at(XT, d, j, dim, i) -= dv;
-                                        }
+                                        // [psi]{J.n}; LDG's trace takes element 1's current only
+                                        // (LDGTraceIntegrator with beta . N > 0)
+                                        const double tw = (ip && ip->ldg) ? (bb == 0 ? 1.0 : 0.0) : 0.5;
+                                        if (tw != 0.0)
+                                            for (int d = 0; d < dim; d++) {
+                                                const double dv = tw * sa * w * Nv[d] * pp;
+                                                at(X, dim, i, d, j) += dv;
+                                                at(XT, d, j, dim, i) -= dv;
+                                            }
                                     }
                             }
                     }

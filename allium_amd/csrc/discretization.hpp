@@ -95,6 +95,9 @@ struct BlockIpParams {
     double kappa;
     double limit;
     bool scale_D;
+    // chive's "ldgsa" (BlockLDGDiscretization, :266-340) instead: penalty alpha/2
+    // (kappa, limit, scale_D unused) and the one-sided LDG trace [psi] J_1.n in D
+    bool ldg = false;
 };
 // chive's "p1sa" operator (P1Discretization::GetOperator, src/moment_discretization.cpp:400-490)
 // in the same element-block storage: per element the unknowns [J_0 (n), .., J_{dim-1} (n), phi (n)],

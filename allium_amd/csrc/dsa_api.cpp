@@ -506,9 +506,10 @@ int add_p1(allium_dsa *d, const double *siga)
     allium::MipMatrix K;
     std::string why;
     bool ok_asm;
-    if (d->method == ALLIUM_DSA_BLOCK_MIP) {
+    if (d->method == ALLIUM_DSA_BLOCK_MIP || d->method == ALLIUM_DSA_LDG) {
         const double alpha = mip_params(d->desc, d->h->ed.p, d->h->quad).alpha;
-        const allium::BlockIpParams ip = block_ip_params(d->desc, d->h->ed.p, alpha);
+        allium::BlockIpParams ip = block_ip_params(d->desc, d->h->ed.p, alpha);
+        ip.ldg = d->method == ALLIUM_DSA_LDG;
         ok_asm = allium::assemble_p1(m, d->h->ed.p, d->h->sigma.data(), siga, alpha, K, why, &ip);
     } else {
         ok_asm = allium::assemble_p1(m, d->h->ed.p, d->h->sigma.data(), siga, kP1Alpha, K, why);
@@ -751,8 +752,8 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
     d->h = h;
     if (desc) d->desc = *desc; else allium_dsa_default_desc(&d->desc);
     if (d->desc.cg_maxit < 1 || d->desc.smooth < 1) return fail(ALLIUM_ERR_ARG, "cg_maxit and smooth must be >= 1");
-    if (d->desc.method != ALLIUM_DSA_MIP && d->desc.method != ALLIUM_DSA_P1 && d->desc.method != ALLIUM_DSA_BLOCK_MIP)
-        return fail(ALLIUM_ERR_ARG, "method must be ALLIUM_DSA_MIP, ALLIUM_DSA_P1 or ALLIUM_DSA_BLOCK_MIP");
+    if (d->desc.method < ALLIUM_DSA_MIP || d->desc.method > ALLIUM_DSA_LDG)
+        return fail(ALLIUM_ERR_ARG, "method must be ALLIUM_DSA_MIP, _P1, _BLOCK_MIP or _LDG");
     d->method = d->desc.method;
     const bool p1 = d->method != ALLIUM_DSA_MIP;
     const allium::Mesh &m = h->mesh;
@@ -953,9 +954,9 @@ int allium_host_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const all
     return export_csr(K, m.Ne, nnz, row_ptr, col, val);
 }
 
-int allium_host_block_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
-                                 const double *sigma_t, const double *sigma_s, const allium_dsa_desc *desc,
-                                 long long *nnz, long long *row_ptr, int *col, double *val)
+static int host_block_matrix(bool ldg, const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
+                             const double *sigma_t, const double *sigma_s, const allium_dsa_desc *desc,
+                             long long *nnz, long long *row_ptr, int *col, double *val)
 {
     if (!mesh || !quad || !sigma_t || !sigma_s || !nnz) return fail(ALLIUM_ERR_ARG, "null argument");
     allium::Mesh m;
@@ -970,10 +971,25 @@ int allium_host_block_mip_matrix(const allium_mesh_desc *mesh, int fe_order, con
     std::vector<double> siga(m.Ne);
     for (int e = 0; e < m.Ne; e++) siga[e] = sigma_t[e] - sigma_s[e];
     const double alpha = mip_params(dd, fe_order, q).alpha;
-    const allium::BlockIpParams ip = block_ip_params(dd, fe_order, alpha);
+    allium::BlockIpParams ip = block_ip_params(dd, fe_order, alpha);
+    ip.ldg = ldg;
     allium::MipMatrix K;
     if (!allium::assemble_p1(m, fe_order, sigma_t, siga.data(), alpha, K, why, &ip)) return fail(ALLIUM_ERR_ARG, why);
     return export_csr(K, m.Ne, nnz, row_ptr, col, val);
+}
+
+int allium_host_block_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
+                                 const double *sigma_t, const double *sigma_s, const allium_dsa_desc *desc,
+                                 long long *nnz, long long *row_ptr, int *col, double *val)
+{
+    return host_block_matrix(false, mesh, fe_order, quad, sigma_t, sigma_s, desc, nnz, row_ptr, col, val);
+}
+
+int allium_host_block_ldg_matrix(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
+                                 const double *sigma_t, const double *sigma_s, const allium_dsa_desc *desc,
+                                 long long *nnz, long long *row_ptr, int *col, double *val)
+{
+    return host_block_matrix(true, mesh, fe_order, quad, sigma_t, sigma_s, desc, nnz, row_ptr, col, val);
 }
 
 int allium_host_p1_matrix(const allium_mesh_desc *mesh, int fe_order, const double *sigma_t, const double *sigma_s,

@@ -5,7 +5,7 @@ test/chive/tdl.lua) written as YAML or a dict, runs the outer-solve path of
 exe/chive.cpp:550-750 (FormTransportSource, InverseAdvectionOperator,
 SLISolver, GMRESSolver or FGMRESSolver on TransportOperator as
 io::CreateIterativeSolver builds them (src/io.cpp:179-230), optional
-"scalar mip", block "mip" or "p1sa" DSA preconditioner, final sweep) and reports with chive's YAML keys ("transport iterations" with
+"scalar mip", block "mip", "ldgsa" or "p1sa" DSA preconditioner, final sweep) and reports with chive's YAML keys ("transport iterations" with
 it / norm / inner solver, "outer iterations", "inner iteration" max / min /
 avg / total; exe/chive.cpp:40-75,738-746), so scripts that parse chive's
 output read this one too.
@@ -94,8 +94,8 @@ def run(deck):
         raise ValueError("kdim must be in 1..200")
     prec = drv.get("preconditioner")
     ptype = str(prec.get("type", "")).lower() if prec else None
-    if prec and ptype not in ("scalar mip", "mip", "p1sa"):
-        raise ValueError(f"preconditioner {prec.get('type')} is not supported (scalar mip, mip, p1sa)")
+    if prec and ptype not in ("scalar mip", "mip", "p1sa", "ldgsa"):
+        raise ValueError(f"preconditioner {prec.get('type')} is not supported (scalar mip, mip, p1sa, ldgsa)")
     # chive's inner solver defaults to "direct"; p1sa takes nothing else (exe/chive.cpp:502-524,609)
     itype = str(prec.get("solver", {}).get("type", "direct")).lower() if prec else None
     if ptype == "p1sa" and itype not in ("direct", "superlu"):
@@ -113,6 +113,14 @@ def run(deck):
         # the direct solve is emulated by GMRES to a relative residual of 1e-12
         accel = A.DSA(Linv, sigma_s, method="p1sa", cg_reltol=1e-12, cg_maxit=20000)
         out["preconditioner"] = {"type": "p1sa", "solver": {"type": itype}}
+    elif ptype == "ldgsa":
+        # chive's block LDG system (exe/chive.cpp:620-634), solved as for "mip"
+        inner = prec.get("solver", {})
+        direct = itype in ("direct", "superlu")
+        accel = A.DSA(Linv, sigma_s, method="ldgsa", cg_reltol=1e-12 if direct else float(inner.get("reltol", 1e-10)),
+                      cg_abstol=0.0 if direct else float(inner.get("abstol", 0.0)),
+                      cg_maxit=20000 if direct else int(inner.get("max_iter", 20000)))
+        out["preconditioner"] = {"type": "ldgsa", "solver": {"type": itype}}
     elif ptype == "mip":
         # chive's block interior-penalty system (exe/chive.cpp:635-662); the reference
         # solves its Schur complement (CG + BoomerAMG or direct), here GMRES solves the

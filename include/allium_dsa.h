@@ -64,6 +64,9 @@ typedef struct {
 #define ALLIUM_DSA_MIP 0
 #define ALLIUM_DSA_P1 1
 #define ALLIUM_DSA_BLOCK_MIP 2
+/* chive's "ldgsa" (exe/chive.cpp:620-634): the block LDG system of
+ * allium_host_block_ldg_matrix, solved like ALLIUM_DSA_P1 (alpha as for MIP) */
+#define ALLIUM_DSA_LDG 3
 
 typedef struct {
     int levels;                 /* DG level + vertex-grid levels */
@@ -107,6 +110,12 @@ int allium_host_mip_matrix(const allium_mesh_desc *mesh, int fe_order, const all
  * ((dim + 1) Ndof rows).  2D p = 1, 2 and 3D p = 1. */
 int allium_host_p1_matrix(const allium_mesh_desc *mesh, int fe_order, const double *sigma_t, const double *sigma_s,
                           long long *nnz, long long *row_ptr, int *col, double *val);
+/* Host only: chive's "ldgsa" block operator, BlockLDGDiscretization::GetOperator
+ * (src/moment_discretization.cpp:266-340) with desc->alpha (<= 0: ComputeAlpha),
+ * in allium_host_p1_matrix's layout. */
+int allium_host_block_ldg_matrix(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
+                                 const double *sigma_t, const double *sigma_s, const allium_dsa_desc *desc,
+                                 long long *nnz, long long *row_ptr, int *col, double *val);
 /* Host only: chive's "mip" block operator, BlockIPDiscretization::GetOperator
  * (src/moment_discretization.cpp:342-398) with the penalty options of desc
  * (kappa, bound_below, alpha, scale_penalty), in allium_host_p1_matrix's

@@ -269,6 +269,24 @@ def block_ip_matrix(dim, nel, vertices, p, sigma_t, sigma_s, omega, weights, bc=
     return _block_moment_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc, alpha, ip)
 
 
+def block_ldg_matrix(dim, nel, vertices, p, sigma_t, sigma_s, omega, weights, bc=None):
+    """chive's "ldgsa" operator, BlockLDGDiscretization::GetOperator
+    (src/moment_discretization.cpp:266-340) with SetAlpha(ComputeAlpha), in
+    p1_matrix's layout:
+      Mt = VectorMass(3 sigma_t);
+      Ma = Mass(sigma_a) + Penalty(alpha/2, unscaled) on interior faces + BoundaryMass(alpha)
+           on Marshak sides;
+      D  = -(grad psi, J) + LDGTraceIntegrator on interior faces (src/moment_integrators.cpp:593-692);
+      G = -D^T.
+    The trace weighs element 1's current by 1 + sign(beta . N) and element 2's by
+    1 - sign(beta . N), with beta = |random| >= 0 (mfem Vector::Randomize) and N out
+    of element 1.  On these meshes element 1 is the lower element along the face's
+    axis, so beta . N > 0 and the trace is element 1's current: [psi] J_1 . N."""
+    alpha = compute_alpha(omega, weights)
+    ip = {"ldg": True}
+    return _block_moment_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc, alpha, ip)
+
+
 def schur_complement(Pb, dim, n):
     """Ma + D Mt^-1 D^T of a block matrix whose Mt is block diagonal by element
     (BlockMomentDiscretization::FormSchurComplement), over the phi unknowns."""
@@ -363,7 +381,7 @@ def _block_moment_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc, alpha, ip)
                 ph1, _ = _basis(dim, p, xi1)
                 ph2, _ = _basis(dim, p, xi2)
                 jmp = [(e1, ph1, 1.0), (e2, ph2, -1.0)]
-                if ip is None:
+                if ip is None or ip.get("ldg"):
                     pen = np.full(len(wf), 0.5 * alpha)
                 else:
                     det1 = np.linalg.det(geo.jac(e1, xi1))
@@ -378,9 +396,15 @@ def _block_moment_matrix(dim, nel, vertices, p, sigma_t, sigma_s, bc, alpha, ip)
                                 for d in range(dim):
                                     add(idx(ea, c), idx(eb, d), (0.5 / alpha) * sa * sb *
                                         np.einsum("q,qi,qj->ij", wf * area * nh[:, c] * nh[:, d], pa, pb))
-                        # [psi]{J.n}: test side sign sa, trial average 1/2
+                        # [psi]{J.n}: test side sign sa, trial average 1/2; LDG: trial element 1 only
+                        if ip is not None and ip.get("ldg"):
+                            if eb != e1:
+                                continue
+                            tw = 1.0
+                        else:
+                            tw = 0.5
                         for d in range(dim):
-                            Df = 0.5 * sa * np.einsum("q,qi,qj->ij", wf * Nn[:, d], pa, pb)
+                            Df = tw * sa * np.einsum("q,qi,qj->ij", wf * Nn[:, d], pa, pb)
                             add(idx(ea, dim), idx(eb, d), Df)
                             add(idx(eb, d), idx(ea, dim), -Df.T)
     Ntot = geo.Ne * N

@@ -26,7 +26,7 @@ def _deck():
     (lambda d: d.update(material_map=["mat"] * 3), "one entry per element"),
     (lambda d: d.update(boundary_map=["vacuum"] * 3), "boundary_map"),
     (lambda d: d["boundary_conditions"]["vacuum"].update(type="periodic"), "not supported"),
-    (lambda d: d["driver"].update(preconditioner={"type": "ldgsa"}), "not supported"),
+    (lambda d: d["driver"].update(preconditioner={"type": "h1sa"}), "not supported"),
     (lambda d: d["driver"].update(preconditioner={"type": "p1sa", "solver": {"type": "cg"}}), "only direct"),
 ])
 def test_deck_errors(edit, msg):
@@ -57,11 +57,11 @@ def test_tdl_deck_outer_iterations():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["p1sa", "mip"])
+@pytest.mark.parametrize("prec", ["p1sa", "mip", "ldgsa"])
 def test_tdl_deck_direct_inner_solver(prec):
     """test/chive/CMakeLists.txt:117-121: the TDL deck with
-    `preconditioner = {type = "<dsa>", solver = {type = "direct"}}` for p1sa
-    and mip (the block interior-penalty system) needs fewer than 12 outer
+    `preconditioner = {type = "<dsa>", solver = {type = "direct"}}` for p1sa,
+    ldgsa and mip (the block interior-penalty system) needs fewer than 12 outer
     iterations."""
     print(f"driver tdl {prec} direct", flush=True)
     d = _deck()

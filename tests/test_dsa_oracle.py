@@ -462,3 +462,92 @@ def test_host_block_mip_assembly_matches_oracle(dim, p, jitter, scale):
     Ph = sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2).toarray()
     Po = M.block_ip_matrix(dim, nel, vert, p, st, ss, om, w, bc=bc, scale=scale).toarray()
     np.testing.assert_allclose(Ph, Po, rtol=1e-12, atol=1e-12 * np.abs(Po).max())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_block_ldg_blocks_on_elementwise_constants(dim):
+    """chive's "ldgsa" block operator (oracle block_ldg_matrix), uniform mesh,
+    p = 1, all sides Marshak, element-wise constant c_e and f_e.  The LDG trace
+    takes the lower element's current (beta . N > 0), so for node i of element e
+    on a side with outward normal n and node face integral A:
+      D row:  0 on the high interior sides (the trace is e's own current),
+              A (c_low - c_e).n on the low interior sides, -A c_e.n on Marshak sides;
+      Ma row: sigma_a vol f_e + (alpha/2) A (f_e - f_nbr) interior + alpha A f_e Marshak;
+      Mt row: 3 sigma_t vol c_e;  G = -D^T."""
+    n_el = 3
+    nel = (n_el,) * dim
+    vert = P.cartesian_vertices(nel)
+    rng = np.random.default_rng(23)
+    ne = n_el ** dim
+    st = rng.uniform(0.5, 3.0, ne)
+    ss = st * rng.uniform(0.1, 0.9, ne)
+    om, w = ls_fixture(4, dim)
+    alpha = M.compute_alpha(om, w)
+    Pm = M.block_ldg_matrix(dim, nel, vert, 1, st, ss, om, w).toarray()
+    n, N = 2 ** dim, (dim + 1) * 2 ** dim
+    h = 1.0 / n_el
+    vol, area = h ** dim / n, h ** (dim - 1) / 2 ** (dim - 1)
+    C = rng.normal(size=(ne, dim))
+    F = rng.normal(size=ne)
+    x = np.zeros((ne, dim + 1, n))
+    x[:, :dim, :] = C[:, :, None]
+    x[:, dim, :] = F[:, None]
+    y = (Pm @ x.ravel()).reshape(ne, dim + 1, n)
+    xf = np.zeros((ne, dim + 1, n))
+    xf[:, dim, :] = F[:, None]
+    yf = (Pm @ xf.ravel()).reshape(ne, dim + 1, n)
+    for e in range(ne):
+        co = [(e // n_el ** d) % n_el for d in range(dim)]
+        Dr = np.zeros(n)
+        Ma = (st[e] - ss[e]) * vol * F[e] * np.ones(n)
+        for ax in range(dim):
+            for hi in (0, 1):
+                mask = np.array([((i >> ax) & 1) == hi for i in range(n)])
+                sgn = 1.0 if hi else -1.0
+                if co[ax] == (n_el - 1) * hi:
+                    Dr[mask] += -area * sgn * C[e][ax]
+                    Ma[mask] += alpha * area * F[e]
+                else:
+                    o = e + (1 if hi else -1) * n_el ** ax
+                    if not hi:
+                        Dr[mask] += area * sgn * (C[o][ax] - C[e][ax])
+                    Ma[mask] += 0.5 * alpha * area * (F[e] - F[o])
+        np.testing.assert_allclose(y[e, dim], Dr + Ma, atol=1e-13)
+        np.testing.assert_allclose(y[e, :dim] - yf[e, :dim], 3 * st[e] * vol * np.repeat(C[e][:, None], n, 1),
+                                   rtol=1e-13, atol=1e-13)
+    jdx = np.concatenate([np.arange(e * N, e * N + dim * n) for e in range(ne)])
+    pdx = np.concatenate([np.arange(e * N + dim * n, (e + 1) * N) for e in range(ne)])
+    np.testing.assert_allclose(Pm[np.ix_(jdx, pdx)], -Pm[np.ix_(pdx, jdx)].T, atol=1e-16)
+
+
+def test_block_ldg_accelerates_the_tdl_deck():
+    """chive.tdl.ldgsa (test/chive/CMakeLists.txt:117-121): under 12 iterations,
+    at the transport fixed point; its Schur complement is SPD."""
+    nel, vert, om, w, st, ss, eps = _tdl()
+    pr = oracle.Problem(2, nel, vert, 1, om, w, st)
+    q = pr.form_source(Q=eps)
+    Pm = M.block_ldg_matrix(2, nel, vert, 1, st, ss, om, w)
+    S = M.schur_complement(Pm, 2, 4).toarray()
+    assert np.linalg.eigvalsh(0.5 * (S + S.T)).min() > 0
+    phi, it, norms = M.dsa_source_iteration(pr, ss, q, None, abstol=1e-6, maxit=50, solve=M.p1_solver(Pm, 2, 4))
+    assert norms[-1] < 1e-6 and it < 12
+    b = pr.D(pr.sweep(q))
+    r = b - phi + pr.D(pr.sweep(pr.DT(pr.mass(ss, phi))))
+    assert np.linalg.norm(r) < 1e-5 * np.linalg.norm(phi)
+
+
+@pytest.mark.parametrize("dim,p,jitter", [(2, 1, 0.0), (2, 2, 0.15), (3, 1, 0.0), (3, 1, 0.15)])
+def test_host_block_ldg_assembly_matches_oracle(dim, p, jitter):
+    n = 4 if dim == 2 else 3
+    nel = (n,) * dim
+    rng = np.random.default_rng(17)
+    vert = P.jittered_vertices(nel, jitter, rng) if jitter else P.cartesian_vertices(nel)
+    st = P.lognormal_sigma(n ** dim, 1.0, rng)
+    ss = st * rng.uniform(0.5, 0.999, n ** dim)
+    bc = [0, 1] + [0] * (2 * dim - 2)
+    om, w = ls_fixture(4, dim)
+    mesh = A.Mesh(nel, vertices=vert, bc=bc)
+    rp, col, val = A.host_block_ldg_matrix(mesh, p, (om, w), st, ss)
+    Ph = sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2).toarray()
+    Po = M.block_ldg_matrix(dim, nel, vert, p, st, ss, om, w, bc=bc).toarray()
+    np.testing.assert_allclose(Ph, Po, rtol=1e-12, atol=1e-12 * np.abs(Po).max())

@@ -170,7 +170,7 @@ def test_p1sa_errors():
     with pytest.raises(A.AlliumError):
         A.DSA(A.InverseAdvectionOperator(mesh, 0, 4, st), st * 0.9, method="p1sa")
     with pytest.raises(KeyError):
-        A.DSA(A.InverseAdvectionOperator(mesh, 1, 4, st), st * 0.9, method="ldgsa")
+        A.DSA(A.InverseAdvectionOperator(mesh, 1, 4, st), st * 0.9, method="h1sa")
 
 
 BLOCK_CASES = [c for c in CASES if c[2] == 1]
@@ -211,6 +211,48 @@ def test_tdl_deck_with_block_mip():
     phi, _, it, norms = A.source_iteration(Linv, ss, q, abstol=1e-6, maxit=50, want_psi=False, accel=dsa)
     pr = oracle.Problem(2, nel, vert, 1, om, w, st)
     Pm = M.block_ip_matrix(2, nel, vert, 1, st, ss, om, w)
+    phi_o, it_o, norms_o = M.dsa_source_iteration(pr, ss, pr.form_source(Q=eps), None, abstol=1e-6, maxit=50,
+                                                  solve=M.p1_solver(Pm, 2, 4))
+    print(f"  GPU {it} its, oracle {it_o}; GMRES its per application "
+          f"{dsa.info()['cg_iterations'] / max(1, dsa.info()['applications']):.1f}")
+    assert it < 12 and it == it_o
+    np.testing.assert_allclose(norms, norms_o, rtol=1e-3, atol=1e-10 * norms_o[0])
+    assert _rel(phi, phi_o) < 1e-8
+
+
+@pytest.mark.parametrize("case", BLOCK_CASES, ids=[f"{c[0]}d-{'x'.join(map(str, c[1]))}" for c in BLOCK_CASES])
+def test_ldg_solve_matches_schur_direct(case):
+    """chive's "ldgsa" (DSA(method="ldgsa")): the GPU block solve's phi-part
+    equals a direct solve of the LDG Schur complement."""
+    dim, nel, p = case[:3]
+    print(f"ldgsa solve {case}", flush=True)
+    mesh, vert, bc, st, ss, om, w = _setup(*case)
+    Linv = A.InverseAdvectionOperator(mesh, p, (om, w), st)
+    dsa = A.DSA(Linv, ss, method="ldgsa", cg_reltol=1e-12, cg_maxit=3000)
+    b = np.random.default_rng(7).normal(size=Linv.phi_size)
+    x = dsa.solve(b)
+    rp, col, val = A.host_block_ldg_matrix(mesh, p, (om, w), st, ss)
+    S = M.schur_complement(sp.csr_matrix((val, col, rp), shape=(rp.size - 1,) * 2), dim, (p + 1) ** dim)
+    ref = spla.spsolve(sp.csc_matrix(S), b)
+    info = dsa.info()
+    print(f"  GMRES its {info['last_cg_iterations']} rel {_rel(x, ref):.2e}")
+    assert info["last_cg_iterations"] < 3000 and _rel(x, ref) < 1e-9
+
+
+def test_tdl_deck_with_ldgsa():
+    """chive.tdl.ldgsa: under 12 iterations, the oracle's iterations and flux."""
+    print("ldgsa tdl", flush=True)
+    eps = 1e-4
+    nel = (8, 8)
+    vert = P.cartesian_vertices(nel)
+    om, w = ls_fixture(4, 2)
+    st, ss = np.full(64, 1 / eps), np.full(64, 1 / eps - eps)
+    Linv = A.InverseAdvectionOperator(A.Mesh(nel, vertices=vert), 1, (om, w), st)
+    q = Linv.form_source(Q=eps)
+    dsa = A.DSA(Linv, ss, method="ldgsa", cg_reltol=1e-13, cg_maxit=3000)
+    phi, _, it, norms = A.source_iteration(Linv, ss, q, abstol=1e-6, maxit=50, want_psi=False, accel=dsa)
+    pr = oracle.Problem(2, nel, vert, 1, om, w, st)
+    Pm = M.block_ldg_matrix(2, nel, vert, 1, st, ss, om, w)
     phi_o, it_o, norms_o = M.dsa_source_iteration(pr, ss, pr.form_source(Q=eps), None, abstol=1e-6, maxit=50,
                                                   solve=M.p1_solver(Pm, 2, 4))
     print(f"  GPU {it} its, oracle {it_o}; GMRES its per application "
