@@ -394,17 +394,25 @@ class AdvectionOperator:
 
 
 class DiscreteToMoment:
-    """0th-moment D and D^T on device vectors (src/transport_op.cpp:3-60)."""
+    """D and D^T on device vectors (src/transport_op.cpp:3-89): the scalar flux
+    (num_moments 1) or [phi, J] (num_moments dim + 1, moment index slowest)."""
 
-    def __init__(self, Linv):
+    def __init__(self, Linv, num_moments=1):
+        if num_moments not in (1, Linv.mesh.dim + 1):
+            raise ValueError("num_moments must be 1 or dim + 1")
         self.Linv = Linv
+        self.num_moments = num_moments
 
     def Mult(self, psi, phi):
-        check(self.Linv._lib.allium_moments_device(self.Linv._h, psi.ptr, phi.ptr, None))
+        fn = self.Linv._lib.allium_moments_device if self.num_moments == 1 else \
+            self.Linv._lib.allium_moments_aniso_device
+        check(fn(self.Linv._h, psi.ptr, phi.ptr, None))
         return phi
 
     def MultTranspose(self, phi, psi):
-        check(self.Linv._lib.allium_moments_transpose_device(self.Linv._h, phi.ptr, psi.ptr, None))
+        fn = self.Linv._lib.allium_moments_transpose_device if self.num_moments == 1 else \
+            self.Linv._lib.allium_moments_aniso_transpose_device
+        check(fn(self.Linv._h, phi.ptr, psi.ptr, None))
         return psi
 
 

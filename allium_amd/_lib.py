@@ -90,6 +90,8 @@ SIGNATURES = {
     "allium_form_source": (ctypes.c_int, [c_vp, c_dp, c_dp, c_dp]),
     "allium_moments_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "allium_moments_transpose_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "allium_moments_aniso_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "allium_moments_aniso_transpose_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "allium_mass_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "allium_source_iteration": (ctypes.c_int, [c_vp, c_dp, c_dp, ctypes.POINTER(SIOptions), c_dp, c_dp, c_dp,
                                                c_ip]),

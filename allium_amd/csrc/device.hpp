@@ -196,6 +196,10 @@ hipError_t launch_balance(const SweepParams &prm, const double *psi, double *par
                           hipStream_t stream);
 // phi = sum_a w_a psi ; psi = phi / wsum ; y = scale * coef_e * Mm(e) x_e
 hipError_t launch_moments(const SweepParams &prm, const double *psi, double *phi, hipStream_t s);
+// [phi, J] ((dim + 1) G Ndof, moment slowest) and its transpose
+hipError_t launch_moments_aniso(const SweepParams &prm, int dim, const double *psi, double *mom, hipStream_t s);
+hipError_t launch_moments_aniso_T(const SweepParams &prm, int dim, double wsum, const double *mom, double *psi,
+                                  hipStream_t s);
 hipError_t launch_moments_T(const SweepParams &prm, double inv_wsum, const double *phi, double *psi,
                             hipStream_t s);
 hipError_t launch_mass(const SweepParams &prm, const double *Mm, const double *coef, double scale,

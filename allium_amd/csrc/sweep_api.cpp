@@ -957,6 +957,20 @@ int allium_moments_transpose_device(allium_sweep_t h, const double *d_phi, doubl
     return ALLIUM_OK;
 }
 
+int allium_moments_aniso_device(allium_sweep_t h, const double *d_psi, double *d_mom, void *stream)
+{
+    if (!h || !d_psi || !d_mom) return fail(ALLIUM_ERR_ARG, "null argument");
+    HIP_TRY(allium::launch_moments_aniso(h->ds.prm, h->mesh.dim, d_psi, d_mom, (hipStream_t)stream));
+    return ALLIUM_OK;
+}
+
+int allium_moments_aniso_transpose_device(allium_sweep_t h, const double *d_mom, double *d_psi, void *stream)
+{
+    if (!h || !d_psi || !d_mom) return fail(ALLIUM_ERR_ARG, "null argument");
+    HIP_TRY(allium::launch_moments_aniso_T(h->ds.prm, h->mesh.dim, h->quad.wsum, d_mom, d_psi, (hipStream_t)stream));
+    return ALLIUM_OK;
+}
+
 int allium_mass_device(allium_sweep_t h, const double *d_coef, const double *d_x, double *d_y, void *stream)
 {
     if (!h || !d_coef || !d_x || !d_y) return fail(ALLIUM_ERR_ARG, "null argument");

@@ -1167,6 +1167,54 @@ hipError_t launch_moments(const SweepParams &prm, const double *psi, double *phi
     return hipGetLastError();
 }
 
+// [phi, J_0, .., J_{dim-1}] with the moment index slowest: mom[(m G + g) Ndof + s]
+// (MomentVectorLayout); J_m = sum_a w_a Omega_m psi_a
+__global__ void k_moments_aniso(const double *__restrict__ psi, const double *__restrict__ omega, int Nomega,
+                                long long Ndof, int G, int dim, double *__restrict__ mom)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ndof * G) return;
+    const long long g = i / Ndof, s = i - g * Ndof;
+    const double *ps = psi + g * Nomega * Ndof + s;
+    double p0 = 0.0, j[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < Nomega; ++a) {
+        const double w = omega[4 * a + 3], v = ps[(long long)a * Ndof];
+        p0 = fma(w, v, p0);
+        for (int m = 0; m < dim; ++m) j[m] = fma(omega[4 * a + m] * w, v, j[m]);
+    }
+    mom[g * Ndof + s] = p0;
+    for (int m = 0; m < dim; ++m) mom[((long long)(1 + m) * G + g) * Ndof + s] = j[m];
+}
+
+// psi = phi / sum w + 3 Omega . J / sum w (DiscreteToMoment::MultTranspose with dim + 1 moments)
+__global__ void k_moments_aniso_T(const double *__restrict__ mom, const double *__restrict__ omega, double wsum,
+                                  int Nomega, long long Ndof, int G, int dim, double *__restrict__ psi)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ndof * Nomega * G) return;
+    const long long s = i % Ndof, a = (i / Ndof) % Nomega, g = i / (Ndof * Nomega);
+    double v = mom[g * Ndof + s] / wsum;
+    for (int m = 0; m < dim; ++m) v += 3 * omega[4 * a + m] * mom[((long long)(1 + m) * G + g) * Ndof + s] / wsum;
+    psi[i] = v;
+}
+
+hipError_t launch_moments_aniso(const SweepParams &prm, int dim, const double *psi, double *mom, hipStream_t s)
+{
+    const long long n = prm.Ndof * prm.G;
+    hipLaunchKernelGGL(k_moments_aniso, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, psi, prm.omega,
+                       prm.Nomega, prm.Ndof, prm.G, dim, mom);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments_aniso_T(const SweepParams &prm, int dim, double wsum, const double *mom, double *psi,
+                                  hipStream_t s)
+{
+    const long long n = prm.Ndof * prm.Nomega * prm.G;
+    hipLaunchKernelGGL(k_moments_aniso_T, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, mom, prm.omega, wsum,
+                       prm.Nomega, prm.Ndof, prm.G, dim, psi);
+    return hipGetLastError();
+}
+
 __global__ void k_moments_T(const double *__restrict__ phi, double inv_wsum, int Nomega, long long Ndof, int G,
                             double *__restrict__ psi)
 {

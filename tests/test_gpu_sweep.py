@@ -507,3 +507,37 @@ def test_operator_getters():
     Linv.SetTimeAbsorption(2.5)
     assert Linv.GetTimeAbsorption() == 2.5
     assert Linv.GetLumpingType() == 0 and not Linv.IsParallelBlockJacobi()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_anisotropic_moments(dim):
+    """DiscreteToMoment with dim + 1 moments (src/transport_op.cpp:3-89): for
+    psi = c0 + Omega . v the level-symmetric set integrates exactly
+    (sum w Omega = 0, sum w Omega Omega^T = (sum w / 3) I), so phi = (sum w) c0,
+    J = (sum w / 3) v, and the transpose returns psi itself."""
+    print(f"anisotropic moments {dim}d", flush=True)
+    G = 2
+    nel = (4,) * dim
+    ne = 4 ** dim
+    om, w = ls_fixture(6, dim)
+    Linv = A.InverseAdvectionOperator(A.Mesh(nel), 1, (om, w), np.ones((G, ne)))
+    No, Ndof = len(w), Linv.phi_size // G
+    rng = np.random.default_rng(4)
+    c0 = rng.normal(size=(G, Ndof))
+    v = rng.normal(size=(G, dim, Ndof))
+    psi = c0[:, None, :] + np.einsum("am,gms->gas", np.asarray(om)[:, :dim], v)
+    D = A.DiscreteToMoment(Linv, dim + 1)
+    dpsi = A.DeviceVector.from_host(psi.ravel())
+    dmom = A.DeviceVector((dim + 1) * G * Ndof)
+    D.Mult(dpsi, dmom)
+    A.synchronize()
+    mom = dmom.to_host().reshape(dim + 1, G, Ndof)
+    wsum = float(np.sum(w))
+    np.testing.assert_allclose(mom[0], wsum * c0, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(mom[1:].transpose(1, 0, 2), wsum / 3 * v, rtol=1e-12, atol=1e-12)
+    dback = A.DeviceVector(psi.size)
+    D.MultTranspose(dmom, dback)
+    A.synchronize()
+    np.testing.assert_allclose(dback.to_host(), psi.ravel(), rtol=1e-12, atol=1e-12)
+    with pytest.raises(ValueError):
+        A.DiscreteToMoment(Linv, 2 if dim == 3 else 4)
