@@ -180,7 +180,11 @@ def run(deck):
         out["inner iteration"] = {"max": int(ii.max()), "min": int(ii.min()), "avg": float(ii.mean()),
                                   "total": int(ii.sum())}
     out["converged"] = converged
-    out["timings"] = {"solve": f"{solve:.3e} s"}
+    # chive's TimingLog / EventLog (exe/chive.cpp:1286-1294): the sweeps' GPU time and count
+    sweep_s, sweeps = Linv.timing()
+    out["timing log"] = {"sweep": f"{sweep_s:.3e} s"}
+    out["event log"] = {"sweeps": int(sweeps)}
+    out["timings"] = {"setup": out["setup time"], "solve": {"total": f"{solve:.3e} s", "sweep": f"{sweep_s:.3e} s"}}
     out["scalar flux"] = {"l2": float(np.linalg.norm(phi)), "min": float(phi.min()), "max": float(phi.max())}
     out["balance"] = float(A.AdvectionOperator(Linv).ComputeBalance(psi))
     return out

@@ -54,6 +54,10 @@ def test_tdl_deck_outer_iterations():
     assert float(rows[-1]["norm"]) < 1e-6 and rows[-1]["inner solver"]["it"] > 0
     assert out["inner iteration"]["total"] == sum(r["inner solver"]["it"] for r in rows)
     assert np.isfinite(out["balance"])
+    # TimingLog "sweep" and EventLog "sweeps": at least one sweep per iteration plus the
+    # first and last (launch autotuning may add trial sweeps)
+    assert out["event log"]["sweeps"] >= out["outer iterations"] + 2
+    assert float(out["timing log"]["sweep"].split()[0]) > 0
 
 
 @pytest.mark.gpu
