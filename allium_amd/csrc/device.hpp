@@ -189,6 +189,8 @@ bool sweep_supported(int dim, int p, std::string &why);
 
 // streaming operator L psi (AdvectionOperator::Mult)
 hipError_t launch_apply_L(const SweepParams &prm, const double *x, double *y, hipStream_t stream);
+// y = q - L x (the FP64 defect of the FP32-refined source iteration); y may not alias x
+hipError_t launch_defect_L(const SweepParams &prm, const double *q, const double *x, double *y, hipStream_t s);
 // per-block partial sums of absorption and leakage (ComputeBalance); out has
 // 2 * balance_blocks(prm) doubles
 int balance_blocks(const SweepParams &prm);

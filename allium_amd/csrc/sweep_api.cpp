@@ -1115,8 +1115,7 @@ static int source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         h->ds.precision = 1;
         for (it = 1;; it++) {
             HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
-            HIP_TRY(allium::launch_apply_L(h->ds.prm, h->a.d(), h->c.d(), s));
-            HIP_TRY(allium::launch_axpby(nps, 1.0, dq, -1.0, h->c.d(), h->c.d(), s));
+            HIP_TRY(allium::launch_defect_L(h->ds.prm, dq, h->a.d(), h->c.d(), s));
             rc = do_sweep(h, h->c.d(), h->iso.d(), h->c.d(), s);
             if (rc) break;
             HIP_TRY(allium::launch_axpby(nps, 1.0, h->a.d(), 1.0, h->c.d(), h->a.d(), s));

@@ -934,7 +934,8 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
 // one thread per (element, direction, group)
 // ---------------------------------------------------------------------------
 template <int DIM, int P, bool UNI>
-__global__ void __launch_bounds__(256) k_apply_L(SweepParams p, const double *__restrict__ x, double *__restrict__ y)
+__global__ void __launch_bounds__(256) k_apply_L(SweepParams p, const double *__restrict__ x, double *__restrict__ y,
+                                                 const double *__restrict__ q)
 {
     using S = Sh<DIM, P>;
     constexpr int N = S::N, NF = S::NF, NS = S::NS;
@@ -1013,33 +1014,50 @@ __global__ void __launch_bounds__(256) k_apply_L(SweepParams p, const double *__
             }
         }
     }
+    // y = L x, or the defect y = q - L x when q is given
     double *ye = y + base + (long long)e * N;
+    if (q) {
+        const double *qe = q + base + (long long)e * N;
 #pragma unroll
-    for (int i = 0; i < N; ++i) ye[i] = yl[i];
+        for (int i = 0; i < N; ++i) ye[i] = qe[i] - yl[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) ye[i] = yl[i];
+    }
 }
 
 template <int DIM, int P>
-static hipError_t apply_L_dp(const SweepParams &prm, const double *x, double *y, hipStream_t s)
+static hipError_t apply_L_dp(const SweepParams &prm, const double *x, double *y, const double *q, hipStream_t s)
 {
     const long long total = (long long)prm.Ne * prm.Nomega * prm.G;
     const int grid = (int)((total + 255) / 256);
-    if (prm.uniform) hipLaunchKernelGGL((k_apply_L<DIM, P, true>), dim3(grid), dim3(256), 0, s, prm, x, y);
-    else hipLaunchKernelGGL((k_apply_L<DIM, P, false>), dim3(grid), dim3(256), 0, s, prm, x, y);
+    if (prm.uniform) hipLaunchKernelGGL((k_apply_L<DIM, P, true>), dim3(grid), dim3(256), 0, s, prm, x, y, q);
+    else hipLaunchKernelGGL((k_apply_L<DIM, P, false>), dim3(grid), dim3(256), 0, s, prm, x, y, q);
     return hipGetLastError();
+}
+
+static hipError_t apply_L_any(const SweepParams &prm, const double *x, double *y, const double *q, hipStream_t s)
+{
+    if (prm.dim == 2) {
+        if (prm.p == 0) return apply_L_dp<2, 0>(prm, x, y, q, s);
+        if (prm.p == 1) return apply_L_dp<2, 1>(prm, x, y, q, s);
+        if (prm.p == 2) return apply_L_dp<2, 2>(prm, x, y, q, s);
+    } else {
+        if (prm.p == 0) return apply_L_dp<3, 0>(prm, x, y, q, s);
+        if (prm.p == 1) return apply_L_dp<3, 1>(prm, x, y, q, s);
+        if (prm.p == 2) return apply_L_dp<3, 2>(prm, x, y, q, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_apply_L(const SweepParams &prm, const double *x, double *y, hipStream_t s)
 {
-    if (prm.dim == 2) {
-        if (prm.p == 0) return apply_L_dp<2, 0>(prm, x, y, s);
-        if (prm.p == 1) return apply_L_dp<2, 1>(prm, x, y, s);
-        if (prm.p == 2) return apply_L_dp<2, 2>(prm, x, y, s);
-    } else {
-        if (prm.p == 0) return apply_L_dp<3, 0>(prm, x, y, s);
-        if (prm.p == 1) return apply_L_dp<3, 1>(prm, x, y, s);
-        if (prm.p == 2) return apply_L_dp<3, 2>(prm, x, y, s);
-    }
-    return hipErrorInvalidValue;
+    return apply_L_any(prm, x, y, nullptr, s);
+}
+
+hipError_t launch_defect_L(const SweepParams &prm, const double *q, const double *x, double *y, hipStream_t s)
+{
+    return apply_L_any(prm, x, y, q, s);
 }
 
 // ---------------------------------------------------------------------------
