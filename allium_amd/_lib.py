@@ -169,6 +169,9 @@ SIGNATURES = {
     "allium_trt_state_size": (ctypes.c_int, [c_vp, c_lp]),
     "allium_trt_get_state": (ctypes.c_int, [c_vp, c_dp, c_ip, c_dp]),
     "allium_trt_set_state": (ctypes.c_int, [c_vp, c_dp, ctypes.c_int, ctypes.c_double]),
+    "allium_trt_track_energy": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "allium_trt_energy_terms": (ctypes.c_int, [c_vp, c_vp]),
+    "allium_trt_set_time_step": (ctypes.c_int, [c_vp, ctypes.c_double]),
     "allium_planck_integrals": (ctypes.c_int, [ctypes.c_double, c_dp, c_dp]),
     "allium_log_groups": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int, c_dp, c_dp]),
     # allium_device.h

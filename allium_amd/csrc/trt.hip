@@ -250,9 +250,11 @@ __global__ void __launch_bounds__(256) k_dB_groups(TrtParams P, const double *__
     for (int i = 0; i < N; ++i) og[i] = accumulate ? fma(scale, v[i], og[i]) : scale * v[i];
 }
 
-// T += Binv (sum_g sigma_g Mu phi_g + resid), floored at 1e-8
+// T += Binv (sum_g sigma_g Mu phi_g + resid), floored at 1e-8; Traw (may be
+// null) receives the update before the floor
 __global__ void k_update_T(TrtParams P, const double *__restrict__ phi, const double *__restrict__ resid,
-                           const double *__restrict__ Binv, double *__restrict__ T, int *__restrict__ floors)
+                           const double *__restrict__ Binv, double *__restrict__ T, int *__restrict__ floors,
+                           double *__restrict__ Traw)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= P.Ne) return;
@@ -275,6 +277,7 @@ __global__ void k_update_T(TrtParams P, const double *__restrict__ phi, const do
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         double v = T[(long long)e * N + i] + dot_basis(B + i * N, t);
+        if (Traw) Traw[(long long)e * N + i] = v;
         if (!(v >= 1e-8)) { v = 1e-8; fl++; }
         T[(long long)e * N + i] = v;
     }
@@ -373,6 +376,84 @@ __global__ void k_wsum(long long n, const double *__restrict__ x, const double *
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// Two block sums written to part[2 blk], part[2 blk + 1] (summed on the host in
+// block order, so the totals are deterministic)
+__device__ inline void block_sum2(double a, double b, double *__restrict__ part)
+{
+    __shared__ double sa[256], sb[256];
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) { sa[threadIdx.x] += sa[threadIdx.x + k]; sb[threadIdx.x] += sb[threadIdx.x + k]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = sa[0]; part[2 * blockIdx.x + 1] = sb[0]; }
+}
+
+// Energy bookkeeping per (element, point), 256 threads per block: a rule-4
+// point adds its linearized gray emission W4_0 + W4_1 (phi4_q . (Traw - T*)) (the
+// Jacobian weights of the pass), a rule-3 point the emission form at the new
+// temperature, w_q sum_g sigma_g b_g(T_q) T_q^4 (the basis sums to one)
+__global__ void __launch_bounds__(256) k_energy_points(TrtParams P, const double *__restrict__ T,
+                                                       const double *__restrict__ Traw,
+                                                       const double *__restrict__ Tstar,
+                                                       const double *__restrict__ W4, double *__restrict__ part)
+{
+    const int nq = P.nq4 + P.nq3;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double gray = 0.0, emis = 0.0;
+    if (t < (long long)P.Ne * nq) {
+        const int e = (int)(t / nq), q = (int)(t - (long long)e * nq);
+        double tv[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) tv[i] = T[(long long)e * N + i];
+        if (q < P.nq4) {
+            double dv[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) dv[i] = Traw[(long long)e * N + i] - Tstar[(long long)e * N + i];
+            const double *w = W4 + ((long long)e * P.nq4 + q) * 2;
+            gray = fma(w[1], dot_basis(P.phi4 + q * N, dv), w[0]);
+        } else {
+            const int r = q - P.nq4;
+            const double Tq = dot_basis(P.phi3 + r * N, tv);
+            const double T2 = Tq * Tq;
+            double sp = 0.0;
+            const double *sig = P.sigma + e;
+            allium::planck_spectrum_each<0>(P.bounds, P.G, Tq, [&](int g, double v) { sp = fma(sig[(long long)g * P.Ne], v, sp); });
+            emis = P.w3[r] * sp * T2 * T2;
+        }
+    }
+    block_sum2(gray, emis, part);
+}
+
+// sum_g sigma_g int phi_g (MG mass row sums rMu) and int sum_g phi_g (time mass
+// row sums rMm) over a grid-stride share, 256 threads per block
+__global__ void __launch_bounds__(256) k_phi_sums(TrtParams P, const double *__restrict__ rMu,
+                                                  const double *__restrict__ rMm, const double *__restrict__ phi,
+                                                  double *__restrict__ part)
+{
+    double a = 0.0, b = 0.0;
+    const long long n = (long long)P.G * P.Ne * N;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % N);
+        const long long ge = i / N;   // g * Ne + e
+        a = fma(P.sigma[ge] * rMu[k], phi[i], a);
+        b = fma(rMm[k], phi[i], b);
+    }
+    block_sum2(a, b, part);
+}
+
+// part[blk] = sum of (a[i] - b[i]) w[i % N] over a grid-stride share
+__global__ void k_wdiff(long long n, const double *__restrict__ a, const double *__restrict__ b,
+                        const double *__restrict__ w, double *__restrict__ part)
+{
+    double acc = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        acc = fma(a[i] - b[i], w[i % N], acc);
+    block_sum2(acc, 0.0, part);
+}
+
 thread_local std::string g_err;
 int tfail(int code, const std::string &m)
 {
@@ -413,6 +494,11 @@ struct allium_trt {
     double *W4 = nullptr, *E3 = nullptr;   // Planck weights per quadrature point (k_planck_points)
     double *UQ = nullptr;                  // Binv-weighted source at the rule-3 points (k_dB_points)
     int *flags = nullptr;
+    // energy bookkeeping (allium_trt_track_energy)
+    bool track = false;
+    double *rMu = nullptr, *rMm = nullptr, *ones = nullptr, *epart = nullptr, *Traw = nullptr;
+    double inflow_rate = 0.0, En = 0.0;   // sum D q_in; int sum_g phi^n_g
+    allium_trt_energy energy{};
     hipStream_t s = nullptr;
     ~allium_trt()
     {
@@ -550,6 +636,14 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
             for (int j = 0; j < N; j++) vw[i] += Mu[(size_t)i * N + j];
         TRT_TRY(h->alloc(&h->volw, vw.size(), vw.data()));
     }
+    {
+        std::vector<double> ru(N, 0.0), rm(N, 0.0), one(N, 1.0);
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) { ru[i] += Mu[(size_t)i * N + j]; rm[i] += Mm[(size_t)i * N + j]; }
+        TRT_TRY(h->alloc(&h->rMu, ru.size(), ru.data()));
+        TRT_TRY(h->alloc(&h->rMm, rm.size(), rm.data()));
+        TRT_TRY(h->alloc(&h->ones, one.size(), one.data()));
+    }
     TRT_TRY(h->alloc(&dp, h->bounds.size(), h->bounds.data())); P.bounds = dp;
     TRT_TRY(h->alloc(&h->dmid, h->mid.size(), h->mid.data()));
     TRT_TRY(h->alloc(&h->sigma, sig0.size(), sig0.data()));
@@ -558,6 +652,7 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     TRT_TRY(h->alloc(&h->T, h->Ndof));
     TRT_TRY(h->alloc(&h->Tn, h->Ndof));
     TRT_TRY(h->alloc(&h->Tk, h->Ndof));
+    TRT_TRY(h->alloc(&h->Traw, h->Ndof));
     TRT_TRY(h->alloc(&h->resid, h->Ndof));
     TRT_TRY(h->alloc(&h->Binv, (size_t)h->Ne * N * N));
     TRT_TRY(h->alloc(&h->dpw, (size_t)h->Ne * P.nq3 * h->G));
@@ -572,6 +667,7 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
     TRT_TRY(h->alloc(&h->phit, h->phi_size));
     TRT_TRY(h->alloc(&h->iso, h->phi_size));
     TRT_TRY(h->alloc(&h->part, 3 * kPart));
+    TRT_TRY(h->alloc(&h->epart, 2 * (((size_t)h->Ne * (P.nq4 + P.nq3) + 255) / 256)));
     TRT_TRY(h->alloc(&h->flags, 4));
     // initial state: T = T0, psi in equilibrium with T0 (PlanckEmissionPSCoefficient,
     // src/phase_coefficient.cpp:72-76: the Planck fraction between the group bounds)
@@ -595,6 +691,9 @@ int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_q
             if ((d.inflow_sides >> s) & 1) inflow[(size_t)g * 6 + s] = b * std::pow(Ti, 4) / fourpi;
     }
     ABI_TRY(allium_form_source(h->L, nullptr, inflow.data(), psi0.data()));
+    // sum D q_in: the inflow rate of the energy bookkeeping
+    for (long long k = 0; k < h->psi_size; k++)
+        if (psi0[k] != 0.0) h->inflow_rate += w[(k / h->Ndof) % h->No] * psi0[k];
     {
         std::vector<long long> idx;
         std::vector<double> val;
@@ -625,6 +724,75 @@ void apply_dB(allium_trt *h, int mode, const double *x, double scale, int accumu
     hipLaunchKernelGGL(k_dB_groups, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, h->s, P, h->UQ, h->dpw,
                        scale, accumulate, out);
 }
+// both block sums of a part buffer (nb blocks), in block order
+int sum_parts(allium_trt *h, int nb, double &a, double &b)
+{
+    std::vector<double> hp(2 * (size_t)nb);
+    TRT_TRY(hipMemcpyAsync(hp.data(), h->part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, h->s));
+    TRT_TRY(hipStreamSynchronize(h->s));
+    a = b = 0.0;
+    for (int k = 0; k < nb; k++) { a += hp[2 * k]; b += hp[2 * k + 1]; }
+    return ALLIUM_OK;
+}
+
+// (sum_g sigma_g int phi_g, int sum_g phi_g) of x (phi-sized)
+int phi_sums(allium_trt *h, const double *x, double &absorbed, double &energy)
+{
+    hipLaunchKernelGGL(k_phi_sums, dim3(kPart / 2), dim3(256), 0, h->s, h->P, h->rMu, h->rMm, x, h->part);
+    TRT_TRY(hipGetLastError());
+    return sum_parts(h, kPart / 2, absorbed, energy);
+}
+
+// the energy bookkeeping of the pass that just ended (allium_trt_energy):
+// T is the new temperature, Tstar the linearization point, the Jacobian
+// weights W4 and the emission em those of the pass; phit is free
+int energy_terms(allium_trt *h, const double *Tstar)
+{
+    allium_trt_energy &E = h->energy;
+    const double tau = 1.0 / h->d.dt;
+    double En1 = 0.0, unused = 0.0;
+    ABI_TRY(allium_moments_device(h->L, h->psi, h->phit, h->s));
+    ABI_TRY(phi_sums(h, h->phit, E.absorption, En1));
+    E.time_absorption = tau * En1;
+    ABI_TRY(phi_sums(h, h->phi, E.absorption_si, unused));
+    double lb = 0.0;
+    ABI_TRY(allium_balance_device(h->L, h->psi, &lb, h->s));
+    E.leakage = lb - E.absorption - E.time_absorption;
+    E.inflow = h->inflow_rate;
+    E.source = h->inflow_rate + tau * h->En;
+    hipLaunchKernelGGL(k_wdiff, dim3(kPart / 2), dim3(256), 0, h->s, (long long)h->Ndof, h->T, h->Tn, h->rMm, h->part);
+    TRT_TRY(hipGetLastError());
+    ABI_TRY(sum_parts(h, kPart / 2, E.material, unused));
+    E.material *= h->P.cvdt;
+    hipLaunchKernelGGL(k_wdiff, dim3(kPart / 2), dim3(256), 0, h->s, (long long)h->Ndof, h->T, h->Traw, h->rMm, h->part);
+    TRT_TRY(hipGetLastError());
+    ABI_TRY(sum_parts(h, kPart / 2, E.floor, unused));
+    E.floor *= h->P.cvdt;
+    hipLaunchKernelGGL(k_wsum, dim3(kPart), dim3(256), 0, h->s, (long long)h->phi_size, h->em, h->ones, h->part);
+    TRT_TRY(hipGetLastError());
+    {
+        std::vector<double> hp(kPart);
+        TRT_TRY(hipMemcpyAsync(hp.data(), h->part, sizeof(double) * kPart, hipMemcpyDeviceToHost, h->s));
+        TRT_TRY(hipStreamSynchronize(h->s));
+        E.emission_linear = 0.0;
+        for (double v : hp) E.emission_linear += v;
+    }
+    const long long pts = (long long)h->Ne * (h->P.nq4 + h->P.nq3);
+    const int nb = (int)((pts + 255) / 256);
+    hipLaunchKernelGGL(k_energy_points, dim3(nb), dim3(256), 0, h->s, h->P, h->T, h->Traw, Tstar, h->W4, h->epart);
+    TRT_TRY(hipGetLastError());
+    {
+        std::vector<double> hp(2 * (size_t)nb);
+        TRT_TRY(hipMemcpyAsync(hp.data(), h->epart, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, h->s));
+        TRT_TRY(hipStreamSynchronize(h->s));
+        E.gray_linear = E.emission = 0.0;
+        for (int k = 0; k < nb; k++) { E.gray_linear += hp[2 * k]; E.emission += hp[2 * k + 1]; }
+    }
+    const double rhs = E.emission + E.source;
+    E.balance = std::fabs(lb - rhs) / rhs;
+    return ALLIUM_OK;
+}
+
 }  // namespace
 
 int allium_trt_step(allium_trt_t h, allium_trt_info *info)
@@ -635,6 +803,12 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
     const int eb = (h->Ne + 63) / 64;
     // previous time level
     TRT_TRY(hipMemcpyAsync(h->Tn, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToDevice, s));
+    if (h->track) {
+        // int sum_g phi^n_g of D psi^n, the radiation energy the time source carries
+        double unused = 0.0;
+        ABI_TRY(allium_moments_device(h->L, h->psi, h->phit, s));
+        ABI_TRY(phi_sums(h, h->phit, unused, h->En));
+    }
     // the time source reads psi^n before the first sweep of the step overwrites it
     const long long cells = h->psi_size / N;
     hipLaunchKernelGGL(k_time_source, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, cells, 1.0 / h->d.dt,
@@ -659,7 +833,7 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
     // LinearizedTRTOperator::Mult linearized at the current T with the sources
     // of the previous time level; one pass without an outer solver
     for (k = 1;; k++) {
-        if (outer_max > 1)
+        if (outer_max > 1 || h->track)
             TRT_TRY(hipMemcpyAsync(h->Tk, h->T, sizeof(double) * h->Ndof, hipMemcpyDeviceToDevice, s));
         if (fix_on) ABI_TRY(allium_sweep_use_fixup(h->L, 0));
         // linearize at the current T
@@ -707,8 +881,10 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
         ABI_TRY(allium_sweep_mult_iso_device(h->L, h->qpsi, h->iso, h->psi, s));
         // temperature update
         hipLaunchKernelGGL(k_update_T, dim3((h->Ne + 255) / 256), dim3(256), 0, s, P, h->phi, h->resid, h->Binv,
-                           h->T, h->flags + 1);
+                           h->T, h->flags + 1, h->track ? h->Traw : nullptr);
         TRT_TRY(hipGetLastError());
+        // bookkeeping with the operator of this pass (before an implicit opacity update)
+        if (h->track) ABI_TRY(energy_terms(h, h->Tk));
         // implicit opacity: OpacityUpdate::Update after every pass (src/trt_linearized.cpp:311)
         if (h->d.implicit_opacity) ABI_TRY(update_opacity(h));
         if (outer_max == 1) break;
@@ -757,6 +933,33 @@ int allium_trt_step(allium_trt_t h, allium_trt_info *info)
         info->radiation_energy = E;
         ABI_TRY(allium_sweep_timing(h->L, &info->sweep_seconds, &info->sweeps));
     }
+    return ALLIUM_OK;
+}
+
+int allium_trt_track_energy(allium_trt_t h, int on)
+{
+    if (!h) return tfail(ALLIUM_ERR_ARG, "null handle");
+    h->track = on != 0;
+    h->energy = allium_trt_energy{};
+    return ALLIUM_OK;
+}
+
+int allium_trt_set_time_step(allium_trt_t h, double dt)
+{
+    if (!h) return tfail(ALLIUM_ERR_ARG, "null handle");
+    if (!(dt > 0) || !std::isfinite(dt)) return tfail(ALLIUM_ERR_ARG, "the time step must be positive");
+    // the time source, c_v/dt and the sweep's time absorption read it from the next step on
+    ABI_TRY(allium_sweep_set_time_absorption(h->L, 1.0 / dt));
+    h->d.dt = dt;
+    h->P.cvdt = h->d.cv / dt;
+    return ALLIUM_OK;
+}
+
+int allium_trt_energy_terms(allium_trt_t h, allium_trt_energy *out)
+{
+    if (!h || !out) return tfail(ALLIUM_ERR_ARG, "null argument");
+    if (!h->track) return tfail(ALLIUM_ERR_ARG, "energy bookkeeping is off (allium_trt_track_energy)");
+    *out = h->energy;
     return ALLIUM_OK;
 }
 

@@ -29,6 +29,12 @@ class TRTInfo(ctypes.Structure):
                 ("outer_residual", ctypes.c_double)]
 
 
+class TRTEnergy(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("absorption", "time_absorption", "leakage", "source", "inflow",
+                                                "emission_linear", "absorption_si", "gray_linear", "material",
+                                                "floor", "emission", "balance")]
+
+
 def planck_integrals(x):
     """(normalized Planck, Rosseland) integrals over [0, x] from the library."""
     p, r = ctypes.c_double(), ctypes.c_double()
@@ -71,6 +77,16 @@ class TRT:
         check(self._lib.allium_trt_step(self._h, ctypes.addressof(info)))
         return {k: getattr(info, k) for k, _ in TRTInfo._fields_}
 
+    def track_energy(self, on=True):
+        """Energy bookkeeping of every step (allium_trt_track_energy)."""
+        check(self._lib.allium_trt_track_energy(self._h, int(bool(on))))
+
+    def energy_terms(self):
+        """The last step's energy terms (allium_trt_energy), as a dict."""
+        e = TRTEnergy()
+        check(self._lib.allium_trt_energy_terms(self._h, ctypes.byref(e)))
+        return {k: getattr(e, k) for k, _ in TRTEnergy._fields_}
+
     def temperature(self):
         T = np.empty(self.Ndof)
         check(self._lib.allium_trt_temperature(self._h, T.ctypes.data_as(_lib.c_dp)))
@@ -100,17 +116,22 @@ class TRT:
         x, cyc, t = self.state()
         write_restart(root, x, output_cycle=output_cycle, simulation_cycle=cyc, time=t, time_step=self.dt, keep=keep)
 
+    def set_time_step(self, dt):
+        """Time step of the following steps (allium_trt_set_time_step)."""
+        check(self._lib.allium_trt_set_time_step(self._h, float(dt)))
+        self.dt = float(dt)
+
     def load_restart(self, base, restart_id=0, use_restart_dt=True):
         """LoadFromRestart into this solver; returns the file's metadata.  With
-        use_restart_dt the file's time step must be this solver's (scallion
-        then continues with it, exe/scallion.cpp:616-631)."""
+        use_restart_dt (scallion's default) the solver continues with the file's
+        time step, as scallion does (exe/scallion.cpp:616-674)."""
         from .restart import load_restart
         n = ctypes.c_long()
         check(self._lib.allium_trt_state_size(self._h, ctypes.byref(n)))
         r = load_restart(base, restart_id, size=n.value)
-        if use_restart_dt and r["time_step"] != self.dt:
-            raise ValueError(f"restart time step {r['time_step']} differs from this solver's {self.dt}")
         self.set_state(r["x"], r["simulation_cycle"], r["time"])
+        if use_restart_dt and r["time_step"] != self.dt:
+            self.set_time_step(r["time_step"])
         return r
 
     def set_launch(self, mode):

@@ -71,6 +71,36 @@ typedef struct {
     double outer_residual;      /* last ||T_k - T_{k-1}||_2 (0 with one pass) */
 } allium_trt_info;
 
+/* Energy bookkeeping of the last step, as rates (energy per unit time), for
+ * the operator of its last pass (that pass's opacities, linearized at its
+ * starting temperature T*).  Masses enter through their row sums:
+ *   absorption       sum_g int sigma_g phi_g, phi = D psi^{n+1} of the final sweep
+ *   time_absorption  (1/dt) int sum_g phi_g^{n+1}
+ *   leakage          outflow of psi^{n+1}: AdvectionOperator::ComputeBalance
+ *                    (src/sweep.cpp:1400-1462) minus the two absorptions
+ *   source           sum D q_psi = inflow + (1/dt) int sum_g (D psi^n)_g
+ *   inflow           sum D q_in
+ *   emission_linear  sum of the emission source of the final sweep (em + Ms phi)
+ *   absorption_si    sum_g int sigma_g phi_g of the source iteration's phi
+ *   gray_linear      sum of the gray emission at T* plus its Jacobian times (T_raw - T*),
+ *                    T_raw = T* + Bt'^{-1}(M_sigma phi + resid) the update before the floor
+ *   material         int c_v (T^{n+1} - T^n) / dt
+ *   floor            int c_v (T^{n+1} - T_raw) / dt: the energy the 1e-8 floor added
+ *   emission         sum of PlanckEmissionNFI at T^{n+1} (scallion's emission form)
+ *   balance          scallion's ComputeBalance (exe/scallion.cpp:157-173):
+ *                    |absorption + time_absorption + leakage - (emission + source)|
+ *                    / (emission + source)
+ * A linearized backward-Euler step satisfies, to rounding:
+ *   absorption + time_absorption + leakage = source + emission_linear  (final sweep)
+ *   material - floor = absorption_si - gray_linear                   (temperature update)
+ * and their sum is total energy conservation up to the floor, the two emission
+ * rules' difference (emission_linear - gray_linear) and the source-iteration
+ * tolerance (absorption_si - absorption). */
+typedef struct {
+    double absorption, time_absorption, leakage, source, inflow;
+    double emission_linear, absorption_si, gray_linear, material, floor, emission, balance;
+} allium_trt_energy;
+
 int allium_trt_create(const allium_mesh_desc *mesh, int fe_order, const allium_quad_desc *quad,
                       const allium_trt_desc *desc, allium_trt_t *out);
 void allium_trt_destroy(allium_trt_t h);
@@ -83,10 +113,20 @@ int allium_trt_scalar_flux(allium_trt_t h, double *phi);
  * the block vector x = [psi (psi_size), T (Ne * n)] with the step count and
  * time.  set_state recomputes the scalar flux and the opacities from it, so a
  * run continued from a saved state repeats the uninterrupted run bit for bit
- * (the time step is the descriptor's). */
+ * (the time step stays the solver's; allium_trt_set_time_step changes it). */
 int allium_trt_state_size(allium_trt_t h, long *size);
 int allium_trt_get_state(allium_trt_t h, double *x, int *cycle, double *time);
 int allium_trt_set_state(allium_trt_t h, const double *x, int cycle, double time);
+/* Energy bookkeeping on (1) or off (0, the default): when on, each step also
+ * computes the terms of allium_trt_energy (one extra pass over psi^n and two
+ * over psi^{n+1}). */
+int allium_trt_track_energy(allium_trt_t h, int on);
+/* the terms of the last step; ALLIUM_ERR_ARG while the bookkeeping is off */
+int allium_trt_energy_terms(allium_trt_t h, allium_trt_energy *out);
+/* Change the time step from the next step on (scallion adopts a restart
+ * file's time step, exe/scallion.cpp:616-674): the time absorption 1/dt, the
+ * time source and c_v/dt follow it. */
+int allium_trt_set_time_step(allium_trt_t h, double dt);
 /* the sweep handle the solver uses (tuning, timing, fixups) */
 allium_sweep_t allium_trt_sweep(allium_trt_t h);
 

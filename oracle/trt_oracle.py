@@ -14,7 +14,12 @@ Restates, in FP64 numpy on top of the C sweep oracle:
     inflow + (1/(c dt)) Mt psi^n, explicit opacity recomputed after each step.
 Dimensionless units: c = 1 and the black-body emission is T^4.
 Pinned in tests/test_trt.py by numerical quadrature of the Planck and
-Rosseland integrands and by the radiative-equilibrium invariant.
+Rosseland integrands, by the radiative-equilibrium invariant, by central
+finite differences of the group and gray emissions (the linearization's
+Jacobians) and by the energy identities of one linearized backward-Euler step
+(energy_terms): the final sweep's DG balance, the material energy update, and
+their sum, total energy conservation up to the emission-rule mismatch and the
+source-iteration tolerance.
 """
 import numpy as np
 
@@ -155,6 +160,7 @@ class TRTOracle:
         pr = self.pr
         Tn, psin = self.T.copy(), self.psi.copy()
         qpsi = self.qin + (1.0 / self.dt) * (psin.reshape(-1, 8) @ self.Mm.T).ravel()
+        self.Tn, self.psin, self.qpsi = Tn, psin, qpsi
         maxit, abstol, reltol, implicit = self.outer
         r0 = 0.0
         for k in range(1, maxit + 1):
@@ -181,6 +187,7 @@ class TRTOracle:
         pr, G, Ne = self.pr, self.G, self.Ne
         cvdt = self.cv / self.dt
         T = self.T
+        self.T_star, self.sigma_pass = T.copy(), self.sigma.copy()
         # the fixup is off for the transport solve (src/trt_linearized.cpp:236-238)
         pr.set_fixup("none")
         # linearization (rule of order 4 for the gray balance, 3 for the groups)
@@ -214,9 +221,71 @@ class TRTOracle:
                 break
         self.iterations.append(it)
         em = em + self._Ms(phi, Binv, dpw)
+        self.em_final = em
         if self.fixup is not None:
             pr.set_fixup(*self.fixup)
         self.psi = pr.sweep(qpsi + pr.DT(em.ravel()))
         t = resid + np.einsum("ge,gei->ei", self.sigma, phi.reshape(G, Ne, 8) @ self.Mu.T)
-        self.T = np.maximum(T + np.einsum("eij,ej->ei", Binv, t), 1e-8)
+        self.T_raw = T + np.einsum("eij,ej->ei", Binv, t)
+        self.T = np.maximum(self.T_raw, 1e-8)
         self.phi = phi
+
+    # ---- emissions at a given temperature (the linearization's functions) ----
+    def group_emission(self, T, sigma=None):
+        """PlanckEmissionNFI on rule 3: (G, Ne, 8) int sigma_g b_g(T) T^4 phi_i."""
+        sig = (self.sigma_pass if sigma is None else sigma).T
+        Tq = np.asarray(T).reshape(self.Ne, 8) @ self.phi3.T
+        sp = spectrum(self.bounds, Tq)
+        return np.einsum("q,eqg,eg,eq,qi->gei", self.w3, sp, sig, Tq ** 4, self.phi3)
+
+    def gray_emission(self, T, sigma=None):
+        """GrayPlanckEmissionNFI on rule 4: (Ne, 8) int sum_g sigma_g b_g(T) T^4 phi_i."""
+        sig = (self.sigma_pass if sigma is None else sigma).T
+        Tq = np.asarray(T).reshape(self.Ne, 8) @ self.phi4.T
+        sp = spectrum(self.bounds, Tq)
+        return np.einsum("q,eqg,eg,eq,qi->ei", self.w4, sp, sig, Tq ** 4, self.phi4)
+
+    def energy_terms(self):
+        """Energy bookkeeping of the last step (all rates: energy per unit time),
+        for the operator of its last pass (opacities sigma_pass, linearized at
+        T_star).  With rowsums of the element masses:
+          absorption       sum_g int sigma_g phi_g        phi = D psi^{n+1}
+          time_absorption  (1/dt) int sum_g phi_g^{n+1}
+          leakage          outflow of psi^{n+1} (AdvectionOperator::ComputeBalance
+                           minus its absorption terms)
+          source           sum D q_psi = inflow + (1/dt) int sum_g phi^n_g
+          inflow           sum D q_in
+          emission_linear  sum of the emission source of the final sweep (em + Ms phi)
+          absorption_si    sum_g int sigma_g phi_g of the source iteration's phi
+          material         int c_v (T^{n+1} - T^n) / dt
+          floor            int c_v (T^{n+1} - T_raw) / dt, what the 1e-8 floor added to
+                           the update T_raw = T* + Bt'^{-1}(M_sigma phi + resid)
+          emission         sum of PlanckEmissionNFI at T^{n+1} (scallion's emission form)
+          balance          scallion's ComputeBalance (exe/scallion.cpp:157-173):
+                           |absorption + time_absorption + leakage - (emission + source)|
+                           / (emission + source)
+        """
+        pr, G, Ne = self.pr, self.G, self.Ne
+        tau = 1.0 / self.dt
+        rMm, rMu = self.Mm.sum(axis=1), self.Mu.sum(axis=1)
+        phif = pr.D(self.psi).reshape(G, Ne, 8)
+        phin = pr.D(self.psin).reshape(G, Ne, 8)
+        sig = self.sigma_pass
+        absorption = float(np.einsum("ge,gei,i->", sig, phif, rMu))
+        t_abs = tau * float(np.einsum("gei,i->", phif, rMm))
+        # ComputeBalance with the step's operator: absorption (sigma + tau) + leakage
+        pr.set_sigma(sig)
+        lb = pr.balance(self.psi)
+        pr.set_sigma(self.sigma)
+        leakage = lb - absorption - t_abs
+        source = float(pr.D(self.qpsi).sum())
+        inflow = float(pr.D(self.qin).sum())
+        phis = self.phi.reshape(G, Ne, 8)
+        out = dict(absorption=absorption, time_absorption=t_abs, leakage=leakage, source=source, inflow=inflow,
+                   emission_linear=float(self.em_final.sum()),
+                   absorption_si=float(np.einsum("ge,gei,i->", sig, phis, rMu)),
+                   material=self.cv / self.dt * float(np.einsum("ei,i->", self.T - self.Tn, rMm)),
+                   floor=self.cv / self.dt * float(np.einsum("ei,i->", self.T - self.T_raw, rMm)),
+                   emission=float(self.group_emission(self.T).sum()))
+        out["balance"] = abs(lb - (out["emission"] + source)) / (out["emission"] + source)
+        return out

@@ -120,3 +120,69 @@ def test_trt_fixup_only_in_the_final_sweep():
     on = ctypes.c_int()
     _lib.check(lib.allium_sweep_get_fixup(L, None, None, ctypes.byref(on)))
     assert on.value == 1
+
+
+def _gpu_identities(e):
+    """(R) final-sweep balance and (M) material update defects, relative."""
+    dR = abs(e["absorption"] + e["time_absorption"] + e["leakage"] - e["source"] - e["emission_linear"]) / \
+        (e["source"] + e["emission_linear"])
+    dM = abs(e["material"] - e["floor"] - (e["absorption_si"] - e["gray_linear"])) / max(abs(e["absorption_si"]),
+                                                                                          abs(e["gray_linear"]))
+    return dR, dM
+
+
+def test_trt_energy_terms_match_oracle():
+    """The GPU's energy bookkeeping (allium_trt_energy) equals the oracle's
+    terms, and scallion's balance value, over three off-equilibrium steps."""
+    print("trt energy parity", flush=True)
+    nel, G = (4, 3, 3), 4
+    om, w = ls_fixture(4, 3)
+    o = O.TRTOracle(nel, om, w, G=G, T0=0.01, T_inflow=1.0, inflow_sides=1, si_reltol=1e-12)
+    g = trt.TRT(nel, sn=4, groups=G, T0=0.01, T_inflow=1.0, inflow_sides=1, si_reltol=1e-12)
+    g.track_energy(True)
+    for step in range(3):
+        o.step()
+        g.step()
+        eo, eg = o.energy_terms(), g.energy_terms()
+        for k in ("absorption", "time_absorption", "leakage", "source", "inflow", "emission_linear",
+                  "absorption_si", "material", "floor", "emission"):
+            assert abs(eg[k] - eo[k]) <= 1e-10 * max(abs(eo[k]), 1e-3 * eo["source"]), (step, k, eg[k], eo[k])
+        assert abs(eg["balance"] - eo["balance"]) < 1e-9
+        dR, dM = _gpu_identities(eg)
+        print(f"  step {step + 1}: R {dR:.1e} M {dM:.1e} balance {eg['balance']:.3e}")
+        assert dR < 1e-11 and dM < 1e-11
+
+
+def test_trt_config4_conserves_energy_over_20_steps():
+    """Config 4 (32^3, p=1, S6, 16 groups, T0 = 0.01, Planckian T = 1 inflow on
+    x = 0, dt = 1e-3) for its 20 steps: each step's final-sweep balance and
+    material update hold to rounding, and total energy,
+      (E_mat + E_rad)^{n+1} - (E_mat + E_rad)^n = dt (inflow - leakage)
+        + dt (emission-rule mismatch) + dt (source-iteration defect),
+    holds to rounding.  The two remaining terms are the discretization's, not
+    errors, and stay small: the mismatch of the rule 2p+1 group emission against
+    the rule 2p+2 gray emission (the reference's two integrators; it grows to
+    ~1e-4 of the inflow as the heating front steepens) below 1e-2 of the inflow,
+    and the SI defect (phi of the source iteration, stopped at 1e-8 of its first
+    change, against D psi of the final sweep) below 1e-4 of it."""
+    print("trt config 4 energy", flush=True)
+    g = trt.TRT((32, 32, 32), sn=6, groups=16, si_reltol=1e-8)
+    g.set_launch(-1)
+    g.track_energy(True)
+    for step in range(20):
+        info = g.step()
+        e = g.energy_terms()
+        dR, dM = _gpu_identities(e)
+        lhs = e["material"] - e["floor"] + e["time_absorption"] - (e["source"] - e["inflow"]) + e["leakage"] - \
+            e["inflow"]
+        mismatch = e["emission_linear"] - e["gray_linear"]
+        si = e["absorption_si"] - e["absorption"]
+        dE = abs(lhs - mismatch - si) / e["source"]
+        if step % 4 == 3 or step == 0:
+            print(f"  step {step + 1}: R {dR:.1e} M {dM:.1e} E {dE:.1e} mismatch/inflow {mismatch / e['inflow']:.1e} "
+                  f"si/inflow {si / e['inflow']:.1e} floored {info['floored']} floor/inflow "
+                  f"{e['floor'] / e['inflow']:.1e} balance {e['balance']:.3e} its {info['transport_iterations']}",
+                  flush=True)
+        assert (info["floored"] == 0) == (e["floor"] == 0)
+        assert dR < 1e-10 and dM < 1e-10 and dE < 1e-10
+        assert abs(mismatch) < 1e-2 * e["inflow"] and abs(si) < 1e-4 * e["inflow"]

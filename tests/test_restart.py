@@ -74,6 +74,17 @@ def test_trt_continues_from_restart_bit_for_bit(tmp_path):
     assert cyc_a == cyc_c == 4 and info["step"] == 4 and ta == tc
     assert np.array_equal(xa, xc)
     assert np.array_equal(a.scalar_flux(), c.scalar_flux())
+    # scallion adopts the file's time step (use_restart_dt, its default): a solver
+    # created with another dt continues exactly as the uninterrupted run
     d = TRT(nel=(6, 6, 6), sn=4, groups=4, dt=2e-3)
-    with pytest.raises(ValueError, match="time step"):
-        d.load_restart(str(tmp_path), 0)
+    d.load_restart(str(tmp_path), 0)
+    assert d.dt == 1e-3
+    for _ in range(2):
+        d.step()
+    xd, cyc_d, td = d.state()
+    assert cyc_d == 4 and np.array_equal(xa, xd)
+    # without it the solver keeps its own time step
+    e = TRT(nel=(6, 6, 6), sn=4, groups=4, dt=2e-3)
+    e.load_restart(str(tmp_path), 0, use_restart_dt=False)
+    e.step()
+    assert e.dt == 2e-3 and abs(e.state()[2] - (2e-3 + 2e-3)) < 1e-15

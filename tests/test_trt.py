@@ -65,3 +65,79 @@ def test_oracle_keeps_radiative_equilibrium():
         eq = np.diff(O.planck_integral(o.bounds))
         phi = o.phi.reshape(o.G, -1)
         assert np.allclose(phi[1:], eq[1:, None], rtol=1e-9, atol=0)
+
+
+def _fd_linear(f, T_star, dT, eps=1e-4):
+    """f(T*) + f'(T*) dT summed, with the derivative by central differences."""
+    return f(T_star).sum() + (f(T_star + eps * dT).sum() - f(T_star - eps * dT).sum()) / (2 * eps)
+
+
+def _identities(o):
+    """The energy identities of the oracle's last step; returns the relative
+    defects of (R) the final sweep's balance, (M) the material update and (E)
+    total energy, with the linearized emissions evaluated here by finite
+    differences of the emission functions (independent of the step's Jacobians)."""
+    e = o.energy_terms()
+    dT = o.T_raw - o.T_star
+    gl = _fd_linear(o.gray_emission, o.T_star, dT)
+    el = _fd_linear(o.group_emission, o.T_star, dT)
+    scale = e["source"] + e["emission_linear"]
+    dR = abs(e["absorption"] + e["time_absorption"] + e["leakage"] - e["source"] - el) / scale
+    dM = abs(e["material"] - e["floor"] - (e["absorption_si"] - gl)) / max(abs(e["absorption_si"]), abs(gl))
+    # total energy: (E_mat^{n+1} - E_mat^n + E_rad^{n+1} - E_rad^n) / dt
+    #   = inflow - leakage + (emission mismatch of the two rules) + (SI tolerance)
+    lhs = e["material"] - e["floor"] + e["time_absorption"] - (e["source"] - e["inflow"]) + e["leakage"] - e["inflow"]
+    rhs = (el - gl) + (e["absorption_si"] - e["absorption"])
+    dE = abs(lhs - rhs) / scale
+    return e, dR, dM, dE, el, gl
+
+
+def test_oracle_linearized_step_conserves_energy():
+    """One linearized backward-Euler step off equilibrium (T0 = 0.01, a T = 1
+    Planckian inflow on x = 0): the final sweep's particle balance with the
+    linearized emission, the material energy update and total energy all hold
+    to the source-iteration tolerance, with the emissions' Jacobians checked
+    by finite differences of the Planck group and gray emissions."""
+    om, w = ls_fixture(4, 3)
+    o = O.TRTOracle((4, 3, 3), om, w, G=4, emin=1e-2, emax=1e1, coef=10.0, cv=1.0, dt=1e-3, T0=0.01, T_inflow=1.0,
+                    inflow_sides=1, si_reltol=1e-13)
+    for _ in range(3):
+        o.step()
+        e, dR, dM, dE, el, gl = _identities(o)
+        assert dR < 1e-11 and dM < 1e-7 and dE < 1e-7, (dR, dM, dE)
+        assert abs(el - e["emission_linear"]) < 1e-7 * abs(el)
+        # the step moves energy: the identities are not satisfied trivially
+        assert abs(e["material"]) > 1e-3 * e["source"] and e["leakage"] > 0
+        assert 0 <= e["balance"] < 1
+
+
+def test_oracle_energy_identity_catches_wrong_heat_capacity_factor():
+    """A wrong c_v/dt in the temperature update (planted in the oracle) breaks
+    the material identity."""
+    om, w = ls_fixture(4, 3)
+
+    class Wrong(O.TRTOracle):
+        def _pass(self, Tn, qpsi):
+            self.cv *= 2.0
+            try:
+                super()._pass(Tn, qpsi)
+            finally:
+                self.cv /= 2.0
+
+    o = Wrong((4, 3, 3), om, w, G=4, T0=0.01, T_inflow=1.0, inflow_sides=1, si_reltol=1e-13)
+    o.step()
+    _, dR, dM, dE, _, _ = _identities(o)
+    assert dM > 1e-3 and dE > 1e-4
+
+
+def test_oracle_energy_identity_catches_wrong_emission_derivative(monkeypatch):
+    """The Rosseland group fractions replaced by the Planck ones (a wrong dB/dT,
+    planted in the oracle) break the identities that use the finite-difference
+    Jacobians."""
+    om, w = ls_fixture(4, 3)
+    monkeypatch.setattr(O, "rosseland_integral", O.planck_integral)
+    o = O.TRTOracle((4, 3, 3), om, w, G=4, T0=0.01, T_inflow=1.0, inflow_sides=1, si_reltol=1e-13)
+    o.step()
+    monkeypatch.undo()
+    _, dR, dM, dE, _, _ = _identities(o)
+    assert max(dR, dM) > 1e-4
