@@ -97,6 +97,65 @@ def test_linear_function_gives_boundary_flux(dim, p):
     np.testing.assert_allclose(y, expect, atol=1e-13)
 
 
+@pytest.mark.parametrize("dim,sym", [(2, -1.0), (3, -1.0), (3, 1.0)])
+def test_scalar_mip_on_elementwise_constants(dim, sym):
+    """The scalar MIP operator (MIPDiffusionIntegrator, src/moment_integrators.cpp:3-195)
+    on element-wise constant f, uniform mesh, p = 1, every side Marshak.  For
+    node i of element e, vol_i = (h/2)^dim its volume integral, A = (h/2)^(dim-1)
+    its face integral on a side through it, and each interior side s of e
+    with neighbour value f_s:
+      (K f)_i = sigma_a vol_i f_e
+              + sum_s [i on s] pen_s A (f_e - f_s)                 (penalty)
+              + sum_s sym (D_e / 2) (A / h) s_i (f_e - f_s)        (consistency)
+              + sum_{Marshak s through i} alpha A f_e,
+    with pen_s = max(kappa (D_e + D_s) / 2 / h, alpha / 2), D = 1/(3 sigma_t),
+    kappa = (p+1)^2 and s_i = +1 for i on s, -1 for i on the opposite side
+    (the normal derivative of phi_i through s is s_i / h on the face); the
+    jump of f and the gradient of phi_i are the only nonzero factors.
+    sigma_t spans both branches of the penalty's max."""
+    n_el = 3 if dim == 3 else 5
+    nel, vert = _uniform(dim, n_el)
+    rng = np.random.default_rng(31)
+    ne = n_el ** dim
+    st = np.exp(rng.uniform(np.log(0.2), np.log(300.0), ne))
+    ss = st * rng.uniform(0.1, 0.9, ne)
+    om, w = ls_fixture(4, dim)
+    alpha = M.compute_alpha(om, w)
+    kappa = 4.0
+    K = M.mip_matrix(dim, nel, vert, 1, st, ss, om, w, sym=sym)
+    h = 1.0 / n_el
+    nn = 2 ** dim
+    vol, A = (h / 2) ** dim, (h / 2) ** (dim - 1)
+    Dc = 1.0 / (3.0 * st)
+    f = rng.normal(size=ne)
+    y = K @ np.repeat(f, nn)
+    expect = np.zeros_like(y)
+    floors = 0
+    for e in range(ne):
+        c = [(e // n_el ** d) % n_el for d in range(dim)]
+        for i in range(nn):
+            b = [(i >> d) & 1 for d in range(dim)]
+            v = (st[e] - ss[e]) * vol * f[e]
+            for ax in range(dim):
+                for hi in (0, 1):
+                    on = b[ax] == hi
+                    if c[ax] == (n_el - 1) * hi:            # boundary side: Marshak
+                        v += alpha * A * f[e] if on else 0.0
+                        continue
+                    nb = e + (n_el ** ax) * (1 if hi else -1)
+                    pen = kappa * 0.5 * (Dc[e] + Dc[nb]) / h
+                    if pen < alpha / 2:
+                        pen = alpha / 2
+                        floors += 1
+                    jump = f[e] - f[nb]
+                    if on:
+                        v += pen * A * jump
+                    v += sym * 0.5 * Dc[e] * (A / h) * (1.0 if on else -1.0) * jump
+            expect[e * nn + i] = v
+    assert floors > 0 and floors < 4 * ne * dim * nn      # both branches of the max
+    np.testing.assert_allclose(y, expect, rtol=1e-12, atol=1e-14 * np.abs(expect).max())
+
+
 @pytest.mark.parametrize("dim,p,jitter", [(2, 1, 0.0), (3, 1, 0.0), (3, 1, 0.15), (2, 2, 0.2), (3, 2, 0.1)])
 def test_oracle_matrix_is_spd(dim, p, jitter):
     n = 3
