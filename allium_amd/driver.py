@@ -25,9 +25,15 @@ Deck keys (YAML):
       # or {type: mip} (the block system) / {type: p1sa, solver: {type: direct}}; a "direct"
       # inner solve is CG (scalar mip) or GMRES (mip, p1sa) to a relative residual of 1e-12
 
-Material `source` and boundary `value` are per-steradian isotropic densities,
-as in chive (exe/chive.cpp:236-247,294-306).  Boundary type "reflective"
-maps to a reflective side.
+Material `source` and boundary `value` are per-steradian densities, as in
+chive (exe/chive.cpp:236-247,294-306): a number, or an expression of the
+position (x, y, z) and direction (mu, eta, xi) standing in for chive's Lua
+functions, with numpy's sin, cos, exp, log, sqrt, ..., pi, the deck's
+`parameters` and its `definitions` (named expressions, evaluated in order).
+`scalar_flux_solution` (an expression of x, y, z) and `current_solution` (one
+expression per axis) make the driver print chive's "L2 error" of the scalar
+flux and the current J = sum_a w_a Omega_a psi_a (exe/chive.cpp:1036-1072).
+Boundary type "reflective" maps to a reflective side.
 
 Usage: ``python -m allium_amd.driver deck.yaml`` prints the output YAML.
 """
@@ -39,6 +45,42 @@ import numpy as np
 import yaml
 
 import allium_amd as A
+from allium_amd import fem
+
+
+_MATH = {k: getattr(np, k) for k in ("sin", "cos", "tan", "exp", "log", "sqrt", "abs", "tanh", "sinh", "cosh",
+                                     "arctan", "arcsin", "arccos", "minimum", "maximum", "where", "pi", "e")}
+
+
+class _Expressions:
+    """Deck expressions of (x, y, z, mu, eta, xi) with the deck's parameters and
+    definitions, evaluated on arrays of points (..., dim) and one direction."""
+
+    def __init__(self, deck):
+        self.params = {str(k): float(v) for k, v in (deck.get("parameters") or {}).items()}
+        self.defs = [(str(k), compile(str(v), f"<definition {k}>", "eval"))
+                     for k, v in (deck.get("definitions") or {}).items()]
+
+    def function(self, value, what):
+        """A constant (a number) or a callable f(X, omega)."""
+        if not isinstance(value, str):
+            return float(value)
+        code = compile(value, f"<{what}>", "eval")
+
+        def f(X, om=None):
+            ns = dict(_MATH)
+            ns.update(self.params)
+            X = np.asarray(X, float)
+            for d, nm in enumerate(("x", "y", "z")):
+                ns[nm] = X[..., d] if d < X.shape[-1] else np.zeros(X.shape[:-1])
+            om = np.zeros(3) if om is None else np.asarray(om, float)
+            for d, nm in enumerate(("mu", "eta", "xi")):
+                ns[nm] = float(om[d]) if d < om.size else 0.0
+            for nm, c in self.defs:
+                ns[nm] = eval(c, {"__builtins__": {}}, ns)
+            return np.broadcast_to(np.asarray(eval(code, {"__builtins__": {}}, ns), float), X.shape[:-1])
+
+        return f
 
 
 def _per_element(value, ne, what):
@@ -57,17 +99,19 @@ def run(deck):
     ext = tuple(float(v) for v in mesh_d.get("extents", [1.0] * dim))
     ne = int(np.prod(nel))
     mats = deck["materials"]
+    ex = _Expressions(deck)
     names = _per_element(deck.get("material_map", next(iter(mats))), ne, "material_map")
     sigma_t = np.array([float(mats[m]["total"]) for m in names])
     sigma_s = np.array([float(mats[m].get("scattering", 0.0)) for m in names])
-    Q = np.array([float(mats[m].get("source", 0.0)) for m in names])
+    src_f = {m: ex.function(mats[m].get("source", 0.0), f"materials.{m}.source") for m in mats}
+    Q = np.array([src_f[m] if not callable(src_f[m]) else 0.0 for m in names])
     bcs = deck.get("boundary_conditions", {"vacuum": {"type": "inflow", "value": 0.0}})
     bmap = deck.get("boundary_map", next(iter(bcs)))
     side_names = [bmap] * (2 * dim) if isinstance(bmap, str) else list(bmap)
     if len(side_names) != 2 * dim:
         raise ValueError(f"boundary_map needs one name or {2 * dim} names")
-    bc, inflow = [], []
-    for nm in side_names:
+    bc, inflow, inflow_f = [], [], {}
+    for side, nm in enumerate(side_names):
         spec = bcs[nm]
         typ = str(spec.get("type", "inflow")).lower()
         if typ in ("reflective", "reflect"):
@@ -75,7 +119,12 @@ def run(deck):
             inflow.append(0.0)
         elif typ in ("inflow", "vacuum"):
             bc.append(A.BC_INFLOW)
-            inflow.append(float(spec.get("value", 0.0)))
+            v = ex.function(spec.get("value", 0.0), f"boundary_conditions.{nm}.value")
+            if callable(v):
+                inflow_f[side] = v
+                inflow.append(0.0)
+            else:
+                inflow.append(v)
         else:
             raise ValueError(f"boundary condition type {typ} is not supported")
     sn = deck.get("sn", {"type": "level symmetric", "order": 4})
@@ -107,6 +156,19 @@ def run(deck):
     Linv = A.InverseAdvectionOperator(mesh, p, int(sn.get("order", 4)), sigma_t.reshape(1, -1))
     Linv.set_launch(-1)
     q = Linv.form_source(Q=Q.reshape(1, -1), inflow=np.array(inflow).reshape(1, -1))
+    space = None
+    funcs = [m for m in mats if callable(src_f[m])]
+    if funcs or inflow_f or "scalar_flux_solution" in deck or "current_solution" in deck:
+        space = fem.BoxSpace(nel, p, extents=ext)
+    if funcs or inflow_f:
+        # function sources and inflows: FormTransportSource's integrals on the host
+        om, _ = Linv.quadrature()
+        qa = q.reshape(len(om), ne, -1)
+        for m in funcs:
+            sel = np.array([nm == m for nm in names])
+            qa[:, sel] += space.project_source(src_f[m], om).reshape(len(om), ne, -1)[:, sel]
+        for side, f in inflow_f.items():
+            qa += space.project_inflow(f, om, [side]).reshape(len(om), ne, -1)
     accel = None
     out["preconditioner"] = "none"
     if ptype == "p1sa":
@@ -187,6 +249,24 @@ def run(deck):
     out["timings"] = {"setup": out["setup time"], "solve": {"total": f"{solve:.3e} s", "sweep": f"{sweep_s:.3e} s"}}
     out["scalar flux"] = {"l2": float(np.linalg.norm(phi)), "min": float(phi.min()), "max": float(phi.max())}
     out["balance"] = float(A.AdvectionOperator(Linv).ComputeBalance(psi))
+    # chive's "L2 error" (exe/chive.cpp:1036-1072), three significant digits
+    errs = {}
+    if "scalar_flux_solution" in deck:
+        f = ex.function(str(deck["scalar_flux_solution"]), "scalar_flux_solution")
+        errs["scalar flux"] = float(f"{space.l2_error(phi, f):.3e}")
+    if "current_solution" in deck:
+        comps = deck["current_solution"]
+        if len(comps) != dim:
+            raise ValueError(f"current_solution needs {dim} expressions")
+        om, w = Linv.quadrature()
+        pa = psi.reshape(len(w), -1)
+        e2 = 0.0
+        for d in range(dim):
+            Jd = (w * om[:, d]) @ pa
+            e2 += space.l2_error(Jd, ex.function(str(comps[d]), f"current_solution[{d}]")) ** 2
+        errs["current"] = float(f"{np.sqrt(e2):.3e}")
+    if errs:
+        out["L2 error"] = errs
     return out
 
 

@@ -102,3 +102,49 @@ def test_krylov_decks(solver):
     print(yaml.safe_dump({k: out[k] for k in ("outer iterations", "inner iteration")}))
     assert out["converged"] and out["outer iterations"] < 12
     assert out["inner iteration"]["total"] > 0
+
+
+MMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "decks", "mms.yaml")
+
+
+def test_deck_expressions():
+    """Deck expressions stand in for chive's Lua functions: parameters and
+    definitions in order, numpy's math, no builtins."""
+    with open(MMS) as fh:
+        d = yaml.safe_load(fh)
+    ex = driver._Expressions(d)
+    X = np.array([[0.1, 0.2], [0.3, 0.7]])
+    om = np.array([0.35002117, 0.35002117, 0.8688903])
+    psi = ex.function(d["definitions"]["psi"], "psi")(X, om)
+    x, y = X[:, 0], X[:, 1]
+    mu, eta = om[0], om[1]
+    ref = (np.sin(np.pi * x) * np.sin(np.pi * y) + 0.5 * (mu + eta) * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y)
+           + 0.25 * (mu ** 2 + eta ** 2) * np.sin(3 * np.pi * x) * np.sin(3 * np.pi * y) + 2.0) / 4 / np.pi
+    np.testing.assert_allclose(psi, ref, rtol=1e-15)
+    assert ex.function(2.5, "c") == 2.5
+    with pytest.raises(NameError):
+        ex.function("open('x')", "bad")(X, om)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["mip", "ldgsa", "p1sa"])
+def test_mms_deck_order_of_accuracy(prec):
+    """test/chive/test_ooa.py on test/chive/mms.lua as chive.mms.{mip, ldgsa,
+    p1sa} run it (test/chive/CMakeLists.txt:31-48): GMRES to 1e-10 with the
+    preconditioner and a direct inner solve, num_elements {Ne, Ne} for Ne = 10
+    and 20; the "L2 error" of the scalar flux and of the current both fall at
+    order 2 within 0.2."""
+    print(f"driver mms {prec}", flush=True)
+    errs = []
+    for ne in (10, 20):
+        with open(MMS) as fh:
+            d = yaml.safe_load(fh)
+        d["mesh"]["num_elements"] = [ne, ne]
+        d["driver"]["preconditioner"] = {"type": prec, "solver": {"type": "direct"}}
+        out = yaml.safe_load(yaml.safe_dump(driver.run(d), sort_keys=False))   # through the printed YAML
+        assert out["converged"]
+        errs.append((out["L2 error"]["scalar flux"], out["L2 error"]["current"]))
+    phi_ooa = np.log2(errs[0][0] / errs[1][0])
+    J_ooa = np.log2(errs[0][1] / errs[1][1])
+    print(f"  phi {phi_ooa:.3f} ({errs[0][0]:.3e}, {errs[1][0]:.3e}), J {J_ooa:.3f} ({errs[0][1]:.3e}, {errs[1][1]:.3e})")
+    assert abs(phi_ooa - 2) < 0.2 and abs(J_ooa - 2) < 0.2
