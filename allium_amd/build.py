@@ -6,6 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "liballium_amd.so")
+SHIM = os.path.join(HERE, "lib", "liballium_operator.so")
 SOURCES = ["csrc/discretization.cpp", "csrc/sweep_api.cpp", "csrc/sweep_kernels.hip", "csrc/box_sweep.hip", "csrc/cnn.hip", "csrc/trt.hip", "csrc/coop_sweep.hip", "csrc/dsa.hip", "csrc/dsa_api.cpp", "csrc/krylov.hip", "csrc/krylov_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -40,7 +41,21 @@ def build(verbose=False, jobs=8):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
+    build_operator_shim(verbose)
     return LIB
+
+
+def build_operator_shim(verbose=False):
+    """lib/liballium_operator.so: the C++ adapter (include/allium_operator.hpp)
+    behind one C entry, host code built with g++ against liballium_amd.so."""
+    libdir = os.path.join(HERE, "lib")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HERE, "csrc", "operator_shim.cpp"), "-L" + libdir, "-lallium_amd", "-Wl,-rpath,$ORIGIN",
+           "-o", SHIM]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return SHIM
 
 
 if __name__ == "__main__":
