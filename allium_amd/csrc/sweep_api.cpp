@@ -577,6 +577,8 @@ int allium_sweep_set_sigma(allium_sweep_t h, const double *sigma_t)
     const long n = (long)h->G * h->mesh.Ne;
     if (!finite_all(sigma_t, n)) return fail(ALLIUM_ERR_NUMERIC, "non-finite sigma_t");
     h->sigma.assign(sigma_t, sigma_t + n);
+    // sweeps still queued on the handle's stream read the old values
+    HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMemcpy(h->d_sigma, sigma_t, sizeof(double) * n, hipMemcpyHostToDevice));
     return ALLIUM_OK;
 }
@@ -585,6 +587,7 @@ int allium_sweep_set_time_absorption(allium_sweep_t h, double tau)
 {
     if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
     if (!std::isfinite(tau)) return fail(ALLIUM_ERR_NUMERIC, "non-finite time absorption");
+    HIP_TRY(hipStreamSynchronize(h->stream));
     h->ds.prm.tau = tau;
     h->ds.prm.timedep = 1;
     if (h->mesh.uniform) {
@@ -1108,7 +1111,10 @@ static int source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         return ALLIUM_OK;
     };
     if (opt->precision == ALLIUM_FP32_REFINED) {
-        // defect correction: psi += L32^{-1} (q + D^T Ms phi - L psi), residuals in FP64
+        // defect correction: psi += L32^{-1} (q + D^T Ms phi - L psi), residuals in FP64.
+        // A fixup would clip the correction, which may legitimately be negative.
+        if (h->ds.fix_on)
+            return fail(ALLIUM_ERR_UNSUPPORTED, "FP32-refined source iteration with a negative-flux fixup");
         HIP_TRY(h->c.ensure(sizeof(double) * nps));
         HIP_TRY(hipMemsetAsync(h->a.p, 0, sizeof(double) * nps, s));
         HIP_TRY(hipMemsetAsync(h->x.p, 0, sizeof(double) * np, s));

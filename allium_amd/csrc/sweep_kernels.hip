@@ -1012,6 +1012,18 @@ __global__ void __launch_bounds__(256) k_apply_L(SweepParams p, const double *__
                 for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], xn[side_node<DIM, P>(s ^ 1, l)], acc);
                 yl[side_node<DIM, P>(s, k)] = fma(dot, acc, yl[side_node<DIM, P>(s, k)]);
             }
+        } else if (q && dot < 0 && ((p.refl_mask >> s) & 1)) {
+            // the defect is that of the operator the sweep inverts, which couples a
+            // reflective inflow side to the reflected direction on the element's own
+            // nodes (AdvectionOperator::Mult leaves it out: y = L x stays as the reference's)
+            const double *xr = x + ((long long)g * p.Nomega + p.refl[a * NS + s]) * p.Ndof + (long long)e * N;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], xr[side_node<DIM, P>(s, l)], acc);
+                yl[side_node<DIM, P>(s, k)] = fma(dot, acc, yl[side_node<DIM, P>(s, k)]);
+            }
         }
     }
     // y = L x, or the defect y = q - L x when q is given

@@ -392,6 +392,33 @@ def test_fp32_refined_source_iteration_reaches_fp64_solution():
     assert abs(it - it_o) <= 2
 
 
+def test_fp32_refined_source_iteration_with_reflective_side():
+    """With a reflective side the defect q + D^T Ms phi - L psi couples the
+    reflected direction exactly as the sweep does, so the FP32-refined SI
+    reaches the FP64 oracle's SI solution (6^3, p=1, S4, x-low reflective)."""
+    print("fp32-refined SI, reflective side", flush=True)
+    nel, ne = (6, 6, 6), 216
+    rng = np.random.default_rng(5)
+    sigma = P.lognormal_sigma(ne, 1.0, rng).reshape(1, ne)
+    sigma_s = rng.uniform(0.5, 0.95, ne) * sigma[0]
+    Linv, pr = _pair(nel, 1, 4, sigma, bc=(1, 0, 0, 0, 0, 0))
+    q = pr.form_source(Q=1.0)
+    phi_o, _, it_o, _ = pr.source_iteration(sigma_s, q, abstol=1e-10, maxit=400)
+    phi, _, it, norms = A.source_iteration(Linv, sigma_s, q, abstol=1e-10, maxit=400, precision=A.FP32_REFINED)
+    err = rel_err(phi, phi_o)
+    print(f"  iterations {it} (oracle FP64 {it_o}), rel err {err:.2e}")
+    assert norms[-1] < 1e-10 and err < 1e-9
+
+
+def test_fp32_refined_source_iteration_rejects_fixup():
+    nel, ne = (4, 4, 4), 64
+    Linv, pr = _pair(nel, 1, 4, np.ones((1, ne)))
+    Linv.SetFixupOperator("zero")
+    q = pr.form_source(Q=1.0)
+    with pytest.raises(A.AlliumError, match="fixup"):
+        A.source_iteration(Linv, 0.5 * np.ones(ne), q, abstol=1e-8, maxit=50, precision=A.FP32_REFINED)
+
+
 def test_errors_are_reported_not_aborted():
     m = A.Mesh((4, 4), bc=(1, 1, 0, 0))
     with pytest.raises(A.AlliumError, match="DAG"):
