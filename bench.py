@@ -51,15 +51,38 @@ def cpu_baseline(n, seed, reps):
         pr.sweep(q, psi, threads=threads)
         times.append(time.perf_counter() - t)
     tm = float(np.median(times))
-    try:
-        model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        model = [l.split(":", 1)[1].strip() for l in model.splitlines() if l.startswith("Model name")][0]
-    except Exception:
-        model = "unknown"
+    host = host_cpu()
     return {"value": pr.size / tm, "unit": "angular unknowns/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64 level-synchronous OpenMP sweep of the same {n}^3 S8 p=1 workload, "
-                      f"median of {reps} sweeps after 1 warm-up (setup {setup:.1f}s excluded); host {model}",
+            "threads_used": threads, "nproc": host["nproc"], "physical_cores": host["physical_cores"],
+            "sample": f"oracle FP64 level-synchronous OpenMP sweep of the same {n}^3 S8 p=1 workload on {threads} "
+                      f"OpenMP threads, median of {reps} sweeps after 1 warm-up (setup {setup:.1f}s excluded); "
+                      f"host {host['model']}, nproc {host['nproc']}, {host['physical_cores']} physical cores "
+                      f"({host['sockets']} sockets x {host['cores_per_socket']} cores, "
+                      f"{host['threads_per_core']} threads per core)",
             "seconds_per_sweep": tm}
+
+
+def host_cpu():
+    """lscpu's model and core counts, and nproc (the CPUs this process may use)."""
+    info = {"model": "unknown", "sockets": None, "cores_per_socket": None, "threads_per_core": None,
+            "physical_cores": None, "nproc": None}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        kv = {l.split(":", 1)[0].strip(): l.split(":", 1)[1].strip() for l in txt.splitlines() if ":" in l}
+        info["model"] = kv.get("Model name", "unknown")
+        for key, name in (("Socket(s)", "sockets"), ("Core(s) per socket", "cores_per_socket"),
+                          ("Thread(s) per core", "threads_per_core")):
+            if kv.get(key, "").isdigit():
+                info[name] = int(kv[key])
+        if info["sockets"] and info["cores_per_socket"]:
+            info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    except Exception:
+        pass
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except Exception:
+        info["nproc"] = len(os.sched_getaffinity(0))
+    return info
 
 
 def measure_traffic(args, tuning):
@@ -119,7 +142,9 @@ def si_run(n, seed, maxit, learned=True, tuning=None, dsa_legs=True):
     sweeps, and falling back to DSA) and as FGMRES's preconditioner.  The
     source is HBM-resident before each timed solve.  The network is trained
     first on the GPU from seeded 32^3 media (not part of the timed solves; its
-    time is reported), as is the DSA setup."""
+    time is reported), as is the DSA setup.  The network is trained on 200
+    seeded 32^3 media: plain-iteration pairs, then two rounds with pairs from
+    iterations that use it."""
     import allium_amd as A
     from allium_amd import cnn
     from allium_amd import problems as P
@@ -182,7 +207,8 @@ def si_run(n, seed, maxit, learned=True, tuning=None, dsa_legs=True):
                    dsa_gpu_s=info["seconds"])
     if not learned:
         return out
-    net, info = cnn.fit(n=32, media=60, rounds=2, steps=1500, seed0=1000, sn=8, log=log)
+    compare("fp32_refined", precision=A.FP32_REFINED)
+    net, info = cnn.fit(n=32, media=200, rounds=2, steps=1500, seed0=1000, sn=8, log=log)
     out["training"] = info
     for name, prec in (("learned_fp64", A.FP64), ("learned_fp32_refined", A.FP32_REFINED)):
         compare(name, accel=net, precision=prec)
@@ -278,19 +304,28 @@ def config3_run(seed=0, reps=3, instances=12):
     unknowns = sum(op.size for op in ops)
     out = {"meshes": ns, "unknowns_per_batch_sweep": unknowns, "setup_s": setup,
            "levels_max": max(op.nlevels for op in ops), "levels_sum": sum(op.nlevels for op in ops)}
-    for name, prec in (("fp64", A.FP64), ("fp32_lu", A.FP32_REFINED)):
+    psis32 = [A.DeviceVector(op.size) for op in ops]
+    for name, prec, dst in (("fp64", A.FP64, psis), ("fp32_lu", A.FP32_REFINED, psis32)):
         for op in ops:
             op.set_precision(prec)
-        batch.Mult(None, psis, isos)
+        batch.Mult(None, dst, isos)
         t_before, n_before = batch.timing()
         for _ in range(reps):
-            batch.Mult(None, psis, isos)
+            batch.Mult(None, dst, isos)
         t_after, n_after = batch.timing()
         dt = (t_after - t_before) / (n_after - n_before)
         out[name] = {"ms_per_batch_sweep": dt * 1e3, "unknowns_per_s": unknowns / dt}
-    for d in psis[:1]:
-        if not np.isfinite(d.to_host()).all():
+    for op in ops:
+        op.set_precision(A.FP64)
+    # FP32 LU against the FP64 sweep, every instance
+    worst = 0.0
+    for d64, d32 in zip(psis, psis32):
+        a, b = d64.to_host(), d32.to_host()
+        if not (np.isfinite(a).all() and np.isfinite(b).all()):
             out["error"] = "non-finite angular flux"
+        worst = max(worst, float(np.abs(b - a).max() / np.abs(a).max()))
+        del a, b
+    out["fp32_lu"]["rel_err_vs_fp64"] = worst
     return out
 
 
@@ -312,6 +347,7 @@ def main():
     ap.add_argument("--no-dsa", action="store_true", help="skip the DSA legs of the 64^3 solves")
     ap.add_argument("--no-trt", action="store_true", help="skip the 20-step multigroup TRT run")
     ap.add_argument("--no-config3", action="store_true", help="skip the batched p=2 jittered instances")
+    ap.add_argument("--no-fp32-check", action="store_true", help="skip the FP32 element-solve sweep of the workload")
     ap.add_argument("--si-maxit", type=int, default=3000)
     ap.add_argument("--tuning", default=None, help="mode,block,lanes (skips the autotuner)")
     ap.add_argument("--box", default=None, help="bx,by,bz,dirs_per_group,threads,prefetch of box sweeps (modes 4/5)")
@@ -379,6 +415,22 @@ def main():
     if not np.isfinite(psi_h).all():
         raise SystemExit("non-finite angular flux")
     log(f"sweep {t_sweep * 1e3:.3f} ms -> {value:.3e} unknowns/s, {achieved:.0f} GB/s algorithmic")
+    fp32 = None
+    if args.precision == "fp64" and not args.no_fp32_check:
+        # the same sweep with FP32 element solves: its error against the FP64 sweep, and its time
+        Linv.set_precision(A.FP32_REFINED)
+        Linv.Mult(dq, dpsi)
+        A.synchronize()
+        A.check(lib.allium_event_timer(0, None))
+        for _ in range(3):
+            Linv.Mult(dq, dpsi)
+        A.check(lib.allium_event_timer(1, ctypes.byref(ms)))
+        psi32 = dpsi.to_host()
+        fp32 = {"rel_err_vs_fp64": float(np.abs(psi32 - psi_h).max() / np.abs(psi_h).max()),
+                "ms_per_sweep": ms.value / 3}
+        del psi32
+        Linv.set_precision(A.FP64)
+        log(f"FP32 element solves: {fp32}")
 
     out = {
         "metric": "angular unknowns swept/s",
@@ -404,6 +456,8 @@ def main():
                      "algorithmic_bytes_per_sweep": BYTES_PER_UNKNOWN_3D_P1 * Linv.size,
                      "bytes_per_unknown": BYTES_PER_UNKNOWN_3D_P1},
     }
+    if fp32 is not None:
+        out["fp32_element_solves"] = fp32
     if not args.no_si:
         try:
             out["source_iteration"] = si_run(args.n, args.seed, args.si_maxit, learned=not args.no_learned,
