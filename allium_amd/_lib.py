@@ -87,6 +87,7 @@ SIGNATURES = {
     "allium_apply_L": (ctypes.c_int, [c_vp, c_dp, c_dp]),
     "allium_balance": (ctypes.c_int, [c_vp, c_dp, c_dp]),
     "allium_balance_device": (ctypes.c_int, [c_vp, c_vp, c_dp, c_vp]),
+    "allium_balance_device_noalloc": (ctypes.c_int, [c_vp, c_vp, c_dp, c_vp]),
     "allium_form_source": (ctypes.c_int, [c_vp, c_dp, c_dp, c_dp]),
     "allium_moments_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "allium_moments_transpose_device": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),

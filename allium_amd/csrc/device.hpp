@@ -196,6 +196,14 @@ hipError_t launch_defect_L(const SweepParams &prm, const double *q, const double
 int balance_blocks(const SweepParams &prm);
 hipError_t launch_balance(const SweepParams &prm, const double *psi, double *partial,
                           hipStream_t stream);
+// the balance without allocation: partials and result in the file-scope device
+// workspace of the calling GPU (balance_blocks(prm) <= kStaticBalanceBlocks,
+// else hipErrorInvalidValue); synchronizes the stream.  Calls that overlap on
+// one GPU share the workspace and race.
+constexpr int kStaticBalanceBlocks = 32768;   // Ne * G <= 8,388,608
+constexpr int kMaxDevices = 64;
+hipError_t balance_static(const SweepParams &prm, const double *psi, double *absorption, double *leakage,
+                          hipStream_t stream);
 // phi = sum_a w_a psi ; psi = phi / wsum ; y = scale * coef_e * Mm(e) x_e
 hipError_t launch_moments(const SweepParams &prm, const double *psi, double *phi, hipStream_t s);
 // [phi, J] ((dim + 1) G Ndof, moment slowest) and its transpose

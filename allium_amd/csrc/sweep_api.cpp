@@ -930,6 +930,17 @@ int allium_balance_device(allium_sweep_t h, const double *d_psi, double *out, vo
     return ALLIUM_OK;
 }
 
+int allium_balance_device_noalloc(allium_sweep_t h, const double *d_psi, double *out, void *stream)
+{
+    if (!h || !d_psi || !out) return fail(ALLIUM_ERR_ARG, "null argument");
+    if (allium::balance_blocks(h->ds.prm) > allium::kStaticBalanceBlocks)
+        return fail(ALLIUM_ERR_ARG, "Ne * G exceeds the static balance workspace (at most 8388608 element-groups)");
+    double ab = 0.0, lk = 0.0;
+    HIP_TRY(allium::balance_static(h->ds.prm, d_psi, &ab, &lk, (hipStream_t)stream));
+    *out = lk + ab;
+    return ALLIUM_OK;
+}
+
 int allium_balance(allium_sweep_t h, const double *psi, double *out)
 {
     if (!h || !psi || !out) return fail(ALLIUM_ERR_ARG, "null argument");

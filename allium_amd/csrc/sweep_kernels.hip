@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 
 namespace allium {
 
@@ -1172,6 +1173,83 @@ hipError_t launch_balance(const SweepParams &prm, const double *psi, double *par
     }
 #undef BAL
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The balance with a static workspace: the block partials and the result live
+// in `scratch`, a file-scope device array (one copy per GPU, held until the
+// process ends), so the call allocates nothing.  Its device address comes
+// from hipGetSymbolAddress once per device; the array's name is a host-side
+// stand-in and is never passed to a kernel or a copy.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[2 * kStaticBalanceBlocks + 2];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// partial[0 .. 2 nb) -> out[0] = sum of absorptions, out[1] = sum of leakages,
+// one wave: lane l sums blocks l, l + 64, ... in order, then a fixed tree
+__global__ void __launch_bounds__(64) k_balance_finish(const double *__restrict__ partial, int nb,
+                                                       double *__restrict__ out)
+{
+    __shared__ double sa[64], sl[64];
+    double a = 0.0, l = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 64) {
+        a += partial[2 * i];
+        l += partial[2 * i + 1];
+    }
+    sa[threadIdx.x] = a;
+    sl[threadIdx.x] = l;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) { sa[threadIdx.x] += sa[threadIdx.x + w]; sl[threadIdx.x] += sl[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = sa[0]; out[1] = sl[0]; }
+}
+
+hipError_t static_scratch(double **ptr)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t balance_static(const SweepParams &prm, const double *psi, double *absorption, double *leakage,
+                          hipStream_t s)
+{
+    const int nb = balance_blocks(prm);
+    if (nb > kStaticBalanceBlocks) return hipErrorInvalidValue;
+    double *ws = nullptr;
+    hipError_t e = static_scratch(&ws);
+    if (e != hipSuccess) return e;
+    // every partial the finish kernel reads is written by launch_balance, and
+    // the result slots by the finish kernel: nothing needs clearing
+    e = launch_balance(prm, psi, ws, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_balance_finish, dim3(1), dim3(64), 0, s, ws, nb, ws + 2 * kStaticBalanceBlocks);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    double res[2] = {0.0, 0.0};
+    e = hipMemcpyAsync(res, ws + 2 * kStaticBalanceBlocks, sizeof(res), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    *absorption = res[0];
+    *leakage = res[1];
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------
