@@ -575,11 +575,12 @@ constexpr int kSwapYZ = 0x1B;   // quad_perm [3,2,1,0]: partner along y and z
 // with A_x = alpha + beta_x J_x, B = A_x^2 + beta_y^2 + beta_z^2,
 // c = 2 beta_y beta_z and K = J_y J_z (commuting, J^2 = -I, K^2 = I), so B and
 // B^2 - c^2 act inside the lane's x pair; psi = W y.
+// The right-hand side of the lane's x pair, v = M^{-1} b + sum_d t_d (m^{-1} e_in) (x) u_d,
+// from (v0, v1) = b / vol there.
 template <int DIM, typename T, int OCT>
-__device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig, T ux, const T (&uy)[2],
-                                            const T (&uz)[2], bool hy, bool hz)
+__device__ __forceinline__ void fd_split_rhs(T &v0, T &v1, const T (&t)[3], T ux, const T (&uy)[2], const T (&uz)[2],
+                                             bool hy, bool hz)
 {
-    constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
     constexpr bool nx = OCT & 1, ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
     // M^{-1}: m^{-1} = [[4,-2],[-2,4]] on every axis
     {
@@ -610,6 +611,14 @@ __device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig
         v0 += wz * uz[0];
         v1 += wz * uz[1];
     }
+}
+
+// (sigma I + sum_d t_d B_d) psi = v for the lane's x pair: W^{-1}, S^{-1}, W (above)
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void fd_split_solve(T &v0, T &v1, const T (&t)[3], T sig, bool hy, bool hz)
+{
+    constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
+    constexpr bool nx = OCT & 1, ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
     // W^{-1}: W+^{-1} = [[1,0],[1/r2,1/r2]], W-^{-1} = [[0,1],[1/r2,1/r2]]
     {
         const T w0 = v0, w1 = v1;
@@ -677,15 +686,48 @@ __device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig
     if constexpr (DIM == 3) wfwd([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
 }
 
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig, T ux, const T (&uy)[2],
+                                            const T (&uz)[2], bool hy, bool hz)
+{
+    fd_split_rhs<DIM, T, OCT>(v0, v1, t, ux, uy, uz, hy, hz);
+    fd_split_solve<DIM, T, OCT>(v0, v1, t, sig, hy, hz);
+}
+
+// (sigma I + sum_d t_d B_d) x for the lane's x pair (x0, x1), with the 1D upwind
+// operators B+ = m^{-1} C+ = [[3, 1], [-3, 1]] and B- = [[1, -3], [1, 3]]: on
+// the upwind node 3 own + other, on the downwind node own - 3 other
+template <int DIM, int OCT>
+__device__ __forceinline__ void fd_split_apply(double x0, double x1, const double (&t)[3], double sig, bool hy,
+                                               bool hz, double &y0, double &y1)
+{
+    constexpr bool nx = OCT & 1, ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
+    y0 = sig * x0 + t[0] * (nx ? x0 - 3.0 * x1 : 3.0 * x0 + x1);
+    y1 = sig * x1 + t[0] * (nx ? 3.0 * x1 + x0 : x1 - 3.0 * x0);
+    {
+        const double p0 = quad_swap<double, kSwapY>(x0), p1 = quad_swap<double, kSwapY>(x1);
+        const bool up = hy == ny;
+        y0 = fma(t[1], up ? fma(3.0, x0, p0) : fma(-3.0, p0, x0), y0);
+        y1 = fma(t[1], up ? fma(3.0, x1, p1) : fma(-3.0, p1, x1), y1);
+    }
+    if constexpr (DIM == 3) {
+        const double p0 = quad_swap<double, kSwapZ>(x0), p1 = quad_swap<double, kSwapZ>(x1);
+        const bool up = hz == nz;
+        y0 = fma(t[2], up ? fma(3.0, x0, p0) : fma(-3.0, p0, x0), y0);
+        y1 = fma(t[2], up ? fma(3.0, x1, p1) : fma(-3.0, p1, x1), y1);
+    }
+}
+
 // one lane's inputs of one cell, loaded ahead of its solve
 struct SplitIn {
     double q0, q1, i0, i1, sig;
+    double o0, o1;   // refinement: the cell's previous psi
     long long off;
     int e, d, ci;   // element, direction slot, box cell in level order
     bool ok;
 };
 
-template <int DIM, int OCT>
+template <int DIM, int OCT, bool REF = false>
 __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int l, int it,
                                             int r, SplitIn &in)
 {
@@ -724,9 +766,15 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
     in.i0 = vi.x;
     in.i1 = vi.y;
     in.sig = sg;
+    if constexpr (REF) {
+        // the cell's psi before this sweep overwrites it (no cell of a later level is written yet)
+        const d2v vo = *reinterpret_cast<const d2v *>(p.psi + off);
+        in.o0 = vo.x;
+        in.o1 = vo.y;
+    }
 }
 
-template <int DIM, typename T, int OCT>
+template <int DIM, typename T, int OCT, bool REF = false>
 __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int r,
                                             const SplitIn &in)
 {
@@ -752,19 +800,43 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
     }
     // x face: node (., r); y face: (i0, i_z) -> i0 + 2 i_z; z face: (i0, i_y) -> i0 + 2 i_y
     const int my0 = DIM == 3 ? 2 * bz : 0;
-    const T ux = (T)slot[0][r];
-    const T uy[2] = {(T)slot[1][my0], (T)slot[1][my0 + 1]};
-    T uz[2] = {(T)0, (T)0};
-    if constexpr (DIM == 3) {
-        uz[0] = (T)slot[2][2 * by];
-        uz[1] = (T)slot[2][2 * by + 1];
-    }
-    const T t[3] = {(T)(L.om[3 * d] * inv_vol), (T)(L.om[3 * d + 1] * inv_vol), (T)(L.om[3 * d + 2] * inv_vol)};
-    T v0 = (T)((in.q0 + in.i0) * inv_vol), v1 = (T)((in.q1 + in.i1) * inv_vol);
-    fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by != 0, bz != 0);
     d2v o;
-    o.x = (double)v0;
-    o.y = (double)v1;
+    if constexpr (REF) {
+        // defect correction: the right-hand side with the new upwind traces minus the
+        // operator on the cell's previous psi, in FP64; the correction solved in T;
+        // psi = previous + correction
+        const double ux = slot[0][r];
+        const double uy[2] = {slot[1][my0], slot[1][my0 + 1]};
+        double uz[2] = {0.0, 0.0};
+        if constexpr (DIM == 3) {
+            uz[0] = slot[2][2 * by];
+            uz[1] = slot[2][2 * by + 1];
+        }
+        const double t[3] = {L.om[3 * d] * inv_vol, L.om[3 * d + 1] * inv_vol, L.om[3 * d + 2] * inv_vol};
+        const double sg = in.sig + (p.timedep ? p.tau : 0.0);
+        double v0 = (in.q0 + in.i0) * inv_vol, v1 = (in.q1 + in.i1) * inv_vol;
+        fd_split_rhs<DIM, double, OCT>(v0, v1, t, ux, uy, uz, by != 0, bz != 0);
+        double a0, a1;
+        fd_split_apply<DIM, OCT>(in.o0, in.o1, t, sg, by != 0, bz != 0, a0, a1);
+        T r0 = (T)(v0 - a0), r1 = (T)(v1 - a1);
+        const T tt[3] = {(T)t[0], (T)t[1], (T)t[2]};
+        fd_split_solve<DIM, T, OCT>(r0, r1, tt, (T)sg, by != 0, bz != 0);
+        o.x = in.o0 + (double)r0;
+        o.y = in.o1 + (double)r1;
+    } else {
+        const T ux = (T)slot[0][r];
+        const T uy[2] = {(T)slot[1][my0], (T)slot[1][my0 + 1]};
+        T uz[2] = {(T)0, (T)0};
+        if constexpr (DIM == 3) {
+            uz[0] = (T)slot[2][2 * by];
+            uz[1] = (T)slot[2][2 * by + 1];
+        }
+        const T t[3] = {(T)(L.om[3 * d] * inv_vol), (T)(L.om[3 * d + 1] * inv_vol), (T)(L.om[3 * d + 2] * inv_vol)};
+        T v0 = (T)((in.q0 + in.i0) * inv_vol), v1 = (T)((in.q1 + in.i1) * inv_vol);
+        fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by != 0, bz != 0);
+        o.x = (double)v0;
+        o.y = (double)v1;
+    }
     *reinterpret_cast<d2v *>(p.psi + in.off) = o;
     // outflow traces
     slot[0][r] = (OCT & 1) ? o.x : o.y;
@@ -784,7 +856,7 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
 // pass of every level is prefetched during the level before (two input
 // buffers with alternating roles, so no register copy waits for loads in
 // flight; the LDS-only barrier lets them cross it), at the price of registers.
-template <int DIM, typename T, int OCT, int PF>
+template <int DIM, typename T, int OCT, int PF, bool REF>
 __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
 {
     constexpr int LPC = 1 << (DIM - 1);
@@ -796,8 +868,8 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
             const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
             for (int it = qd; it < items; it += nq) {   // uniform over the lanes of a cell
                 SplitIn in;
-                split_fetch<DIM, OCT>(p, bp, cx, l, it, r, in);
-                if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in);
+                split_fetch<DIM, OCT, REF>(p, bp, cx, l, it, r, in);
+                if (in.ok) split_solve<DIM, T, OCT, REF>(p, bp, cx, r, in);
             }
             lds_barrier();
         }
@@ -808,26 +880,26 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     constexpr int AHEAD = PF == 3 ? 2 : 1;
     auto level = [&](int l, SplitIn &cur, SplitIn &nxt) {
         const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
-        split_fetch<DIM, OCT>(p, bp, cx, l + AHEAD, qd, r, nxt);
-        if (cur.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, cur);
+        split_fetch<DIM, OCT, REF>(p, bp, cx, l + AHEAD, qd, r, nxt);
+        if (cur.ok) split_solve<DIM, T, OCT, REF>(p, bp, cx, r, cur);
         for (int it = qd + nq; it < items; it += nq) {   // uniform over the lanes of a cell
             SplitIn in;
-            split_fetch<DIM, OCT>(p, bp, cx, l, it, r, in);
-            if (in.ok) split_solve<DIM, T, OCT>(p, bp, cx, r, in);
+            split_fetch<DIM, OCT, REF>(p, bp, cx, l, it, r, in);
+            if (in.ok) split_solve<DIM, T, OCT, REF>(p, bp, cx, r, in);
         }
         lds_barrier();
     };
     if constexpr (AHEAD == 1) {
         SplitIn b0, b1;
-        split_fetch<DIM, OCT>(p, bp, cx, 0, qd, r, b0);
+        split_fetch<DIM, OCT, REF>(p, bp, cx, 0, qd, r, b0);
         for (int l = 0; l < bp.nl; l += 2) {
             level(l, b0, b1);
             if (l + 1 < bp.nl) level(l + 1, b1, b0);
         }
     } else {
         SplitIn b0, b1, b2;
-        split_fetch<DIM, OCT>(p, bp, cx, 0, qd, r, b0);
-        split_fetch<DIM, OCT>(p, bp, cx, 1, qd, r, b1);
+        split_fetch<DIM, OCT, REF>(p, bp, cx, 0, qd, r, b0);
+        split_fetch<DIM, OCT, REF>(p, bp, cx, 1, qd, r, b1);
         for (int l = 0; l < bp.nl; l += 3) {
             level(l, b0, b2);
             if (l + 1 < bp.nl) level(l + 1, b1, b0);
@@ -836,24 +908,24 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     }
 }
 
-template <int DIM, typename T, int PF>
+template <int DIM, typename T, int PF, bool REF>
 __global__ void __launch_bounds__(1024) k_sweep_box_split(SweepParams p, BoxPlan bp, int box_base)
 {
     extern __shared__ double lds[];
     const int oct = blockIdx.x % bp.noct;   // workgroup-uniform
     switch (oct) {
-    case 0: box_task_split<DIM, T, 0, PF>(p, bp, box_base, lds); break;
-    case 1: box_task_split<DIM, T, 1, PF>(p, bp, box_base, lds); break;
-    case 2: box_task_split<DIM, T, 2, PF>(p, bp, box_base, lds); break;
-    case 3: box_task_split<DIM, T, 3, PF>(p, bp, box_base, lds); break;
-    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4, PF>(p, bp, box_base, lds); break;
-    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5, PF>(p, bp, box_base, lds); break;
-    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6, PF>(p, bp, box_base, lds); break;
-    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7, PF>(p, bp, box_base, lds); break;
+    case 0: box_task_split<DIM, T, 0, PF, REF>(p, bp, box_base, lds); break;
+    case 1: box_task_split<DIM, T, 1, PF, REF>(p, bp, box_base, lds); break;
+    case 2: box_task_split<DIM, T, 2, PF, REF>(p, bp, box_base, lds); break;
+    case 3: box_task_split<DIM, T, 3, PF, REF>(p, bp, box_base, lds); break;
+    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4, PF, REF>(p, bp, box_base, lds); break;
+    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5, PF, REF>(p, bp, box_base, lds); break;
+    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6, PF, REF>(p, bp, box_base, lds); break;
+    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7, PF, REF>(p, bp, box_base, lds); break;
     }
 }
 
-template <int DIM, typename T, int PF>
+template <int DIM, typename T, int PF, bool REF = false>
 static hipError_t run_boxes_split(const DeviceSweep &ds, const SweepParams &prm, hipStream_t s)
 {
     const BoxPlan &bp = ds.box;
@@ -861,7 +933,7 @@ static hipError_t run_boxes_split(const DeviceSweep &ds, const SweepParams &prm,
     for (int k = 0; k < nlev; ++k) {
         const int nbox = ds.box_ptr[k + 1] - ds.box_ptr[k];
         if (nbox == 0) continue;
-        hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
+        hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF, REF>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
                            dim3(ds.box_threads), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
     }
     return hipGetLastError();
@@ -911,6 +983,12 @@ static hipError_t run_boxes_p(const DeviceSweep &ds, const SweepParams &prm, hip
     if constexpr (P == 1) {
         // lane-split cells; the workgroup must hold whole cells
         if (ds.box_threads % (1 << (DIM - 1))) return hipErrorInvalidValue;
+        if (ds.refine) {
+            // in-place defect correction with FP32 cell solves (psi holds the previous iterate)
+            if (ds.box_prefetch == 3) return run_boxes_split<DIM, float, 3, true>(ds, prm, s);
+            if (ds.box_prefetch == 2) return run_boxes_split<DIM, float, 2, true>(ds, prm, s);
+            return run_boxes_split<DIM, float, 0, true>(ds, prm, s);
+        }
         if (ds.precision == 1) {
             if (ds.box_prefetch == 3) return run_boxes_split<DIM, float, 3>(ds, prm, s);
             if (ds.box_prefetch == 2) return run_boxes_split<DIM, float, 2>(ds, prm, s);

@@ -1121,7 +1121,45 @@ static int source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
         }
         return ALLIUM_OK;
     };
-    if (opt->precision == ALLIUM_FP32_REFINED) {
+    // FP32-refined iteration inside the box sweep (uniform p = 1, box launches): the
+    // scattered part psi_s = L^{-1} D^T Ms phi is corrected in place each iteration,
+    // psi_s += L32^{-1}(D^T Ms phi - L psi_s), the defect formed by the sweep itself in
+    // FP64 from the new upwind traces and the cell's previous psi_s; phi = D L^{-1} q + D psi_s
+    const bool fused_refine = opt->precision == ALLIUM_FP32_REFINED && !h->ds.fix_on && h->mesh.uniform &&
+                              h->ed.p == 1 && (h->ds.launch_mode == 4 || h->ds.launch_mode == 5) && h->ds.box_buf;
+    if (fused_refine) {
+        h->ds.precision = 0;
+        rc = do_sweep(h, dq, nullptr, h->a.d(), s, h->b.d());
+        if (rc) { h->ds.precision = prec_save; return rc; }
+        HIP_TRY(h->c.ensure(sizeof(double) * nps));
+        HIP_TRY(hipMemsetAsync(h->c.p, 0, sizeof(double) * nps, s));
+        HIP_TRY(hipMemsetAsync(h->x.p, 0, sizeof(double) * np, s));
+        for (it = 1;; it++) {
+            HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
+            h->ds.refine = 1;
+            rc = do_sweep(h, nullptr, h->iso.d(), h->c.d(), s, h->y.d());
+            h->ds.refine = 0;
+            if (rc) break;
+            HIP_TRY(allium::launch_si_update(np, h->b.d(), h->y.d(), h->x.d(), h->part.d(), s,
+                                             want_r ? h->r.d() : nullptr));
+            double nrm = 0.0;
+            rc = residual_norm(nrm);
+            if (rc) break;
+            if (norms) norms[it - 1] = nrm;
+            if (!std::isfinite(nrm)) { rc = fail(ALLIUM_ERR_NUMERIC, "non-finite residual"); break; }
+            if (it == 1) r0 = std::max(nrm * opt->reltol, opt->abstol);
+            if (nrm < r0 || it >= opt->maxit) break;
+            rc = after_update(nrm);
+            if (rc) break;
+        }
+        if (rc) { h->ds.precision = prec_save; return rc; }
+        // final FP64 sweep: psi = L^{-1}(D^T Ms phi + q)
+        HIP_TRY(allium::launch_mass(h->ds.prm, h->ds.prm.Mt, h->coef.d(), inv_w, h->x.d(), h->iso.d(), s));
+        rc = do_sweep(h, dq, h->iso.d(), h->a.d(), s);
+        h->ds.precision = prec_save;
+        if (rc) return rc;
+        if (psi) HIP_TRY(hipMemcpyAsync(psi, h->a.p, sizeof(double) * nps, out_kind, s));
+    } else if (opt->precision == ALLIUM_FP32_REFINED) {
         // defect correction: psi += L32^{-1} (q + D^T Ms phi - L psi), residuals in FP64.
         // A fixup would clip the correction, which may legitimately be negative.
         if (h->ds.fix_on)

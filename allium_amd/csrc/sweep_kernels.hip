@@ -898,8 +898,9 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
     } else if (ds.launch_mode == 1 || ds.launch_mode == 5) {
         // hipGraph of the level (1) or box-level (5) launches, re-captured when
         // the vectors change
+        const int variant = ds.precision + 2 * ds.refine;
         if (!ds.graph_exec || ds.graph_src != src || ds.graph_iso != iso || ds.graph_psi != psi ||
-            ds.graph_prec != ds.precision) {
+            ds.graph_prec != variant) {
             if (ds.graph_exec) { hipGraphExecDestroy(ds.graph_exec); ds.graph_exec = nullptr; }
             if (!ds.graph_stream) hipStreamCreateWithFlags(&ds.graph_stream, hipStreamNonBlocking);
             hipGraph_t graph;
@@ -916,7 +917,7 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
             err = hipGraphInstantiate(&ds.graph_exec, graph, nullptr, nullptr, 0);
             hipGraphDestroy(graph);
             if (err != hipSuccess) return err;
-            ds.graph_src = src; ds.graph_iso = iso; ds.graph_psi = psi; ds.graph_prec = ds.precision;
+            ds.graph_src = src; ds.graph_iso = iso; ds.graph_psi = psi; ds.graph_prec = variant;
         }
         err = hipGraphLaunch(ds.graph_exec, stream);
     } else if (ds.launch_mode == 4) {
