@@ -47,14 +47,16 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
     static_assert(N <= 32, "one row per lane of a 32-lane group");
     __shared__ T piv[8][N + 1];
     const int slot = threadIdx.x >> 5, r = threadIdx.x & 31;
-    const long long item = (long long)blockIdx.x * 8 + slot;
+    // each instance's items start on a multiple of 8, so the instance (and with it
+    // every parameter read through p) is uniform over the workgroup
+    const long long wg0 = (long long)blockIdx.x * 8;
     int inst = 0;
-    while (inst + 1 < cl.ninst && item >= cl.start[inst + 1]) ++inst;
-    const bool valid = item < cl.start[cl.ninst];
+    while (inst + 1 < cl.ninst && wg0 >= cl.start[inst + 1]) ++inst;
     const SweepParams &p = prms ? prms[inst] : p0;
+    const int local = (int)(wg0 - cl.start[inst]) + slot;
+    const bool valid = local < cl.ncells[inst] * p.G;
     int e = 0, a = 0, g = 0;
     if (valid) {
-        const int local = (int)(item - cl.start[inst]);
         const int nc = cl.ncells[inst];
         g = local / nc;
         const uint32_t cell = p.cells[cl.cbeg[inst] + (local - g * nc)];
@@ -167,7 +169,7 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
         }
         cell_sync();
         if (r > k && r < N) {
-            const T l = A[k] / piv[slot][k];
+            const T l = A[k] * fast_rcp(piv[slot][k]);
 #pragma unroll
             for (int j = k + 1; j < N; ++j) A[j] = fma(-l, piv[slot][j], A[j]);
             bb = fma(-l, piv[slot][N], bb);
@@ -179,7 +181,7 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
 #pragma unroll
     for (int k = N - 1; k >= 0; --k) {
         if (r == k) {
-            x = bb / A[k];
+            x = bb * fast_rcp(A[k]);
             piv[slot][0] = x;
         }
         cell_sync();

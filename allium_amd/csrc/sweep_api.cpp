@@ -1290,7 +1290,7 @@ int allium_sweep_batch_create(const allium_sweep_t *handles, int count, allium_s
             const bool has = L < ds.nlevels;
             cl.cbeg[i] = has ? ds.level_ptr[L] : 0;
             cl.ncells[i] = has ? (int)(ds.level_ptr[L + 1] - ds.level_ptr[L]) : 0;
-            cl.start[i + 1] = cl.start[i] + cl.ncells[i] * b->h[i]->G;
+            cl.start[i + 1] = cl.start[i] + (cl.ncells[i] * b->h[i]->G + 7) / 8 * 8;   // workgroup-aligned
         }
         b->levels.push_back(cl);
     }
