@@ -428,24 +428,35 @@ __device__ __forceinline__ BoxCtx box_stage(const SweepParams &p, const BoxPlan 
         for (int ax = 0; ax < 3; ++ax)
             omw[3 * i + ax] = (a >= 0 && ax < DIM) ? fabs(p.omega[4 * a + ax]) * bp.farea[ax] : 0.0;
     }
-    for (int i = tid; i < vol; i += nt) lcw[i] = bp.lcells[i];
+    if constexpr (P != 1)
+        for (int i = tid; i < vol; i += nt) lcw[i] = bp.lcells[i];
     for (int i = tid; i <= bp.nl; i += nt) lpw[i] = bp.lptr[i];
     if constexpr (P == 1) {
         for (int i = tid; i < bp.nd; i += nt) {
             const int a = i < ndir ? gdirs[i] : -1;
             dbw[i] = a >= 0 ? ((long long)g * p.Nomega + a) * p.Ndof : -1;
         }
+        // per box cell: its element, and its slot in each axis's trace plane
+        // (the other axes' local coordinates), packed 10 bits per axis, so the
+        // cell loop computes no plane index
         for (int i = tid; i < vol; i += nt) {
             const uint32_t cw = bp.lcells[i];
             const int lc[3] = {(int)(cw & 1023u), (int)((cw >> 10) & 1023u), (int)(cw >> 20)};
             int c[3] = {0, 0, 0};
             bool in_box = true;
+            uint32_t sw = 0;
 #pragma unroll
             for (int ax = 0; ax < DIM; ++ax) {
                 in_box = in_box && lc[ax] < ext[ax];
                 c[ax] = ((OCT >> ax) & 1) ? lo[ax] + ext[ax] - 1 - lc[ax] : lo[ax] + lc[ax];
+                int sidx = 0, mul = 1;
+#pragma unroll
+                for (int o = 0; o < DIM; ++o)
+                    if (o != ax) { sidx += mul * lc[o]; mul *= bp.bs[o]; }
+                sw |= (uint32_t)sidx << (10 * ax);
             }
             cew[i] = in_box ? c[0] + p.nel[0] * (c[1] + p.nel[1] * c[2]) : -1;
+            lcw[i] = sw;
         }
     }
     // ---- inflow planes from the upwind neighbours (zero on the boundary) ---
@@ -727,7 +738,7 @@ struct SplitIn {
     bool ok;
 };
 
-template <int DIM, int OCT, bool REF = false>
+template <int DIM, int OCT, bool REF = false, bool ONE = false>
 __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int l, int it,
                                             int r, SplitIn &in)
 {
@@ -741,9 +752,10 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
     const bool lv = l < bp.nl;
     const int lc = lv ? l : 0;
     const int c0 = L.lpl[lc], cnt = L.lpl[lc + 1] - c0;
-    const bool inr = lv && it < cnt * cx.ndir;
+    // ONE: one direction per workgroup, so the item is the cell (no division)
+    const bool inr = lv && it < (ONE ? cnt : cnt * cx.ndir);
     const int itc = inr ? it : 0;
-    const int d = fast_div(itc, cnt, 1.0f / (float)cnt), ci = c0 + (itc - d * cnt);
+    const int d = ONE ? 0 : fast_div(itc, cnt, 1.0f / (float)cnt), ci = c0 + (itc - d * cnt);
     const int e = L.cell_e[ci];
     const long long base = L.dbase[d];
     const bool ok = inr && e >= 0 && base >= 0;
@@ -774,7 +786,7 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
     }
 }
 
-template <int DIM, typename T, int OCT, bool REF = false>
+template <int DIM, typename T, int OCT, bool REF = false, bool ONE = false>
 __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int r,
                                             const SplitIn &in)
 {
@@ -782,19 +794,15 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
     const BoxLds &L = cx.L;
     const int by = r & 1, bz = (r >> 1) & 1, d = in.d;
     const double inv_vol = 1.0 / bp.vol;
-    // trace slots of this cell: the other axes' local coordinates
+    // trace slots of this cell (precomputed per box cell by box_stage)
     double *slot[DIM];
     {
-        const uint32_t cw = L.lcl[in.ci];
-        const int lc[3] = {(int)(cw & 1023u), (int)((cw >> 10) & 1023u), (int)(cw >> 20)};
+        const uint32_t sw = L.lcl[in.ci];
         double *pl = L.planes;
 #pragma unroll
         for (int ax = 0; ax < DIM; ++ax) {
-            int sidx = 0, mul = 1;
-#pragma unroll
-            for (int o = 0; o < DIM; ++o)
-                if (o != ax) { sidx += mul * lc[o]; mul *= bp.bs[o]; }
-            slot[ax] = pl + (d * bp.psz[ax] + sidx) * PS;
+            const int sidx = (int)((sw >> (10 * ax)) & 1023u);
+            slot[ax] = pl + (ONE ? sidx : d * bp.psz[ax] + sidx) * PS;
             pl += bp.nd * bp.psz[ax] * PS;
         }
     }
@@ -856,7 +864,7 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
 // pass of every level is prefetched during the level before (two input
 // buffers with alternating roles, so no register copy waits for loads in
 // flight; the LDS-only barrier lets them cross it), at the price of registers.
-template <int DIM, typename T, int OCT, int PF, bool REF>
+template <int DIM, typename T, int OCT, int PF, bool REF, bool ONE>
 __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
 {
     constexpr int LPC = 1 << (DIM - 1);
@@ -865,11 +873,11 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     const int r = tid & (LPC - 1), qd = tid / LPC, nq = nt / LPC;
     if constexpr (PF == 0) {
         for (int l = 0; l < bp.nl; ++l) {
-            const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
+            const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * (ONE ? 1 : cx.ndir);
             for (int it = qd; it < items; it += nq) {   // uniform over the lanes of a cell
                 SplitIn in;
-                split_fetch<DIM, OCT, REF>(p, bp, cx, l, it, r, in);
-                if (in.ok) split_solve<DIM, T, OCT, REF>(p, bp, cx, r, in);
+                split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l, it, r, in);
+                if (in.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, in);
             }
             lds_barrier();
         }
@@ -879,27 +887,27 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     // while level l solves (PF 2: one level ahead, PF 3: two)
     constexpr int AHEAD = PF == 3 ? 2 : 1;
     auto level = [&](int l, SplitIn &cur, SplitIn &nxt) {
-        const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * cx.ndir;
-        split_fetch<DIM, OCT, REF>(p, bp, cx, l + AHEAD, qd, r, nxt);
-        if (cur.ok) split_solve<DIM, T, OCT, REF>(p, bp, cx, r, cur);
+        const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * (ONE ? 1 : cx.ndir);
+        split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l + AHEAD, qd, r, nxt);
+        if (cur.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, cur);
         for (int it = qd + nq; it < items; it += nq) {   // uniform over the lanes of a cell
             SplitIn in;
-            split_fetch<DIM, OCT, REF>(p, bp, cx, l, it, r, in);
-            if (in.ok) split_solve<DIM, T, OCT, REF>(p, bp, cx, r, in);
+            split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l, it, r, in);
+            if (in.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, in);
         }
         lds_barrier();
     };
     if constexpr (AHEAD == 1) {
         SplitIn b0, b1;
-        split_fetch<DIM, OCT, REF>(p, bp, cx, 0, qd, r, b0);
+        split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, 0, qd, r, b0);
         for (int l = 0; l < bp.nl; l += 2) {
             level(l, b0, b1);
             if (l + 1 < bp.nl) level(l + 1, b1, b0);
         }
     } else {
         SplitIn b0, b1, b2;
-        split_fetch<DIM, OCT, REF>(p, bp, cx, 0, qd, r, b0);
-        split_fetch<DIM, OCT, REF>(p, bp, cx, 1, qd, r, b1);
+        split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, 0, qd, r, b0);
+        split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, 1, qd, r, b1);
         for (int l = 0; l < bp.nl; l += 3) {
             level(l, b0, b2);
             if (l + 1 < bp.nl) level(l + 1, b1, b0);
@@ -908,20 +916,20 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     }
 }
 
-template <int DIM, typename T, int PF, bool REF>
+template <int DIM, typename T, int PF, bool REF, bool ONE>
 __global__ void __launch_bounds__(1024) k_sweep_box_split(SweepParams p, BoxPlan bp, int box_base)
 {
     extern __shared__ double lds[];
     const int oct = blockIdx.x % bp.noct;   // workgroup-uniform
     switch (oct) {
-    case 0: box_task_split<DIM, T, 0, PF, REF>(p, bp, box_base, lds); break;
-    case 1: box_task_split<DIM, T, 1, PF, REF>(p, bp, box_base, lds); break;
-    case 2: box_task_split<DIM, T, 2, PF, REF>(p, bp, box_base, lds); break;
-    case 3: box_task_split<DIM, T, 3, PF, REF>(p, bp, box_base, lds); break;
-    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4, PF, REF>(p, bp, box_base, lds); break;
-    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5, PF, REF>(p, bp, box_base, lds); break;
-    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6, PF, REF>(p, bp, box_base, lds); break;
-    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7, PF, REF>(p, bp, box_base, lds); break;
+    case 0: box_task_split<DIM, T, 0, PF, REF, ONE>(p, bp, box_base, lds); break;
+    case 1: box_task_split<DIM, T, 1, PF, REF, ONE>(p, bp, box_base, lds); break;
+    case 2: box_task_split<DIM, T, 2, PF, REF, ONE>(p, bp, box_base, lds); break;
+    case 3: box_task_split<DIM, T, 3, PF, REF, ONE>(p, bp, box_base, lds); break;
+    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4, PF, REF, ONE>(p, bp, box_base, lds); break;
+    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5, PF, REF, ONE>(p, bp, box_base, lds); break;
+    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6, PF, REF, ONE>(p, bp, box_base, lds); break;
+    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7, PF, REF, ONE>(p, bp, box_base, lds); break;
     }
 }
 
@@ -933,8 +941,14 @@ static hipError_t run_boxes_split(const DeviceSweep &ds, const SweepParams &prm,
     for (int k = 0; k < nlev; ++k) {
         const int nbox = ds.box_ptr[k + 1] - ds.box_ptr[k];
         if (nbox == 0) continue;
-        hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF, REF>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
-                           dim3(ds.box_threads), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
+        // one direction per workgroup (the tiling the autotuner picks at 64^3) has its own
+        // instantiation without the item -> (direction, cell) division
+        if (bp.nd == 1)
+            hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF, REF, true>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
+                               dim3(ds.box_threads), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
+        else
+            hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF, REF, false>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
+                               dim3(ds.box_threads), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
     }
     return hipGetLastError();
 }

@@ -108,7 +108,8 @@ def measure_traffic(args, tuning):
         d = tempfile.mkdtemp(prefix="allium_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run",
                "--", sys.executable, os.path.abspath(__file__), "--n", str(args.n), "--seed", str(args.seed),
-               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-trt", "--no-config3", "--no-config1", "--no-traffic", "--tuning", tun,
+               "--steps", "2", "--warmup", "0", "--no-cpu", "--no-si", "--no-trt", "--no-config3", "--no-config1", "--no-traffic",
+               "--no-fp32-check", "--tuning", tun,
                "--copy-mib", str(copy_mib), "--precision", args.precision, *box]
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=os.environ.get("TMPDIR", "/tmp"))
         path = os.path.join(d, "run_counter_collection.csv")
