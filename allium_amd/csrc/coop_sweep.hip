@@ -39,27 +39,28 @@ __device__ __forceinline__ void fence_row(double (&r)[N])
 }
 
 template <int DIM, int P, bool UNI, typename T>
-__global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepParams *__restrict__ prms,
-                                                    CoopLevel cl)
+__device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel &cl, int inst, long long wg0)
 {
     using S = Sh<DIM, P>;
     constexpr int N = S::N, NF = S::NF, NS = S::NS;
     static_assert(N <= 32, "one row per lane of a 32-lane group");
     __shared__ T piv[8][N + 1];
     const int slot = threadIdx.x >> 5, r = threadIdx.x & 31;
-    // each instance's items start on a multiple of 8, so the instance (and with it
-    // every parameter read through p) is uniform over the workgroup
-    const long long wg0 = (long long)blockIdx.x * 8;
-    int inst = 0;
-    while (inst + 1 < cl.ninst && wg0 >= cl.start[inst + 1]) ++inst;
-    const SweepParams &p = prms ? prms[inst] : p0;
+    // every table read through p is global memory
+    const auto g_cells = as_global(p.cells);
+    const auto g_omega = as_global(p.omega), g_sigma = as_global(p.sigma), g_K = as_global(p.K);
+    const auto g_Mu = as_global(p.Mu), g_Mt = as_global(p.Mt), g_Gd = as_global(p.Gd);
+    const auto g_src = as_global(p.src), g_iso = as_global(p.iso), g_F = as_global(p.Fface);
+    const auto g_fnor = as_global(p.fnor);
+    const auto g_refl = as_global(p.refl);
+    const auto g_psi = as_global(p.psi);
     const int local = (int)(wg0 - cl.start[inst]) + slot;
     const bool valid = local < cl.ncells[inst] * p.G;
     int e = 0, a = 0, g = 0;
     if (valid) {
         const int nc = cl.ncells[inst];
         g = local / nc;
-        const uint32_t cell = p.cells[cl.cbeg[inst] + (local - g * nc)];
+        const uint32_t cell = g_cells[cl.cbeg[inst] + (local - g * nc)];
         e = (int)(cell >> 10);
         a = (int)(cell & 1023u);
     }
@@ -68,67 +69,70 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
 
     // ---- row rr of the local system -----------------------------------------
     double Ar[N], b = 0.0;
-    const double om[3] = {p.omega[4 * a], p.omega[4 * a + 1], p.omega[4 * a + 2]};
-    const double sig = p.sigma[(long long)g * p.Ne + e];
+    const double om[3] = {g_omega[4 * a], g_omega[4 * a + 1], g_omega[4 * a + 2]};
+    const double sig = g_sigma[(long long)g * p.Ne + e];
     const long long base = ((long long)g * p.Nomega + a) * p.Ndof;
     const long long off = base + (long long)e * N;
+    // Each matrix row is read in two halves.  The half's row pointers pass
+    // through an empty asm after fences on every entry of the running sum, so
+    // the compiler cannot hoist all the row loads ahead of the sums (all 5 N of
+    // the jittered case took 256 VGPRs + AGPRs, one wave per SIMD); halves keep
+    // the kernel at 128 VGPRs, four waves per SIMD.
+    constexpr int H = (N + 1) / 2;
     if constexpr (UNI) {
-        const double *K = p.K + ((size_t)a * N + rr) * N;   // includes tau * Mt
-        const double *Mu = p.Mu + (size_t)rr * N;
+        auto K = g_K + ((size_t)a * N + rr) * N;   // includes tau * Mt
+        auto Mu = g_Mu + (size_t)rr * N;
 #pragma unroll
-        for (int j = 0; j < N; ++j) Ar[j] = fma(sig, Mu[j], K[j]);
+        for (int j = 0; j < H; ++j) Ar[j] = fma(sig, Mu[j], K[j]);
+        fence_row<H>(reinterpret_cast<double(&)[H]>(Ar));
+        asm volatile("" : "+v"(K), "+v"(Mu));
+#pragma unroll
+        for (int j = H; j < N; ++j) Ar[j] = fma(sig, Mu[j], K[j]);
     } else {
-        const double *Gd = p.Gd + (size_t)e * DIM * N * N + (size_t)rr * N;
-        const double *Mu = p.Mu + (size_t)e * N * N + (size_t)rr * N;
-        const double *Mt = p.Mt + (size_t)e * N * N + (size_t)rr * N;
-        const double tau = p.timedep ? p.tau : 0.0;
-        // one matrix row at a time: each phase's row pointer passes through an
-        // empty asm after fences on every entry of the running sum, so the
-        // compiler cannot hoist all 5 N loads ahead of the sums (that took
-        // 256 VGPRs + AGPRs, one wave per SIMD)
-        const double *q = Gd;
+        auto Gd = g_Gd + (size_t)e * DIM * N * N + (size_t)rr * N;
+        auto Mu = g_Mu + (size_t)e * N * N + (size_t)rr * N;
+        auto Mt = g_Mt + (size_t)e * N * N + (size_t)rr * N;
+        // A steady problem (tau = 0) skips the time-mass row, whose
+        // fma(0, Mt, .) would leave the sum unchanged.
+        auto phase = [&](auto q, double c, bool first) {
+            if (!first) {
+                fence_row<N>(Ar);
+                asm volatile("" : "+v"(q));
+            }
 #pragma unroll
-        for (int j = 0; j < N; ++j) Ar[j] = om[0] * q[j];
-#pragma unroll
-        for (int d = 1; d < DIM; ++d) {
-            q = Gd + (size_t)d * N * N;
-            fence_row<N>(Ar);
+            for (int j = 0; j < H; ++j) Ar[j] = first ? c * q[j] : fma(c, q[j], Ar[j]);
+            fence_row<H>(reinterpret_cast<double(&)[H]>(Ar));
             asm volatile("" : "+v"(q));
 #pragma unroll
-            for (int j = 0; j < N; ++j) Ar[j] = fma(om[d], q[j], Ar[j]);
-        }
-        q = Mu;
-        fence_row<N>(Ar);
-        asm volatile("" : "+v"(q));
+            for (int j = H; j < N; ++j) Ar[j] = first ? c * q[j] : fma(c, q[j], Ar[j]);
+        };
+        phase(Gd, om[0], true);
 #pragma unroll
-        for (int j = 0; j < N; ++j) Ar[j] = fma(sig, q[j], Ar[j]);
-        q = Mt;
-        fence_row<N>(Ar);
-        asm volatile("" : "+v"(q));
-#pragma unroll
-        for (int j = 0; j < N; ++j) Ar[j] = fma(tau, q[j], Ar[j]);
+        for (int d = 1; d < DIM; ++d) phase(Gd + (size_t)d * N * N, om[d], false);
+        phase(Mu, sig, false);
+        if (p.timedep) phase(Mt, p.tau, false);
     }
-    if (p.src) b = p.src[off + rr];
-    if (p.iso) b += p.iso[(long long)g * p.Ndof + (long long)e * N + rr];
+    if (p.src) b = g_src[off + rr];
+    if (p.iso) b += g_iso[(long long)g * p.Ndof + (long long)e * N + rr];
     int c[3] = {e % p.nel[0], (e / p.nel[0]) % p.nel[1], e / (p.nel[0] * p.nel[1])};
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int ax = s >> 1, hi = s & 1;
         const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
         double dot;
-        const double *F;
+        gptr<const double> F;
         if constexpr (UNI) {
             dot = hi ? om[ax] : -om[ax];
-            F = p.Fface + s * NF * NF;
+            F = g_F + s * NF * NF;
         } else {
             int cf[3] = {c[0], c[1], c[2]};
             cf[ax] += hi;
             const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
             dot = 0.0;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) dot = fma(om[d], p.fnor[3 * f + d], dot);
+            for (int d = 0; d < DIM; ++d) dot = fma(om[d], g_fnor[3 * f + d], dot);
             if (!hi && interior) dot = -dot;
-            F = p.Fface + (size_t)f * NF * NF;
+            F = g_F + (size_t)f * NF * NF;
         }
         int k = 0;
         if (!on_side<DIM, P>(rr, s, k)) continue;
@@ -142,11 +146,11 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
         if (dot < 0 && (interior || refl)) {
             double acc = 0.0;
             if (interior) {
-                const double *un = p.psi + base + (long long)(hi ? e + p.stride[ax] : e - p.stride[ax]) * N;
+                const auto un = g_psi + base + (long long)(hi ? e + p.stride[ax] : e - p.stride[ax]) * N;
 #pragma unroll
                 for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], un[side_node<DIM, P>(s ^ 1, l)], acc);
             } else {
-                const double *ur = p.psi + ((long long)g * p.Nomega + p.refl[a * NS + s]) * p.Ndof + (long long)e * N;
+                const auto ur = g_psi + ((long long)g * p.Nomega + g_refl[a * NS + s]) * p.Ndof + (long long)e * N;
 #pragma unroll
                 for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], ur[side_node<DIM, P>(s, l)], acc);
             }
@@ -189,7 +193,26 @@ __global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepP
         if (r < k) bb = fma(-A[k], xk, bb);
         cell_sync();
     }
-    if (row) p.psi[off + rr] = (double)x;
+    if (row) g_psi[off + rr] = (double)x;
+}
+
+// BATCH: the instances' parameters are the device array prms (a level of
+// several instances); otherwise p0, one instance.  The two are separate
+// instantiations so that the parameters' address space is known (a pointer
+// selected between the two would be generic and its loads flat).
+template <int DIM, int P, bool UNI, typename T, bool BATCH>
+__global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepParams *__restrict__ prms,
+                                                    CoopLevel cl)
+{
+    // each instance's items start on a multiple of 8, so the instance (and with it
+    // every parameter read through p) is uniform over the workgroup
+    const long long wg0 = (long long)blockIdx.x * 8;
+    int inst = 0;
+    while (inst + 1 < cl.ninst && wg0 >= cl.start[inst + 1]) ++inst;
+    if constexpr (BATCH)
+        coop_cell<DIM, P, UNI, T>(*(const SweepParams *)as_global(prms + inst), cl, inst, wg0);
+    else
+        coop_cell<DIM, P, UNI, T>(p0, cl, inst, wg0);
 }
 
 template <int DIM, int P, bool UNI>
@@ -199,10 +222,17 @@ static hipError_t launch_dp(const SweepParams &p0, const SweepParams *d_prms, co
     const long long items = cl.start[cl.ninst];
     if (items <= 0) return hipSuccess;
     const int grid = (int)((items + 7) / 8);
-    if (precision == 1)
-        hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, float>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
-    else
-        hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, double>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
+    if (precision == 1) {
+        if (d_prms)
+            hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, float, true>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
+        else
+            hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, float, false>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
+    } else {
+        if (d_prms)
+            hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, double, true>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
+        else
+            hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, double, false>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
+    }
     return hipGetLastError();
 }
 

@@ -11,6 +11,14 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 // addresses become scalar loads, non-uniform ones stay vector loads
 typedef const __attribute__((address_space(4))) double *cdptr;
 __device__ __forceinline__ cdptr as_const(const double *p) { return (cdptr)p; }
+// Global-address-space views of pointers the compiler cannot prove global
+// (pointers read from device memory are generic): loads through them are
+// global_load instructions instead of flat ones, which also count against the
+// LDS wait counter and so make every LDS wait drain them.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(T *p) { return (gptr<T>)p; }
 
 template <int DIM, int P>
 struct Sh {
