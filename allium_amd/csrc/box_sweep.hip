@@ -941,14 +941,19 @@ static hipError_t run_boxes_split(const DeviceSweep &ds, const SweepParams &prm,
     for (int k = 0; k < nlev; ++k) {
         const int nbox = ds.box_ptr[k + 1] - ds.box_ptr[k];
         if (nbox == 0) continue;
+        // A box level with at most one workgroup per CU (the levels near the
+        // mesh corners) runs twice the threads per workgroup, up to 1024: the
+        // CU has the room, and a box level then takes fewer passes over its cells.
+        const long long wgs = (long long)nbox * bp.noct * prm.G * bp.nchunk;
+        const int th = (wgs <= ds.box_ncu && ds.box_threads <= 512) ? 2 * ds.box_threads : ds.box_threads;
         // one direction per workgroup (the tiling the autotuner picks at 64^3) has its own
         // instantiation without the item -> (direction, cell) division
         if (bp.nd == 1)
             hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF, REF, true>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
-                               dim3(ds.box_threads), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
+                               dim3(th), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
         else
             hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF, REF, false>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
-                               dim3(ds.box_threads), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
+                               dim3(th), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);
     }
     return hipGetLastError();
 }

@@ -116,6 +116,7 @@ struct DeviceSweep {
     BoxPlan box{};
     std::vector<int> box_ptr;    // box level -> first entry of box.boxes
     int box_threads = 256;
+    int box_ncu = 0;             // CUs of the device (box launches of at most one workgroup per CU)
     int refine = 0;              // box sweeps at p = 1: in-place defect correction with FP32 cell solves
     int box_prefetch = 2;        // 0 none, 2 next level's inputs loaded during the current solve
     void *box_buf = nullptr;     // device memory of the box plan tables

@@ -334,9 +334,10 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
             return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps: element matrices are not the tensor-product masses "
                                                 "of an axis-aligned box");
     }
-    int dev = 0, maxlds = 65536;
+    int dev = 0, maxlds = 65536, ncu = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (lds > std::min(maxlds, 65536))
         return fail(ALLIUM_ERR_ARG, "box tiling needs " + std::to_string(lds) +
                                         " bytes of LDS per workgroup (limit 65536): use smaller boxes or fewer "
@@ -389,6 +390,7 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
     ds.box = bp;
     ds.box_ptr = bptr;
     ds.box_threads = threads;
+    ds.box_ncu = ncu;
     ds.box_prefetch = prefetch;
     if (ds.graph_exec) { hipGraphExecDestroy(ds.graph_exec); ds.graph_exec = nullptr; }
     return ALLIUM_OK;
