@@ -249,6 +249,9 @@ int build_box_plan(allium_sweep *h, int bx, int by, int bz, int nd, int threads,
         return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps need directions that do not couple (no reflective side)");
     if (!allium::box_sweep_supported(dim, h->ed.p))
         return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps implement 2D p=0..2 and 3D p=0..1");
+    // the box kernels build the cell operator from the exact 1D masses and derivatives
+    if (h->ed.lump)
+        return fail(ALLIUM_ERR_UNSUPPORTED, "box sweeps take the unlumped element matrices (lump = 0)");
     int bs[3] = {bx, by, dim == 3 ? bz : 1};
     for (int d = 0; d < dim; d++) {
         if (bs[d] < 1 || bs[d] > 1023) return fail(ALLIUM_ERR_ARG, "box extents must be 1..1023");
@@ -503,7 +506,12 @@ int allium_sweep_create(const allium_mesh_desc *mesh, int fe_order, const allium
 {
     if (!mesh || !quad || !sigma_t || !out || G < 1) return fail(ALLIUM_ERR_ARG, "null argument or G < 1");
     *out = nullptr;
-    if (lump != 0) return fail(ALLIUM_ERR_UNSUPPORTED, "lumped sweep matrices are not implemented on the GPU");
+    if (lump < 0 || lump > 7)
+        return fail(ALLIUM_ERR_ARG, "lump is a bit set: 1 mass, 2 gradient, 4 face terms on the vertex rule");
+    // the vertex rule leaves a p = 2 mass matrix singular (8 points, 27 or 9
+    // dofs); the GPU cell solves factor without pivoting
+    if (lump != 0 && fe_order > 1)
+        return fail(ALLIUM_ERR_UNSUPPORTED, "lumped matrices are implemented for fe_order 0 and 1");
     std::string why;
     if (!allium::sweep_supported(mesh->dim, fe_order, why)) return fail(ALLIUM_ERR_UNSUPPORTED, why);
     auto h = std::make_unique<allium_sweep>();
@@ -614,7 +622,7 @@ int allium_sweep_get_time_absorption(allium_sweep_t h, double *tau)
 int allium_sweep_get_lumping(allium_sweep_t h, int *lump, int *parallel_block_jacobi)
 {
     if (!h) return fail(ALLIUM_ERR_ARG, "null handle");
-    if (lump) *lump = 0;
+    if (lump) *lump = h->ed.lump;
     if (parallel_block_jacobi) *parallel_block_jacobi = 0;
     return ALLIUM_OK;
 }
@@ -765,7 +773,7 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
                 if (ln > 0 && h->ed.n % ln) continue;
                 cands.push_back({md, bl, ln, 0, 0, 0, 0, 0});
             }
-    const bool boxes = !coop && h->mesh.uniform && h->ds.angle_independent &&
+    const bool boxes = !coop && h->mesh.uniform && h->ds.angle_independent && h->ed.lump == 0 &&
                        allium::box_sweep_supported(h->mesh.dim, h->ed.p);
     if (boxes) {
         const bool d3 = h->mesh.dim == 3;
