@@ -4,11 +4,11 @@
 // in the reference; it stands in for the diffusion synthetic acceleration of
 // src/transport_op.hpp:38-59 inside the source iteration.
 //
-// Convolution as an implicit GEMM on v_mfma_f32_16x16x32_bf16: a wave owns 16
-// consecutive voxels (rows of the A tile) and all output channels (N tiles of
-// 16); the K dimension runs over (kernel offset, input channel) in chunks of
-// 8 channels of one offset, so every A operand is one 16-byte load of a
-// channels-last voxel row.  Lane l holds A[l % 16][8 (l / 16) + t],
+// Convolution as an implicit GEMM on v_mfma_f32_16x16x32_bf16: a wave owns 4
+// tiles of 16 consecutive voxels (rows of the A tiles) and all output channels
+// (N tiles of 16); the K dimension runs over (kernel offset, input channel) in
+// chunks of 8 channels of one offset, so every A operand is one 16-byte load
+// of a channels-last voxel row.  Lane l holds A[l % 16][8 (l / 16) + t],
 // B[8 (l / 16) + t][l % 16] and D[4 (l / 16) + r][l % 16].
 //
 // Weight gradients: a workgroup owns one (sample, z) plane; for each row and
@@ -54,74 +54,97 @@ constexpr int kMaxDil = 8;
 // MODE 1: + bias, FP32 out [T] (channel 0)
 // MODE 2: masked by (mask[T][COP] > 0), bf16 out [T][COP]
 // ---------------------------------------------------------------------------
+// A wave owns MT consecutive 16-voxel row tiles: every B operand it loads (a
+// 16-byte weight row) feeds MT MFMAs, and each K step issues MT + NT loads for
+// MT * NT MFMAs.  Each output element still accumulates its K steps in order.
+constexpr int kConvMT = 4;
+constexpr int kConvVoxels = 4 * 16 * kConvMT;   // per 256-thread workgroup
+
 template <int CIP, int COP, int MODE>
 __global__ void __launch_bounds__(256) k_conv(Geo g, const bf16 *__restrict__ in, const bf16 *__restrict__ W,
                                               const float *__restrict__ bias, bf16 *__restrict__ out_bf,
                                               float *__restrict__ out_f, const bf16 *__restrict__ mask)
 {
-    constexpr int CH = CIP / 8, NCH = 27 * CH, KS = (NCH + 3) / 4, NT = COP / 16;
+    constexpr int CH = CIP / 8, NCH = 27 * CH, KS = (NCH + 3) / 4, NT = COP / 16, MT = kConvMT;
     static_assert(CIP % 8 == 0 && COP % 16 == 0, "padded channel counts");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, grp = lane >> 4;
-    const long long m0 = ((long long)blockIdx.x * 4 + w) * 16;
+    const long long m0 = ((long long)blockIdx.x * 4 + w) * (16 * MT);
     if (m0 >= g.T) return;   // wave-uniform; this kernel has no barriers
-    const long long m = m0 + (lane & 15);
-    const bool mv = m < g.T;
-    int x = 0, y = 0, z = 0;
-    if (mv) {
-        long long r = m;
-        x = (int)(r % g.nx);
-        r /= g.nx;
-        y = (int)(r % g.ny);
-        r /= g.ny;
-        z = (int)(r % g.nz);
-    }
-    f32x4 acc[NT];
+    long long m[MT];
+    bool mv[MT];
+    int x[MT], y[MT], z[MT];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < MT; ++i) {
+        m[i] = m0 + 16 * i + (lane & 15);
+        mv[i] = m[i] < g.T;
+        x[i] = y[i] = z[i] = 0;
+        if (mv[i]) {
+            long long r = m[i];
+            x[i] = (int)(r % g.nx);
+            r /= g.nx;
+            y[i] = (int)(r % g.ny);
+            r /= g.ny;
+            z[i] = (int)(r % g.nz);
+        }
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[i][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // Every load is issued on every path, from an in-bounds address (voxel 0 or
+    // weight row 0 when the operand is padding), and the padding is zeroed by a
+    // select: loads behind branches made the compiler wait for each pair of them
+    // before the next was issued.
+    const bf16x8 zero8 = {};
     for (int s = 0; s < KS; ++s) {
         const int c = 4 * s + grp;
         const bool cv = c < NCH;
-        const int o = cv ? c / CH : 0;
-        const int ci0 = cv ? (c - o * CH) * 8 : 0;
-        bf16x8 a;
+        const int cc = cv ? c : 0;
+        const int o = cc / CH;
+        const int ci0 = (cc - o * CH) * 8;
+        const int dz = (o / 9 - 1) * g.dil, dy = ((o / 3) % 3 - 1) * g.dil, dx = (o % 3 - 1) * g.dil;
+        bf16x8 a[MT];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) a[t] = (bf16)0.f;
-        if (cv && mv) {
-            const int dz = (o / 9 - 1) * g.dil, dy = ((o / 3) % 3 - 1) * g.dil, dx = (o % 3 - 1) * g.dil;
-            const int xx = x + dx, yy = y + dy, zz = z + dz;
-            if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz) {
-                const long long mn = m + dx + (long long)g.nx * (dy + (long long)g.ny * dz);
-                a = *reinterpret_cast<const bf16x8 *>(in + mn * CIP + ci0);
-            }
+        for (int i = 0; i < MT; ++i) {
+            const int xx = x[i] + dx, yy = y[i] + dy, zz = z[i] + dz;
+            const bool ok = cv && mv[i] && xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
+            const long long mn = ok ? m[i] + dx + (long long)g.nx * (dy + (long long)g.ny * dz) : 0;
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(in + mn * CIP + ci0);
+            a[i] = ok ? v : zero8;
         }
+        bf16x8 b[NT];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
-            bf16x8 b;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) b[t] = (bf16)0.f;
-            if (cv) b = *reinterpret_cast<const bf16x8 *>(W + ((long long)o * COP + nt * 16 + (lane & 15)) * CIP + ci0);
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(W + ((long long)o * COP + nt * 16 + (lane & 15)) * CIP + ci0);
+            b[nt] = cv ? v : zero8;
         }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int i = 0; i < MT; ++i) acc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[nt], acc[i][nt], 0, 0, 0);
     }
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int j = nt * 16 + (lane & 15);
+    for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const long long mo = m0 + 4 * grp + r;
-            if (mo >= g.T) continue;
-            float v = acc[nt][r];
-            if constexpr (MODE == 0) {
-                v = fmaxf(v + bias[j], 0.f);
-                out_bf[mo * COP + j] = (bf16)v;
-            } else if constexpr (MODE == 1) {
-                if (j == 0) out_f[mo] = v + bias[0];
-            } else {
-                v = (float)mask[mo * COP + j] > 0.f ? v : 0.f;
-                out_bf[mo * COP + j] = (bf16)v;
+        for (int nt = 0; nt < NT; ++nt) {
+            const int j = nt * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long mo = m0 + 16 * i + 4 * grp + r;
+                if (mo >= g.T) continue;
+                float v = acc[i][nt][r];
+                if constexpr (MODE == 0) {
+                    v = fmaxf(v + bias[j], 0.f);
+                    out_bf[mo * COP + j] = (bf16)v;
+                } else if constexpr (MODE == 1) {
+                    if (j == 0) out_f[mo] = v + bias[0];
+                } else {
+                    v = (float)mask[mo * COP + j] > 0.f ? v : 0.f;
+                    out_bf[mo * COP + j] = (bf16)v;
+                }
             }
         }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -500,7 +523,7 @@ Geo layer_geo(const allium_cnn *h, const Geo &g, int l)
 
 int forward_packed(allium_cnn *h, const Geo &g, hipStream_t s)
 {
-    const int grid = (int)((g.T + 63) / 64);
+    const int grid = (int)((g.T + kConvVoxels - 1) / kConvVoxels);
     hipLaunchKernelGGL((k_conv<8, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 0), h->xin, h->Wf[0],
                        h->bias[0], h->act[0], nullptr, nullptr);
     hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 1), h->act[0], h->Wf[1],
@@ -538,7 +561,7 @@ int gradient(allium_cnn *h, const Geo &g, const float *d_x, const float *d_t, do
     rc = forward_packed(h, g, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, s, g.T, h->out, d_t, h->ga, h->lpart);
-    const int grid = (int)((g.T + 63) / 64);
+    const int grid = (int)((g.T + kConvVoxels - 1) / kConvVoxels);
     const int planes = g.B * g.nz;
     // layer 5: gz = ga [T][16]
     const bf16 *gz = h->ga;
