@@ -151,10 +151,17 @@ def generate_training_data(n=32, media=200, seed0=1000, iters=(1, 2, 3, 4, 6, 8,
     Xs, Ts = [], []
     shape = (n, n, n)
     t0 = time.perf_counter()
+    Linv = q = None
     for i in range(media):
         cfg = P.problem_3d_hetero(n=n, seed=seed0 + i, log_sd=log_sd, c_const=c)
-        Linv = A.InverseAdvectionOperator(A.Mesh(cfg["nel"]), 1, sn, cfg["sigma_t"])
-        q = Linv.form_source(Q=1.0)
+        if Linv is None:
+            # one sweep handle for every medium: only sigma_t changes (the source
+            # Q = 1 does not depend on it), and the launch strategy is tuned once
+            Linv = A.InverseAdvectionOperator(A.Mesh(cfg["nel"]), 1, sn, cfg["sigma_t"])
+            Linv.set_launch(-1)
+            q = Linv.form_source(Q=1.0)
+        else:
+            Linv.AssembleLocalMatrices(cfg["sigma_t"])
         hist = np.zeros((maxit, 2, n ** 3))
         phi, _, it, _ = A.source_iteration(Linv, cfg["sigma_s"], q, abstol=1e-8, maxit=maxit, want_psi=False,
                                            history=hist, accel=accel, patience=3)
