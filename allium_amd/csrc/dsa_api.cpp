@@ -748,6 +748,9 @@ int allium_dsa_create(allium_sweep_t h, const double *sigma_s, const allium_dsa_
     // the vertex coarse space needs nodal traces (p >= 1): with p = 0 the
     // Galerkin operator couples vertices two elements apart
     if (h->ed.p < 1) return fail(ALLIUM_ERR_UNSUPPORTED, "DSA needs fe_order >= 1");
+    // chive assembles its DSA operators with the transport's lumping
+    // (exe/chive.cpp:610-666); these are assembled unlumped
+    if (h->ed.lump) return fail(ALLIUM_ERR_UNSUPPORTED, "DSA is implemented for unlumped sweep handles (lump = 0)");
     auto d = std::make_unique<allium_dsa>();
     d->h = h;
     if (desc) d->desc = *desc; else allium_dsa_default_desc(&d->desc);

@@ -20,6 +20,7 @@ Deck keys (YAML):
     sn: {type: level symmetric, order: 4}
     driver:
       fe_order: 1
+      lumping: 0                       # chive's lumping bits (1 mass, 2 gradient, 4 faces); 7 lumps all
       solver: {type: sli, abstol: 1.0e-6, reltol: 0, max_iter: 50}   # or gmres / fgmres with kdim
       preconditioner: {type: scalar mip, kappa: 4, solver: {type: cg, reltol: 1.0e-10, max_iter: 200}}
       # or {type: mip} (the block system) / {type: p1sa, solver: {type: direct}}; a "direct"
@@ -132,6 +133,9 @@ def run(deck):
         raise ValueError("only level-symmetric quadrature is supported")
     drv = deck.get("driver", {})
     p = int(drv.get("fe_order", 1))
+    lump = int(drv.get("lumping", 0))   # chive's bit set: 1 mass, 2 gradient, 4 faces
+    if not 0 <= lump <= 7:
+        raise ValueError("lumping is a bit set in 0..7 (1 mass, 2 gradient, 4 faces)")
     sol = drv.get("solver", {"type": "sli"})
     stype = str(sol.get("type", "sli")).lower()
     if stype not in ("sli", "gmres", "fgmres"):
@@ -153,7 +157,8 @@ def run(deck):
     out = {}
     t0 = time.perf_counter()
     mesh = A.Mesh(nel, extents=ext, bc=bc)
-    Linv = A.InverseAdvectionOperator(mesh, p, int(sn.get("order", 4)), sigma_t.reshape(1, -1))
+    Linv = A.InverseAdvectionOperator(mesh, p, int(sn.get("order", 4)), sigma_t.reshape(1, -1), lump=lump)
+    out["lumping type"] = {"mass": bool(lump & 1), "gradient": bool(lump & 2), "faces": bool(lump & 4)}
     Linv.set_launch(-1)
     q = Linv.form_source(Q=Q.reshape(1, -1), inflow=np.array(inflow).reshape(1, -1))
     space = None

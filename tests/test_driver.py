@@ -9,7 +9,10 @@ import numpy as np
 import pytest
 import yaml
 
+import oracle
 from allium_amd import driver
+from allium_amd import problems as P
+from tests.helpers import ls_fixture
 
 DECK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "decks", "tdl.yaml")
 
@@ -28,6 +31,7 @@ def _deck():
     (lambda d: d["boundary_conditions"]["vacuum"].update(type="periodic"), "not supported"),
     (lambda d: d["driver"].update(preconditioner={"type": "h1sa"}), "not supported"),
     (lambda d: d["driver"].update(preconditioner={"type": "p1sa", "solver": {"type": "cg"}}), "only direct"),
+    (lambda d: d["driver"].update(lumping=9), "lumping is a bit set"),
 ])
 def test_deck_errors(edit, msg):
     d = _deck()
@@ -148,3 +152,25 @@ def test_mms_deck_order_of_accuracy(prec):
     J_ooa = np.log2(errs[0][1] / errs[1][1])
     print(f"  phi {phi_ooa:.3f} ({errs[0][0]:.3e}, {errs[1][0]:.3e}), J {J_ooa:.3f} ({errs[0][1]:.3e}, {errs[1][1]:.3e})")
     assert abs(phi_ooa - 2) < 0.2 and abs(J_ooa - 2) < 0.2
+
+
+@pytest.mark.gpu
+def test_lumped_deck_matches_oracle():
+    """chive's `driver.lumping = 7` (exe/chive.cpp:532-568,594): a scattering
+    deck with every matrix on the vertex rule, the scattering mass included,
+    converges to the oracle's lumped source iteration and reports chive's
+    "lumping type" keys."""
+    print("driver lumped deck", flush=True)
+    d = {"mesh": {"num_elements": [8, 8], "extents": [1, 1]},
+         "materials": {"mat": {"total": 1.0, "scattering": 0.5, "source": 1.0}},
+         "material_map": "mat", "boundary_conditions": {"vacuum": {"type": "inflow", "value": 0}},
+         "boundary_map": "vacuum", "sn": {"type": "level symmetric", "order": 4},
+         "driver": {"fe_order": 1, "lumping": 7, "solver": {"type": "sli", "abstol": 1e-12, "max_iter": 200}}}
+    out = driver.run(d)
+    assert out["converged"] and out["lumping type"] == {"mass": True, "gradient": True, "faces": True}
+    om, w = ls_fixture(4, 2)
+    pr = oracle.Problem(2, (8, 8), P.cartesian_vertices((8, 8)), 1, om, w, np.ones((1, 64)), lump=7)
+    phi, _, it, _ = pr.source_iteration(0.5, pr.form_source(Q=1.0), abstol=1e-12, maxit=200)
+    print(f"  driver {out['outer iterations']} its, oracle {it} its")
+    assert out["outer iterations"] == it
+    assert abs(out["scalar flux"]["l2"] - np.linalg.norm(phi)) < 1e-10 * np.linalg.norm(phi)

@@ -120,3 +120,12 @@ def test_lumped_handles_decline_box_sweeps_and_autotune_to_levels():
 def test_lumped_p2_is_declined():
     with pytest.raises(A.AlliumError, match="fe_order 0 and 1"):
         A.InverseAdvectionOperator(A.Mesh((4, 4)), 2, 4, np.ones((1, 16)), lump=7)
+
+
+def test_dsa_declines_lumped_handles():
+    """chive assembles its DSA operators with the transport lumping
+    (exe/chive.cpp:610-666); the library's DSA operators are unlumped, so a
+    lumped handle is declined rather than preconditioned inconsistently."""
+    Linv = A.InverseAdvectionOperator(A.Mesh((4, 4, 4)), 1, 4, np.ones((1, 64)), lump=1)
+    with pytest.raises(A.AlliumError, match="unlumped"):
+        A.DSA(Linv, 0.5 * np.ones(64))
