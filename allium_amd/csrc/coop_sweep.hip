@@ -73,6 +73,85 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
     const double sig = g_sigma[(long long)g * p.Ne + e];
     const long long base = ((long long)g * p.Nomega + a) * p.Ndof;
     const long long off = base + (long long)e * N;
+    if (p.src) b = g_src[off + rr];
+    if (p.iso) b += g_iso[(long long)g * p.Ndof + (long long)e * N + rr];
+
+    // Faces, one axis at a time: a node lies on at most one side per axis (the
+    // low side where its coordinate on the axis is 0, the high side where it
+    // is P).  Every address follows from the cell alone, so the face normal,
+    // the face-mass row and the upwind trace are loaded without branches, and
+    // the sign of Omega.n only selects what they are added to.  The inflow
+    // traces go into b before the element matrices are read (the row sum is
+    // not live yet); on non-uniform geometry the outflow face blocks join the
+    // row after them (a uniform mesh has them in K).  Both passes take the
+    // sides in the order 2 axis + {low, high}, as one loop over the sides would.
+    constexpr int Q = P + 1;
+    const int ic[3] = {rr % Q, (rr / Q) % Q, rr / (Q * Q)};
+    const int c[3] = {e % p.nel[0], (e / p.nel[0]) % p.nel[1], e / (p.nel[0] * p.nel[1])};
+    // the node's side on axis ax (-1: none) and its face-lexicographic index there
+    auto side_of = [&](int ax, int &k) {
+        k = ax == 0 ? ic[1] + Q * ic[2] : (ax == 1 ? ic[0] + Q * ic[2] : ic[0] + Q * ic[1]);
+        return ic[ax] == 0 ? 2 * ax : (ic[ax] == P ? 2 * ax + 1 : -1);
+    };
+    // node l of a side of axis ax, with coordinate 0 on the axis
+    auto side_base = [](int ax, int l) {
+        return ax == 0 ? Q * (l % Q) + Q * Q * (l / Q) : (ax == 1 ? l % Q + Q * Q * (l / Q) : l);
+    };
+    // face of side s (non-uniform geometry)
+    auto face_of = [&](int s) {
+        const int ax = s >> 1;
+        int cf[3] = {c[0], c[1], c[2]};
+        cf[ax] += s & 1;
+        return p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
+    };
+    double dots[DIM];   // Omega.n on the node's side of each axis, 0 where it has none
+    auto gF = g_F;
+    auto gN = g_fnor;
+    auto gP = g_psi;
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+        // one axis's loads in flight at a time
+        if (ax > 0) asm volatile("" : "+v"(b), "+s"(gF), "+s"(gN), "+s"(gP));
+        int k;
+        const int sd = side_of(ax, k);
+        const bool on = sd >= 0;
+        const int hi = sd & 1, s = 2 * ax + hi;
+        const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
+        double dot;
+        gptr<const double> F;
+        if constexpr (UNI) {
+            dot = hi ? om[ax] : -om[ax];
+            F = gF + s * NF * NF + k * NF;
+        } else {
+            const int f = face_of(s);
+            dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dot = fma(om[d], gN[3 * f + d], dot);
+            if (!hi && interior) dot = -dot;
+            F = gF + (size_t)f * NF * NF + k * NF;
+        }
+        // upwind trace: the neighbour's nodes on the opposite side, or on a
+        // reflective side the reflected direction's own nodes; elsewhere the
+        // cell's own entries stand in as an in-bounds address and are not used
+        const bool refl = !interior && ((p.refl_mask >> s) & 1);
+        const int sax = ax == 0 ? 1 : (ax == 1 ? Q : Q * Q);   // node stride along the axis
+        gptr<const double> u = gP + off;
+        int ub = 0;
+        if (interior) {
+            u = gP + base + (long long)(hi ? e + p.stride[ax] : e - p.stride[ax]) * N;
+            ub = hi ? 0 : P * sax;
+        } else if (refl) {
+            u = gP + ((long long)g * p.Nomega + g_refl[a * NS + s]) * p.Ndof + (long long)e * N;
+            ub = hi ? P * sax : 0;
+        }
+        dots[ax] = on ? dot : 0.0;
+        double acc = 0.0;
+#pragma unroll
+        for (int l = 0; l < NF; ++l) acc = fma(F[l], u[ub + side_base(ax, l)], acc);
+        const bool inflow = on && dot < 0 && (interior || refl);
+        b = inflow ? fma(-dot, acc, b) : b;
+    }
+
     // Each matrix row is read in two halves.  The half's row pointers pass
     // through an empty asm after fences on every entry of the running sum, so
     // the compiler cannot hoist all the row loads ahead of the sums (all 5 N of
@@ -80,7 +159,7 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
     // the kernel at 128 VGPRs, four waves per SIMD.
     constexpr int H = (N + 1) / 2;
     if constexpr (UNI) {
-        auto K = g_K + ((size_t)a * N + rr) * N;   // includes tau * Mt
+        auto K = g_K + ((size_t)a * N + rr) * N;   // includes tau * Mt and the outflow faces
         auto Mu = g_Mu + (size_t)rr * N;
 #pragma unroll
         for (int j = 0; j < H; ++j) Ar[j] = fma(sig, Mu[j], K[j]);
@@ -94,11 +173,11 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
         auto Mt = g_Mt + (size_t)e * N * N + (size_t)rr * N;
         // A steady problem (tau = 0) skips the time-mass row, whose
         // fma(0, Mt, .) would leave the sum unchanged.
+        // (the first phase starts after the face pass: its pointer passes through
+        // the same empty asm as b)
         auto phase = [&](auto q, double c, bool first) {
-            if (!first) {
-                fence_row<N>(Ar);
-                asm volatile("" : "+v"(q));
-            }
+            if (!first) fence_row<N>(Ar);
+            asm volatile("" : "+v"(q), "+v"(b));
 #pragma unroll
             for (int j = 0; j < H; ++j) Ar[j] = first ? c * q[j] : fma(c, q[j], Ar[j]);
             fence_row<H>(reinterpret_cast<double(&)[H]>(Ar));
@@ -111,50 +190,36 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
         for (int d = 1; d < DIM; ++d) phase(Gd + (size_t)d * N * N, om[d], false);
         phase(Mu, sig, false);
         if (p.timedep) phase(Mt, p.tau, false);
-    }
-    if (p.src) b = g_src[off + rr];
-    if (p.iso) b += g_iso[(long long)g * p.Ndof + (long long)e * N + rr];
-    int c[3] = {e % p.nel[0], (e / p.nel[0]) % p.nel[1], e / (p.nel[0] * p.nel[1])};
+        // outflow: the own face block joins the row (the face-mass rows were
+        // read by the face pass, so these loads are cache hits)
+        // All three rows are loaded before any is used (a row loaded only where
+        // it is added would become one branch and one round trip per entry).
+        fence_row<N>(Ar);
+        asm volatile("" : "+s"(gF));
+        double Fk[DIM][NF];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const int ax = s >> 1, hi = s & 1;
-        const bool interior = hi ? (c[ax] < p.nel[ax] - 1) : (c[ax] > 0);
-        double dot;
-        gptr<const double> F;
-        if constexpr (UNI) {
-            dot = hi ? om[ax] : -om[ax];
-            F = g_F + s * NF * NF;
-        } else {
-            int cf[3] = {c[0], c[1], c[2]};
-            cf[ax] += hi;
-            const int f = p.foff[ax] + cf[0] + p.fext[ax][0] * (cf[1] + p.fext[ax][1] * cf[2]);
-            dot = 0.0;
+        for (int ax = 0; ax < DIM; ++ax) {
+            int k;
+            const int hi = side_of(ax, k) & 1;
+            const auto F = gF + (size_t)face_of(2 * ax + hi) * NF * NF + k * NF;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) dot = fma(om[d], g_fnor[3 * f + d], dot);
-            if (!hi && interior) dot = -dot;
-            F = g_F + (size_t)f * NF * NF;
+            for (int l = 0; l < NF; ++l) Fk[ax][l] = F[l];
         }
-        int k = 0;
-        if (!on_side<DIM, P>(rr, s, k)) continue;
-        if constexpr (!UNI) {
-            if (dot > 0) {   // outflow: own face block
 #pragma unroll
-                for (int l = 0; l < NF; ++l) Ar[side_node<DIM, P>(s, l)] = fma(dot, F[k * NF + l], Ar[side_node<DIM, P>(s, l)]);
-            }
-        }
-        const bool refl = !interior && ((p.refl_mask >> s) & 1);
-        if (dot < 0 && (interior || refl)) {
-            double acc = 0.0;
-            if (interior) {
-                const auto un = g_psi + base + (long long)(hi ? e + p.stride[ax] : e - p.stride[ax]) * N;
+        for (int ax = 0; ax < DIM; ++ax) fence_row<NF>(Fk[ax]);
 #pragma unroll
-                for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], un[side_node<DIM, P>(s ^ 1, l)], acc);
-            } else {
-                const auto ur = g_psi + ((long long)g * p.Nomega + g_refl[a * NS + s]) * p.Ndof + (long long)e * N;
+        for (int ax = 0; ax < DIM; ++ax) {
+            int k;
+            const int hi = side_of(ax, k) & 1;
+            const int sax = ax == 0 ? 1 : (ax == 1 ? Q : Q * Q);
+            const bool outflow = dots[ax] > 0;
 #pragma unroll
-                for (int l = 0; l < NF; ++l) acc = fma(F[k * NF + l], ur[side_node<DIM, P>(s, l)], acc);
-            }
-            b = fma(-dot, acc, b);
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int l = 0; l < NF; ++l) {
+                    const int j = h * P * sax + side_base(ax, l);
+                    Ar[j] = (outflow && hi == h) ? fma(dots[ax], Fk[ax][l], Ar[j]) : Ar[j];
+                }
         }
     }
     // lanes without a row carry an identity row so that nothing is singular
