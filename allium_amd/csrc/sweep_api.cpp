@@ -781,10 +781,16 @@ int allium_sweep_set_launch(allium_sweep_t h, int mode)
         const bool split = h->ed.p == 1;
         const std::vector<int> ths = split ? std::vector<int>{256, 512, 1024} : std::vector<int>{128, 256};
         const std::vector<int> pfs = split ? std::vector<int>{0, 2, 3} : std::vector<int>{0, 2};
-        for (int b : d3 ? std::vector<int>{4, 8, 16} : std::vector<int>{8, 16, 32})
+        // cubic boxes, and boxes halved along one axis (shorter in-box level
+        // chains, smaller LDS planes)
+        struct Shape { int x, y, z; };
+        const std::vector<Shape> shapes =
+            d3 ? std::vector<Shape>{{4, 4, 4}, {8, 8, 8}, {16, 16, 16}, {16, 16, 8}, {16, 8, 16}, {8, 16, 16}}
+               : std::vector<Shape>{{8, 8, 1}, {16, 16, 1}, {32, 32, 1}, {32, 16, 1}, {16, 32, 1}};
+        for (const Shape &b : shapes)
             for (int nd : {1, 2, 3, 5, 10})
                 for (int th : ths)
-                    for (int pf : pfs) cands.push_back({5, th, 1, b, b, d3 ? b : 1, nd, pf});
+                    for (int pf : pfs) cands.push_back({5, th, 1, b.x, b.y, b.z, nd, pf});
     }
     // keep the caller's box tiling to restore it when a non-box mode wins
     const allium::BoxPlan keep_box = h->ds.box;

@@ -130,6 +130,7 @@ BOX_TILINGS = [
     ((9, 7), 2, 8, (4, 3), 3, 64),
     ((33, 31), 1, 8, (16, 16), 10, 256),
     ((12, 10), 0, 4, (5, 2), 1, 64),
+    ((20, 18, 17), 1, 8, (16, 16, 8), 1, 512),
 ]
 
 
