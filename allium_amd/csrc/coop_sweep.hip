@@ -38,6 +38,11 @@ __device__ __forceinline__ void fence_row(double (&r)[N])
     for (int j = 0; j < N; ++j) asm volatile("" : "+v"(r[j]));
 }
 
+template <int First, int Count>
+struct AxisRange {
+    static constexpr int first = First, count = Count;
+};
+
 template <int DIM, int P, bool UNI, typename T>
 __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel &cl, int inst, long long wg0)
 {
@@ -192,35 +197,43 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
         if (p.timedep) phase(Mt, p.tau, false);
         // outflow: the own face block joins the row (the face-mass rows were
         // read by the face pass, so these loads are cache hits)
-        // All three rows are loaded before any is used (a row loaded only where
-        // it is added would become one branch and one round trip per entry).
-        fence_row<N>(Ar);
-        asm volatile("" : "+s"(gF));
-        double Fk[DIM][NF];
+        // Each row is loaded before any of it is used (a row loaded only where
+        // it is added would become one branch and one round trip per entry):
+        // the first axis's row, then the other two in one batch.
+        auto outflow_rows = [&](auto axes) {
+            fence_row<N>(Ar);
+            asm volatile("" : "+s"(gF));
+            constexpr int A0 = decltype(axes)::first, NA = decltype(axes)::count;
+            double Fk[NA][NF];
 #pragma unroll
-        for (int ax = 0; ax < DIM; ++ax) {
-            int k;
-            const int hi = side_of(ax, k) & 1;
-            const auto F = gF + (size_t)face_of(2 * ax + hi) * NF * NF + k * NF;
+            for (int t = 0; t < NA; ++t) {
+                const int ax = A0 + t;
+                int k;
+                const int hi = side_of(ax, k) & 1;
+                const auto F = gF + (size_t)face_of(2 * ax + hi) * NF * NF + k * NF;
 #pragma unroll
-            for (int l = 0; l < NF; ++l) Fk[ax][l] = F[l];
-        }
+                for (int l = 0; l < NF; ++l) Fk[t][l] = F[l];
+            }
 #pragma unroll
-        for (int ax = 0; ax < DIM; ++ax) fence_row<NF>(Fk[ax]);
+            for (int t = 0; t < NA; ++t) fence_row<NF>(Fk[t]);
 #pragma unroll
-        for (int ax = 0; ax < DIM; ++ax) {
-            int k;
-            const int hi = side_of(ax, k) & 1;
-            const int sax = ax == 0 ? 1 : (ax == 1 ? Q : Q * Q);
-            const bool outflow = dots[ax] > 0;
+            for (int t = 0; t < NA; ++t) {
+                const int ax = A0 + t;
+                int k;
+                const int hi = side_of(ax, k) & 1;
+                const int sax = ax == 0 ? 1 : (ax == 1 ? Q : Q * Q);
+                const bool outflow = dots[ax] > 0;
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int l = 0; l < NF; ++l) {
-                    const int j = h * P * sax + side_base(ax, l);
-                    Ar[j] = (outflow && hi == h) ? fma(dots[ax], Fk[ax][l], Ar[j]) : Ar[j];
-                }
-        }
+                    for (int l = 0; l < NF; ++l) {
+                        const int j = h * P * sax + side_base(ax, l);
+                        Ar[j] = (outflow && hi == h) ? fma(dots[ax], Fk[t][l], Ar[j]) : Ar[j];
+                    }
+            }
+        };
+        outflow_rows(AxisRange<0, 1>{});
+        outflow_rows(AxisRange<1, DIM - 1>{});
     }
     // lanes without a row carry an identity row so that nothing is singular
     T A[N], bb = (T)b;
