@@ -166,6 +166,7 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
     if constexpr (UNI) {
         auto K = g_K + ((size_t)a * N + rr) * N;   // includes tau * Mt and the outflow faces
         auto Mu = g_Mu + (size_t)rr * N;
+        asm volatile("" : "+v"(K), "+v"(Mu), "+v"(b));
 #pragma unroll
         for (int j = 0; j < H; ++j) Ar[j] = fma(sig, Mu[j], K[j]);
         fence_row<H>(reinterpret_cast<double(&)[H]>(Ar));
