@@ -1,33 +1,26 @@
 // Cooperative cell solves for local DG systems that do not fit one lane's
 // registers (3D p = 2: 27 x 27), restating src/sweep.cpp:366-475.
 //
-// 32 lanes solve one cell (e, a, g): lane r < n assembles row r of
+// 16 lanes solve one cell (e, a, g), one row of a wave's DPP row each: lane t
+// assembles rows t and t + 16 (t + 16 < n) of
 //   A = sum_d Omega_d G_d + sigma M + tau Mt + sum_{outflow} (Omega.n) F
 // and of the right-hand side (source, isotropic source, upwind traces), then
 // the cell's lanes run an LU without pivoting (A + A^T is positive definite)
-// in which the owner of pivot row k publishes it through LDS, and a
-// column-oriented back substitution.  A 256-thread workgroup holds 8 cells,
-// two per wave; the lanes of a cell synchronise within their wave only.
+// and a column-oriented back substitution.  The pivot row and each solved
+// unknown reach the other lanes of the cell by DPP row broadcast
+// (row_newbcast) from the owner lane's registers: no LDS, no synchronisation.
+// A 256-thread workgroup holds 16 cells, four per wave.
 // The launch covers one level of up to kMaxBatch independent instances, so
 // a batch of uneven meshes sweeps in max(levels) launches instead of
 // sum(levels).
+#include <utility>
+
 #include "device.hpp"
 #include "kernel_common.hpp"
 
 namespace allium {
 
 bool coop_supported(int dim, int p) { return dim == 3 && p == 2; }
-
-// The 32 lanes of a cell are half of one wave and its pivot row is LDS that
-// no other cell touches, so the exchange needs no workgroup barrier: LDS
-// executes a wave's instructions in order, and the wavefront-scope fences keep
-// the compiler from moving the reads above the owner lane's write.
-__device__ __forceinline__ void cell_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // every entry of r is computed before any code after this point that the
 // compiler would otherwise hoist (empty asm statements keep their order)
@@ -43,41 +36,78 @@ struct AxisRange {
     static constexpr int first = First, count = Count;
 };
 
-template <int DIM, int P, bool UNI, typename T>
-__device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel &cl, int inst, long long wg0)
+// lane L of each 16-lane row of the wave, to every lane of that row.  Every
+// lane is active where this is called, so the source lane is always valid and
+// bound_ctrl (zero for an invalid source) only spares the copy of an old value.
+template <int L, typename T>
+__device__ __forceinline__ T row_bcast(T x)
+{
+    static_assert(L >= 0 && L < 16, "row_newbcast takes a lane of the row");
+    return __builtin_amdgcn_update_dpp((T)0, x, 0x150 + L, 0xF, 0xF, true);
+}
+
+template <typename F, int... K>
+__device__ __forceinline__ void for_each_k(F &&f, std::integer_sequence<int, K...>)
+{
+    (f(std::integral_constant<int, K>{}), ...);
+}
+
+// one cell (e, a, g) of a level and what every row of it shares
+template <int DIM, int P>
+struct CellRef {
+    int e, a, g;
+    double om[3], sig;
+    long long base, off;   // psi offsets of direction (g, a) and of the cell
+    int c[3];              // element coordinates
+};
+
+// the cell of workgroup-local slot `slot`; valid is false past the level's end
+template <int DIM, int P>
+__device__ __forceinline__ CellRef<DIM, P> load_cell(const SweepParams &p, const CoopLevel &cl, int inst, long long wg0,
+                                                     int slot, bool &valid)
+{
+    constexpr int N = Sh<DIM, P>::N;
+    CellRef<DIM, P> cr;
+    const int local = (int)(wg0 - cl.start[inst]) + slot;
+    valid = local < cl.ncells[inst] * p.G;
+    cr.e = cr.a = cr.g = 0;
+    if (valid) {
+        const int nc = cl.ncells[inst];
+        cr.g = local / nc;
+        const uint32_t cell = as_global(p.cells)[cl.cbeg[inst] + (local - cr.g * nc)];
+        cr.e = (int)(cell >> 10);
+        cr.a = (int)(cell & 1023u);
+    }
+    const auto g_omega = as_global(p.omega);
+    for (int d = 0; d < 3; ++d) cr.om[d] = g_omega[4 * cr.a + d];
+    cr.sig = as_global(p.sigma)[(long long)cr.g * p.Ne + cr.e];
+    cr.base = ((long long)cr.g * p.Nomega + cr.a) * p.Ndof;
+    cr.off = cr.base + (long long)cr.e * N;
+    cr.c[0] = cr.e % p.nel[0];
+    cr.c[1] = (cr.e / p.nel[0]) % p.nel[1];
+    cr.c[2] = cr.e / (p.nel[0] * p.nel[1]);
+    return cr;
+}
+
+// row rr of the cell's local system into (Ar, b)
+template <int DIM, int P, bool UNI>
+__device__ __forceinline__ void assemble_row(const SweepParams &p, const CellRef<DIM, P> &cr, const int rr,
+                                             double (&Ar)[Sh<DIM, P>::N], double &b)
 {
     using S = Sh<DIM, P>;
-    constexpr int N = S::N, NF = S::NF, NS = S::NS;
-    static_assert(N <= 32, "one row per lane of a 32-lane group");
-    __shared__ T piv[8][N + 1];
-    const int slot = threadIdx.x >> 5, r = threadIdx.x & 31;
+    constexpr int N = S::N, NF = S::NF, NS = S::NS, Q = P + 1;
     // every table read through p is global memory
-    const auto g_cells = as_global(p.cells);
-    const auto g_omega = as_global(p.omega), g_sigma = as_global(p.sigma), g_K = as_global(p.K);
-    const auto g_Mu = as_global(p.Mu), g_Mt = as_global(p.Mt), g_Gd = as_global(p.Gd);
+    const auto g_K = as_global(p.K), g_Mu = as_global(p.Mu), g_Mt = as_global(p.Mt), g_Gd = as_global(p.Gd);
     const auto g_src = as_global(p.src), g_iso = as_global(p.iso), g_F = as_global(p.Fface);
     const auto g_fnor = as_global(p.fnor);
     const auto g_refl = as_global(p.refl);
     const auto g_psi = as_global(p.psi);
-    const int local = (int)(wg0 - cl.start[inst]) + slot;
-    const bool valid = local < cl.ncells[inst] * p.G;
-    int e = 0, a = 0, g = 0;
-    if (valid) {
-        const int nc = cl.ncells[inst];
-        g = local / nc;
-        const uint32_t cell = g_cells[cl.cbeg[inst] + (local - g * nc)];
-        e = (int)(cell >> 10);
-        a = (int)(cell & 1023u);
-    }
-    const bool row = valid && r < N;
-    const int rr = r < N ? r : 0;
-
-    // ---- row rr of the local system -----------------------------------------
-    double Ar[N], b = 0.0;
-    const double om[3] = {g_omega[4 * a], g_omega[4 * a + 1], g_omega[4 * a + 2]};
-    const double sig = g_sigma[(long long)g * p.Ne + e];
-    const long long base = ((long long)g * p.Nomega + a) * p.Ndof;
-    const long long off = base + (long long)e * N;
+    const int e = cr.e, a = cr.a, g = cr.g;
+    const double(&om)[3] = cr.om;
+    const double sig = cr.sig;
+    const long long base = cr.base, off = cr.off;
+    const int(&c)[3] = cr.c;
+    b = 0.0;
     if (p.src) b = g_src[off + rr];
     if (p.iso) b += g_iso[(long long)g * p.Ndof + (long long)e * N + rr];
 
@@ -90,9 +120,7 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
     // not live yet); on non-uniform geometry the outflow face blocks join the
     // row after them (a uniform mesh has them in K).  Both passes take the
     // sides in the order 2 axis + {low, high}, as one loop over the sides would.
-    constexpr int Q = P + 1;
     const int ic[3] = {rr % Q, (rr / Q) % Q, rr / (Q * Q)};
-    const int c[3] = {e % p.nel[0], (e / p.nel[0]) % p.nel[1], e / (p.nel[0] * p.nel[1])};
     // the node's side on axis ax (-1: none) and its face-lexicographic index there
     auto side_of = [&](int ax, int &k) {
         k = ax == 0 ? ic[1] + Q * ic[2] : (ax == 1 ? ic[0] + Q * ic[2] : ic[0] + Q * ic[1]);
@@ -236,13 +264,134 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
         outflow_rows(AxisRange<0, 1>{});
         outflow_rows(AxisRange<1, DIM - 1>{});
     }
+}
+
+// 16 lanes per cell, two rows per lane, LU by DPP row broadcast (no LDS
+// exchange): the FP64 solves and the uniform-geometry FP32 ones
+template <int DIM, int P, bool UNI, typename T>
+__device__ __forceinline__ void coop_cell_dpp(const SweepParams &p, const CoopLevel &cl, int inst, long long wg0)
+{
+    constexpr int N = Sh<DIM, P>::N;
+    static_assert(N > 16 && N <= 32, "two rows per lane of a 16-lane row");
+    const int slot = threadIdx.x >> 4, t = threadIdx.x & 15;
+    bool valid;
+    const CellRef<DIM, P> cr = load_cell<DIM, P>(p, cl, inst, wg0, slot, valid);
+    const auto g_psi = as_global(p.psi);
+    const long long off = cr.off;
+    auto assemble = [&](const int rr, double (&Ar)[N], double &b) { assemble_row<DIM, P, UNI>(p, cr, rr, Ar, b); };
+
+    // rows t and t + 16 of the cell (a lane without a second row assembles its
+    // first one again; that copy takes no part in the solve)
+    constexpr int R1 = N - 16;   // lanes with a second row
+    const int r1 = t < R1 ? t + 16 : t;
+    T A0[N], A1[N], b0, b1;
+    {
+        double Ar[N], b;
+        assemble(t, Ar, b);
+        if constexpr (sizeof(T) == 4) {
+            // FP32 rows wait in LDS (one column per thread, 27.6 KB per
+            // workgroup) while the second row is assembled: registers for
+            // more waves.  The empty asm keeps the compiler from carrying
+            // the stored values across in registers.
+            __shared__ float stash[N][256];
+#pragma unroll
+            for (int j = 0; j < N; ++j) stash[j][threadIdx.x] = (float)Ar[j];
+            b0 = (T)b;
+            asm volatile("" ::: "memory");
+            fence_row<N>(Ar);
+            assemble(r1, Ar, b);
+#pragma unroll
+            for (int j = 0; j < N; ++j) A1[j] = (T)Ar[j];
+            b1 = (T)b;
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int j = 0; j < N; ++j) A0[j] = stash[j][threadIdx.x];
+        } else {
+#pragma unroll
+            for (int j = 0; j < N; ++j) A0[j] = (T)Ar[j];
+            b0 = (T)b;
+            fence_row<N>(Ar);
+            assemble(r1, Ar, b);
+#pragma unroll
+            for (int j = 0; j < N; ++j) A1[j] = (T)Ar[j];
+            b1 = (T)b;
+        }
+    }
+
+    // ---- LU without pivoting ------------------------------------------------
+    // Pivot row k is row (k & 15) of slot k >> 4 on lane k & 15.  Every lane
+    // runs every update; a row at or above the pivot takes multiplier 0.
+    for_each_k(
+        [&](auto kc) {
+            constexpr int k = decltype(kc)::value, kl = k & 15;
+            const T piv = k < 16 ? row_bcast<kl>(A0[k]) : row_bcast<kl>(A1[k]);
+            const T rp = fast_rcp(piv);
+            T l0 = (T)0, l1 = (T)0;
+            if constexpr (k < 15) l0 = t > k ? A0[k] * rp : (T)0;
+            l1 = (t < R1 && t + 16 > k) ? A1[k] * rp : (T)0;
+#pragma unroll
+            for (int j = k + 1; j < N; ++j) {
+                const T pj = k < 16 ? row_bcast<kl>(A0[j]) : row_bcast<kl>(A1[j]);
+                if constexpr (k < 15) A0[j] = fma(-l0, pj, A0[j]);
+                A1[j] = fma(-l1, pj, A1[j]);
+            }
+            const T pb = k < 16 ? row_bcast<kl>(b0) : row_bcast<kl>(b1);
+            if constexpr (k < 15) b0 = fma(-l0, pb, b0);
+            b1 = fma(-l1, pb, b1);
+        },
+        std::make_integer_sequence<int, N>{});
+
+    // ---- back substitution, one unknown per step ----------------------------
+    T x0 = (T)0, x1 = (T)0;
+    for_each_k(
+        [&](auto kc) {
+            constexpr int k = N - 1 - decltype(kc)::value, kl = k & 15;
+            const T own = k < 16 ? b0 * fast_rcp(A0[k]) : b1 * fast_rcp(A1[k]);
+            const T xk = row_bcast<kl>(own);
+            if constexpr (k < 16) x0 = t == kl ? xk : x0;
+            else x1 = t == kl ? xk : x1;
+            b0 = t < k ? fma(-A0[k], xk, b0) : b0;
+            b1 = (t < R1 && t + 16 < k) ? fma(-A1[k], xk, b1) : b1;
+        },
+        std::make_integer_sequence<int, N>{});
+    if (valid) {
+        g_psi[off + t] = (double)x0;
+        if (t < R1) g_psi[off + t + 16] = (double)x1;
+    }
+}
+
+// 32 lanes per cell, one row per lane, the pivot row published through LDS:
+// the non-uniform FP32 solves, whose limit is the memory pipeline rather than
+// the LDS, and which keep four waves per SIMD this way
+template <int DIM, int P, bool UNI, typename T>
+__device__ __forceinline__ void coop_cell_lds(const SweepParams &p, const CoopLevel &cl, int inst, long long wg0)
+{
+    constexpr int N = Sh<DIM, P>::N;
+    static_assert(N <= 32, "one row per lane of a 32-lane group");
+    __shared__ T piv[8][N + 1];
+    const int slot = threadIdx.x >> 5, r = threadIdx.x & 31;
+    bool valid;
+    const CellRef<DIM, P> cr = load_cell<DIM, P>(p, cl, inst, wg0, slot, valid);
+    const bool row = valid && r < N;
+    const int rr = r < N ? r : 0;
+    double Ar[N], b;
+    assemble_row<DIM, P, UNI>(p, cr, rr, Ar, b);
     // lanes without a row carry an identity row so that nothing is singular
     T A[N], bb = (T)b;
 #pragma unroll
     for (int j = 0; j < N; ++j) A[j] = row ? (T)Ar[j] : (T)(j == r ? 1.0 : 0.0);
     if (!row) bb = (T)0;
-
-    // ---- LU without pivoting across the cell's lanes ------------------------
+    // LU without pivoting across the cell's lanes: the owner of pivot row k
+    // publishes it through LDS; the 32 lanes of a cell are half of one wave
+    // and its pivot row is LDS no other cell touches, so the exchange needs
+    // no workgroup barrier (LDS executes a wave's instructions in order, and
+    // the wavefront-scope fences keep the compiler from moving the reads
+    // above the owner lane's write)
+    auto cell_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         if (r == k) {
@@ -259,7 +408,7 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
         }
         cell_sync();
     }
-    // ---- back substitution, one unknown per step ----------------------------
+    // back substitution, one unknown per step
     T x = (T)0;
 #pragma unroll
     for (int k = N - 1; k >= 0; --k) {
@@ -272,26 +421,41 @@ __device__ __forceinline__ void coop_cell(const SweepParams &p, const CoopLevel 
         if (r < k) bb = fma(-A[k], xk, bb);
         cell_sync();
     }
-    if (row) g_psi[off + rr] = (double)x;
+    if (row) as_global(p.psi)[cr.off + rr] = (double)x;
 }
 
 // BATCH: the instances' parameters are the device array prms (a level of
 // several instances); otherwise p0, one instance.  The two are separate
 // instantiations so that the parameters' address space is known (a pointer
 // selected between the two would be generic and its loads flat).
+// Layout per instantiation: the DPP cells (16 per workgroup) except for
+// non-uniform FP32 solves, which keep the LDS exchange (8 cells per workgroup).
+template <bool UNI, typename T>
+constexpr bool kCoopDpp = UNI || sizeof(T) == 8;
+template <bool UNI, typename T>
+constexpr int kCoopCellsPerGroup = kCoopDpp<UNI, T> ? 16 : 8;
+static_assert(kCoopCells % 16 == 0, "instances start on a workgroup of either layout");
+
 template <int DIM, int P, bool UNI, typename T, bool BATCH>
-__global__ void __launch_bounds__(256) k_sweep_coop(SweepParams p0, const SweepParams *__restrict__ prms,
-                                                    CoopLevel cl)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
+k_sweep_coop(SweepParams p0, const SweepParams *__restrict__ prms, CoopLevel cl)
 {
-    // each instance's items start on a multiple of 8, so the instance (and with it
-    // every parameter read through p) is uniform over the workgroup
-    const long long wg0 = (long long)blockIdx.x * 8;
+    // each instance's items start on a multiple of kCoopCells, so the instance (and
+    // with it every parameter read through p) is uniform over the workgroup
+    const long long wg0 = (long long)blockIdx.x * kCoopCellsPerGroup<UNI, T>;
     int inst = 0;
     while (inst + 1 < cl.ninst && wg0 >= cl.start[inst + 1]) ++inst;
-    if constexpr (BATCH)
-        coop_cell<DIM, P, UNI, T>(*(const SweepParams *)as_global(prms + inst), cl, inst, wg0);
-    else
-        coop_cell<DIM, P, UNI, T>(p0, cl, inst, wg0);
+    if constexpr (kCoopDpp<UNI, T>) {
+        if constexpr (BATCH)
+            coop_cell_dpp<DIM, P, UNI, T>(*(const SweepParams *)as_global(prms + inst), cl, inst, wg0);
+        else
+            coop_cell_dpp<DIM, P, UNI, T>(p0, cl, inst, wg0);
+    } else {
+        if constexpr (BATCH)
+            coop_cell_lds<DIM, P, UNI, T>(*(const SweepParams *)as_global(prms + inst), cl, inst, wg0);
+        else
+            coop_cell_lds<DIM, P, UNI, T>(p0, cl, inst, wg0);
+    }
 }
 
 template <int DIM, int P, bool UNI>
@@ -300,13 +464,15 @@ static hipError_t launch_dp(const SweepParams &p0, const SweepParams *d_prms, co
 {
     const long long items = cl.start[cl.ninst];
     if (items <= 0) return hipSuccess;
-    const int grid = (int)((items + 7) / 8);
+    auto grid_of = [items](int cells) { return (int)((items + cells - 1) / cells); };
     if (precision == 1) {
+        const int grid = grid_of(kCoopCellsPerGroup<UNI, float>);
         if (d_prms)
             hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, float, true>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
         else
             hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, float, false>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
     } else {
+        const int grid = grid_of(kCoopCellsPerGroup<UNI, double>);
         if (d_prms)
             hipLaunchKernelGGL((k_sweep_coop<DIM, P, UNI, double, true>), dim3(grid), dim3(256), 0, s, p0, d_prms, cl);
         else

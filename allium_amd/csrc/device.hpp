@@ -168,9 +168,11 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
                         hipStream_t stream, double *phi = nullptr);
 
 // Cooperative cell solves (local systems too large for one lane: 3D p = 2):
-// 32 lanes per cell, lane r owns row r.  One launch runs one level of up to
-// kMaxBatch independent instances (batched sweeps of uneven meshes).
+// 16 lanes per cell, lane t owns rows t and t + 16, kCoopCells cells per
+// workgroup.  One launch runs one level of up to kMaxBatch independent
+// instances (batched sweeps of uneven meshes).
 constexpr int kMaxBatch = 16;
+constexpr int kCoopCells = 16;
 struct CoopLevel {
     int ninst;
     int start[kMaxBatch + 1];    // prefix over instances of (cells x groups) in this launch

@@ -1306,7 +1306,8 @@ int allium_sweep_batch_create(const allium_sweep_t *handles, int count, allium_s
             const bool has = L < ds.nlevels;
             cl.cbeg[i] = has ? ds.level_ptr[L] : 0;
             cl.ncells[i] = has ? (int)(ds.level_ptr[L + 1] - ds.level_ptr[L]) : 0;
-            cl.start[i + 1] = cl.start[i] + (cl.ncells[i] * b->h[i]->G + 7) / 8 * 8;   // workgroup-aligned
+            const int items = cl.ncells[i] * b->h[i]->G;   // workgroup-aligned
+            cl.start[i + 1] = cl.start[i] + (items + allium::kCoopCells - 1) / allium::kCoopCells * allium::kCoopCells;
         }
         b->levels.push_back(cl);
     }
