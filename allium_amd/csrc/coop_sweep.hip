@@ -440,9 +440,15 @@ template <int DIM, int P, bool UNI, typename T, bool BATCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
 k_sweep_coop(SweepParams p0, const SweepParams *__restrict__ prms, CoopLevel cl)
 {
+    // Workgroups go to the 8 XCDs round robin (blockIdx.x % 8).  Each XCD takes
+    // a contiguous run of the level's cells instead, so the cells of one element
+    // (adjacent in the element-ordered level) share that XCD's L2.
+    const int nwg = (int)gridDim.x, x = (int)(blockIdx.x & 7u), q = (int)(blockIdx.x >> 3);
+    const int per = nwg >> 3, rem = nwg & 7;
+    const long long wg = (long long)x * per + (x < rem ? x : rem) + q;
     // each instance's items start on a multiple of kCoopCells, so the instance (and
     // with it every parameter read through p) is uniform over the workgroup
-    const long long wg0 = (long long)blockIdx.x * kCoopCellsPerGroup<UNI, T>;
+    const long long wg0 = wg * kCoopCellsPerGroup<UNI, T>;
     int inst = 0;
     while (inst + 1 < cl.ninst && wg0 >= cl.start[inst + 1]) ++inst;
     if constexpr (kCoopDpp<UNI, T>) {
