@@ -508,6 +508,11 @@ def test_p2_reflective_side_time_absorption_two_groups(jitter):
     for mode in (0, 1):
         Linv.set_launch(mode)
         assert rel_err(Linv.Mult(q, np.empty_like(q)), ref) < FP64_TOL
+    # FP32 LU: the DPP cells on uniform geometry, the LDS-exchange cells on jittered
+    Linv.set_precision(A.FP32_REFINED)
+    e32 = rel_err(Linv.Mult(q, np.empty_like(q)), ref)
+    print(f"  fp32 LU rel err {e32:.2e}")
+    assert e32 < 1e-4
 
 
 def test_p2_batch_of_uneven_instances():
