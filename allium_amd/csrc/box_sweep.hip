@@ -575,6 +575,36 @@ constexpr int kSwapY = 0xB1;    // quad_perm [1,0,3,2]: partner along y (lane bi
 constexpr int kSwapZ = 0x4E;    // quad_perm [2,3,0,1]: partner along z (lane bit 1)
 constexpr int kSwapYZ = 0x1B;   // quad_perm [3,2,1,0]: partner along y and z
 
+// A direction's constants of the lane-split solve (t_d = |Omega_d| area_d / vol
+// on the unit cell): computed once per workgroup when it sweeps one direction.
+template <typename T>
+struct DirConst {
+    T t[3];
+    T s2t;        // 2 (t_x + t_y + t_z)
+    T bx, by, bz; // sqrt(2) t_d
+    T q;          // by^2 + bz^2 - bx^2
+    T b2;         // 2 bx
+    T c, c2;      // 2 by bz and its square
+};
+template <int DIM, typename T>
+__device__ __forceinline__ DirConst<T> dir_const(const T (&t)[3])
+{
+    constexpr T r2 = (T)1.4142135623730951;
+    DirConst<T> k;
+    k.t[0] = t[0];
+    k.t[1] = t[1];
+    k.t[2] = DIM == 3 ? t[2] : (T)0;
+    k.s2t = (T)2 * (k.t[0] + k.t[1] + k.t[2]);
+    k.bx = r2 * k.t[0];
+    k.by = r2 * k.t[1];
+    k.bz = r2 * k.t[2];
+    k.q = k.by * k.by + k.bz * k.bz - k.bx * k.bx;
+    k.b2 = (T)2 * k.bx;
+    k.c = (T)2 * k.by * k.bz;
+    k.c2 = k.c * k.c;
+    return k;
+}
+
 // Operators along y and z act between the lanes of a cell: lane bit b is the
 // node index on that axis, and the partner lane (quad_perm swap) holds the
 // other node.  J = [[0,1],[-1,0]] is the partner's value with sign + on the
@@ -697,6 +727,87 @@ __device__ __forceinline__ void fd_split_solve(T &v0, T &v1, const T (&t)[3], T 
     if constexpr (DIM == 3) wfwd([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
 }
 
+// fd_split_solve with the direction's constants precomputed (one direction per
+// workgroup): the same arithmetic less the per-cell products of t_d
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void fd_split_solve_k(T &v0, T &v1, const DirConst<T> &K, T sig, bool hy, bool hz)
+{
+    constexpr T r2 = (T)1.4142135623730951, ir2 = (T)0.70710678118654752;
+    constexpr bool nx = OCT & 1, ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
+    // W^{-1}: W+^{-1} = [[1,0],[1/r2,1/r2]], W-^{-1} = [[0,1],[1/r2,1/r2]]
+    {
+        const T w0 = v0, w1 = v1;
+        v0 = nx ? w1 : w0;
+        v1 = ir2 * (w0 + w1);
+    }
+    auto winv = [&](auto swp, bool hi, bool neg) {
+        const T p0 = swp(v0), p1 = swp(v1);
+        const T a0 = ir2 * (v0 + p0), a1 = ir2 * (v1 + p1);
+        v0 = hi ? a0 : (neg ? p0 : v0);
+        v1 = hi ? a1 : (neg ? p1 : v1);
+    };
+    winv([](T x) { return quad_swap<T, kSwapY>(x); }, hy, ny);
+    if constexpr (DIM == 3) winv([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
+    // S^{-1}
+    const T alpha = sig + K.s2t;
+    const T bx = K.bx, by = K.by, bz = K.bz;
+    T u0 = alpha * v0 + bx * v1, u1 = alpha * v1 - bx * v0;   // A_x v
+    {
+        // - beta_y J_y v: J_y v = +partner on the bit-0 lane, -partner on the bit-1 lane
+        const T gy0 = by * quad_swap<T, kSwapY>(v0), gy1 = by * quad_swap<T, kSwapY>(v1);
+        u0 = hy ? u0 + gy0 : u0 - gy0;
+        u1 = hy ? u1 + gy1 : u1 - gy1;
+        if constexpr (DIM == 3) {
+            const T gz0 = bz * quad_swap<T, kSwapZ>(v0), gz1 = bz * quad_swap<T, kSwapZ>(v1);
+            u0 = hz ? u0 + gz0 : u0 - gz0;
+            u1 = hz ? u1 + gz1 : u1 - gz1;
+        }
+    }
+    const T a = fma(alpha, alpha, K.q), b = alpha * K.b2;
+    if constexpr (DIM == 3) {
+        // K u = s_y s_z u(partner along y and z): + when the bits agree
+        const T c = K.c;
+        const T k0 = c * quad_swap<T, kSwapYZ>(u0), k1 = c * quad_swap<T, kSwapYZ>(u1);
+        const bool same = hy == hz;
+        const T w0 = a * u0 + b * u1 + (same ? k0 : -k0);
+        const T w1 = a * u1 - b * u0 + (same ? k1 : -k1);
+        const T e = a * a - b * b - K.c2, f = (T)2 * a * b;
+        const T inv = fast_rcp(e * e + f * f);
+        v0 = (e * w0 - f * w1) * inv;
+        v1 = (e * w1 + f * w0) * inv;
+    } else {
+        const T inv = fast_rcp(a * a + b * b);
+        v0 = (a * u0 - b * u1) * inv;
+        v1 = (a * u1 + b * u0) * inv;
+    }
+    // W: W+ = [[1,0],[-1,r2]], W- = [[-1,r2],[1,0]]
+    {
+        const T w0 = v0, w1 = v1;
+        v0 = nx ? -w0 + r2 * w1 : w0;
+        v1 = nx ? w0 : -w0 + r2 * w1;
+    }
+    auto wfwd = [&](auto swp, bool hi, bool neg) {
+        const T p0 = swp(v0), p1 = swp(v1);
+        if (neg) {
+            v0 = hi ? p0 : r2 * p0 - v0;
+            v1 = hi ? p1 : r2 * p1 - v1;
+        } else {
+            v0 = hi ? r2 * v0 - p0 : v0;
+            v1 = hi ? r2 * v1 - p1 : v1;
+        }
+    };
+    wfwd([](T x) { return quad_swap<T, kSwapY>(x); }, hy, ny);
+    if constexpr (DIM == 3) wfwd([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
+}
+
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void fd_split_p1_k(T &v0, T &v1, const DirConst<T> &K, T sig, T ux, const T (&uy)[2],
+                                              const T (&uz)[2], bool hy, bool hz)
+{
+    fd_split_rhs<DIM, T, OCT>(v0, v1, K.t, ux, uy, uz, hy, hz);
+    fd_split_solve_k<DIM, T, OCT>(v0, v1, K, sig, hy, hz);
+}
+
 template <int DIM, typename T, int OCT>
 __device__ __forceinline__ void fd_split_p1(T &v0, T &v1, const T (&t)[3], T sig, T ux, const T (&uy)[2],
                                             const T (&uz)[2], bool hy, bool hz)
@@ -788,7 +899,7 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
 
 template <int DIM, typename T, int OCT, bool REF = false, bool ONE = false>
 __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int r,
-                                            const SplitIn &in)
+                                            const SplitIn &in, const DirConst<T> &K)
 {
     constexpr int LPC = 1 << (DIM - 1), NF = LPC, PS = box_slot_stride(NF);
     const BoxLds &L = cx.L;
@@ -839,9 +950,16 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
             uz[0] = (T)slot[2][2 * by];
             uz[1] = (T)slot[2][2 * by + 1];
         }
-        const T t[3] = {(T)(L.om[3 * d] * inv_vol), (T)(L.om[3 * d + 1] * inv_vol), (T)(L.om[3 * d + 2] * inv_vol)};
         T v0 = (T)((in.q0 + in.i0) * inv_vol), v1 = (T)((in.q1 + in.i1) * inv_vol);
-        fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by != 0, bz != 0);
+        if constexpr (ONE) {
+            // one direction per workgroup: its constants were formed once, in registers
+            fd_split_p1_k<DIM, T, OCT>(v0, v1, K, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by != 0,
+                                       bz != 0);
+        } else {
+            const T t[3] = {(T)(L.om[3 * d] * inv_vol), (T)(L.om[3 * d + 1] * inv_vol),
+                            (T)(L.om[3 * d + 2] * inv_vol)};
+            fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by != 0, bz != 0);
+        }
         o.x = (double)v0;
         o.y = (double)v1;
     }
@@ -870,6 +988,15 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     constexpr int LPC = 1 << (DIM - 1);
     const BoxCtx cx = box_stage<DIM, 1, OCT>(p, bp, box_base, lds);
     const int tid = threadIdx.x, nt = blockDim.x;
+    // One direction per workgroup: its solve constants once, in registers.
+    // (Kept in LDS instead, the reads sat in each cell's dependent chain and
+    // cost more than the products they saved.)
+    DirConst<T> K{};
+    if constexpr (ONE && !REF) {
+        const double inv_vol = 1.0 / bp.vol;
+        const T t[3] = {(T)(cx.L.om[0] * inv_vol), (T)(cx.L.om[1] * inv_vol), (T)(cx.L.om[2] * inv_vol)};
+        K = dir_const<DIM, T>(t);
+    }
     const int r = tid & (LPC - 1), qd = tid / LPC, nq = nt / LPC;
     if constexpr (PF == 0) {
         for (int l = 0; l < bp.nl; ++l) {
@@ -877,7 +1004,7 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
             for (int it = qd; it < items; it += nq) {   // uniform over the lanes of a cell
                 SplitIn in;
                 split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l, it, r, in);
-                if (in.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, in);
+                if (in.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, in, K);
             }
             lds_barrier();
         }
@@ -889,11 +1016,11 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     auto level = [&](int l, SplitIn &cur, SplitIn &nxt) {
         const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * (ONE ? 1 : cx.ndir);
         split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l + AHEAD, qd, r, nxt);
-        if (cur.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, cur);
+        if (cur.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, cur, K);
         for (int it = qd + nq; it < items; it += nq) {   // uniform over the lanes of a cell
             SplitIn in;
             split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l, it, r, in);
-            if (in.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, in);
+            if (in.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, in, K);
         }
         lds_barrier();
     };
