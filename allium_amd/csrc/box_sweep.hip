@@ -585,6 +585,9 @@ struct DirConst {
     T q;          // by^2 + bz^2 - bx^2
     T b2;         // 2 bx
     T c, c2;      // 2 by bz and its square
+    // signed by this lane's node bits (constant per lane): +by on a bit-0 lane
+    // along y, -by on a bit-1 lane; the same along z; +c where the bits agree
+    T syb, szb, cs;
 };
 template <int DIM, typename T>
 __device__ __forceinline__ DirConst<T> dir_const(const T (&t)[3])
@@ -750,27 +753,20 @@ __device__ __forceinline__ void fd_split_solve_k(T &v0, T &v1, const DirConst<T>
     if constexpr (DIM == 3) winv([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
     // S^{-1}
     const T alpha = sig + K.s2t;
-    const T bx = K.bx, by = K.by, bz = K.bz;
+    const T bx = K.bx;
     T u0 = alpha * v0 + bx * v1, u1 = alpha * v1 - bx * v0;   // A_x v
-    {
-        // - beta_y J_y v: J_y v = +partner on the bit-0 lane, -partner on the bit-1 lane
-        const T gy0 = by * quad_swap<T, kSwapY>(v0), gy1 = by * quad_swap<T, kSwapY>(v1);
-        u0 = hy ? u0 + gy0 : u0 - gy0;
-        u1 = hy ? u1 + gy1 : u1 - gy1;
-        if constexpr (DIM == 3) {
-            const T gz0 = bz * quad_swap<T, kSwapZ>(v0), gz1 = bz * quad_swap<T, kSwapZ>(v1);
-            u0 = hz ? u0 + gz0 : u0 - gz0;
-            u1 = hz ? u1 + gz1 : u1 - gz1;
-        }
+    // - beta_y J_y v - beta_z J_z v, the signs carried by the lane's constants
+    u0 = fma(K.syb, quad_swap<T, kSwapY>(v0), u0);
+    u1 = fma(K.syb, quad_swap<T, kSwapY>(v1), u1);
+    if constexpr (DIM == 3) {
+        u0 = fma(K.szb, quad_swap<T, kSwapZ>(v0), u0);
+        u1 = fma(K.szb, quad_swap<T, kSwapZ>(v1), u1);
     }
     const T a = fma(alpha, alpha, K.q), b = alpha * K.b2;
     if constexpr (DIM == 3) {
         // K u = s_y s_z u(partner along y and z): + when the bits agree
-        const T c = K.c;
-        const T k0 = c * quad_swap<T, kSwapYZ>(u0), k1 = c * quad_swap<T, kSwapYZ>(u1);
-        const bool same = hy == hz;
-        const T w0 = a * u0 + b * u1 + (same ? k0 : -k0);
-        const T w1 = a * u1 - b * u0 + (same ? k1 : -k1);
+        const T w0 = fma(K.cs, quad_swap<T, kSwapYZ>(u0), a * u0 + b * u1);
+        const T w1 = fma(K.cs, quad_swap<T, kSwapYZ>(u1), a * u1 - b * u0);
         const T e = a * a - b * b - K.c2, f = (T)2 * a * b;
         const T inv = fast_rcp(e * e + f * f);
         v0 = (e * w0 - f * w1) * inv;
@@ -996,6 +992,11 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
         const double inv_vol = 1.0 / bp.vol;
         const T t[3] = {(T)(cx.L.om[0] * inv_vol), (T)(cx.L.om[1] * inv_vol), (T)(cx.L.om[2] * inv_vol)};
         K = dir_const<DIM, T>(t);
+        // this lane's node bits (the lane's r = tid mod 2^(DIM-1) in the level loop)
+        const bool ly = (tid & 1) != 0, lz = DIM == 3 && ((tid >> 1) & 1) != 0;
+        K.syb = ly ? K.by : -K.by;
+        K.szb = lz ? K.bz : -K.bz;
+        K.cs = ly == lz ? K.c : -K.c;
     }
     const int r = tid & (LPC - 1), qd = tid / LPC, nq = nt / LPC;
     if constexpr (PF == 0) {
