@@ -588,6 +588,7 @@ struct DirConst {
     // signed by this lane's node bits (constant per lane): +by on a bit-0 lane
     // along y, -by on a bit-1 lane; the same along z; +c where the bits agree
     T syb, szb, cs;
+    T wiy[2], wiz[2];   // the lane's W_y^{-1} and W_z^{-1} rows: v <- [0] v + [1] partner
 };
 template <int DIM, typename T>
 __device__ __forceinline__ DirConst<T> dir_const(const T (&t)[3])
@@ -743,14 +744,15 @@ __device__ __forceinline__ void fd_split_solve_k(T &v0, T &v1, const DirConst<T>
         v0 = nx ? w1 : w0;
         v1 = ir2 * (w0 + w1);
     }
-    auto winv = [&](auto swp, bool hi, bool neg) {
+    // W_y^{-1}, W_z^{-1} between the lanes: v <- cA v + cB partner, with the
+    // lane's (cA, cB) = (1/r2, 1/r2) on a bit-1 lane, else (0, 1) or (1, 0)
+    auto winv = [&](auto swp, T cA, T cB) {
         const T p0 = swp(v0), p1 = swp(v1);
-        const T a0 = ir2 * (v0 + p0), a1 = ir2 * (v1 + p1);
-        v0 = hi ? a0 : (neg ? p0 : v0);
-        v1 = hi ? a1 : (neg ? p1 : v1);
+        v0 = fma(cB, p0, cA * v0);
+        v1 = fma(cB, p1, cA * v1);
     };
-    winv([](T x) { return quad_swap<T, kSwapY>(x); }, hy, ny);
-    if constexpr (DIM == 3) winv([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
+    winv([](T x) { return quad_swap<T, kSwapY>(x); }, K.wiy[0], K.wiy[1]);
+    if constexpr (DIM == 3) winv([](T x) { return quad_swap<T, kSwapZ>(x); }, K.wiz[0], K.wiz[1]);
     // S^{-1}
     const T alpha = sig + K.s2t;
     const T bx = K.bx;
@@ -997,6 +999,12 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
         K.syb = ly ? K.by : -K.by;
         K.szb = lz ? K.bz : -K.bz;
         K.cs = ly == lz ? K.c : -K.c;
+        constexpr T ir2 = (T)0.70710678118654752;
+        constexpr bool ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
+        K.wiy[0] = ly ? ir2 : (ny ? (T)0 : (T)1);
+        K.wiy[1] = ly ? ir2 : (ny ? (T)1 : (T)0);
+        K.wiz[0] = lz ? ir2 : (nz ? (T)0 : (T)1);
+        K.wiz[1] = lz ? ir2 : (nz ? (T)1 : (T)0);
     }
     const int r = tid & (LPC - 1), qd = tid / LPC, nq = nt / LPC;
     if constexpr (PF == 0) {
