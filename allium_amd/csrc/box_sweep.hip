@@ -798,11 +798,49 @@ __device__ __forceinline__ void fd_split_solve_k(T &v0, T &v1, const DirConst<T>
     if constexpr (DIM == 3) wfwd([](T x) { return quad_swap<T, kSwapZ>(x); }, hz, nz);
 }
 
+// fd_split_rhs for a source already scaled by 2^dim / vol: m^{-1} = 2 m~ with
+// m~ = [[2,-1],[-1,2]] on every axis, the 2s folded into that scale, so each
+// axis is one FMA per value
+template <int DIM, typename T, int OCT>
+__device__ __forceinline__ void fd_split_rhs_h(T &v0, T &v1, const T (&t)[3], T ux, const T (&uy)[2],
+                                               const T (&uz)[2], bool hy, bool hz)
+{
+    constexpr bool nx = OCT & 1, ny = (OCT >> 1) & 1, nz = (OCT >> 2) & 1;
+    {
+        const T w0 = v0, w1 = v1;
+        v0 = fma((T)2, w0, -w1);
+        v1 = fma((T)2, w1, -w0);
+    }
+    {
+        const T p0 = quad_swap<T, kSwapY>(v0), p1 = quad_swap<T, kSwapY>(v1);
+        v0 = fma((T)2, v0, -p0);
+        v1 = fma((T)2, v1, -p1);
+    }
+    if constexpr (DIM == 3) {
+        const T p0 = quad_swap<T, kSwapZ>(v0), p1 = quad_swap<T, kSwapZ>(v1);
+        v0 = fma((T)2, v0, -p0);
+        v1 = fma((T)2, v1, -p1);
+    }
+    // inflow faces as in fd_split_rhs
+    v0 += t[0] * (nx ? (T)-2 : (T)4) * ux;
+    v1 += t[0] * (nx ? (T)4 : (T)-2) * ux;
+    {
+        const T wy = t[1] * ((hy == ny) ? (T)4 : (T)-2);
+        v0 += wy * uy[0];
+        v1 += wy * uy[1];
+    }
+    if constexpr (DIM == 3) {
+        const T wz = t[2] * ((hz == nz) ? (T)4 : (T)-2);
+        v0 += wz * uz[0];
+        v1 += wz * uz[1];
+    }
+}
+
 template <int DIM, typename T, int OCT>
 __device__ __forceinline__ void fd_split_p1_k(T &v0, T &v1, const DirConst<T> &K, T sig, T ux, const T (&uy)[2],
                                               const T (&uz)[2], bool hy, bool hz)
 {
-    fd_split_rhs<DIM, T, OCT>(v0, v1, K.t, ux, uy, uz, hy, hz);
+    fd_split_rhs_h<DIM, T, OCT>(v0, v1, K.t, ux, uy, uz, hy, hz);
     fd_split_solve_k<DIM, T, OCT>(v0, v1, K, sig, hy, hz);
 }
 
@@ -948,18 +986,23 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
             uz[0] = (T)slot[2][2 * by];
             uz[1] = (T)slot[2][2 * by + 1];
         }
-        T v0 = (T)((in.q0 + in.i0) * inv_vol), v1 = (T)((in.q1 + in.i1) * inv_vol);
         if constexpr (ONE) {
-            // one direction per workgroup: its constants were formed once, in registers
+            // one direction per workgroup: its constants were formed once, in registers;
+            // the source carries M^{-1}'s factor 2^dim with its 1 / vol
+            constexpr double sc = DIM == 3 ? 8.0 : 4.0;
+            T v0 = (T)((in.q0 + in.i0) * (sc * inv_vol)), v1 = (T)((in.q1 + in.i1) * (sc * inv_vol));
             fd_split_p1_k<DIM, T, OCT>(v0, v1, K, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by != 0,
                                        bz != 0);
+            o.x = (double)v0;
+            o.y = (double)v1;
         } else {
+            T v0 = (T)((in.q0 + in.i0) * inv_vol), v1 = (T)((in.q1 + in.i1) * inv_vol);
             const T t[3] = {(T)(L.om[3 * d] * inv_vol), (T)(L.om[3 * d + 1] * inv_vol),
                             (T)(L.om[3 * d + 2] * inv_vol)};
             fd_split_p1<DIM, T, OCT>(v0, v1, t, (T)(in.sig + (p.timedep ? p.tau : 0.0)), ux, uy, uz, by != 0, bz != 0);
+            o.x = (double)v0;
+            o.y = (double)v1;
         }
-        o.x = (double)v0;
-        o.y = (double)v1;
     }
     *reinterpret_cast<d2v *>(p.psi + in.off) = o;
     // outflow traces
