@@ -131,6 +131,8 @@ BOX_TILINGS = [
     ((33, 31), 1, 8, (16, 16), 10, 256),
     ((12, 10), 0, 4, (5, 2), 1, 64),
     ((20, 18, 17), 1, 8, (16, 16, 8), 1, 512),
+    ((33, 31), 1, 8, (16, 16), 1, 256),
+    ((14, 13, 12), 1, 8, (8, 8, 8), 1, 1024),
 ]
 
 
