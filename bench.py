@@ -487,7 +487,9 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(args.n, args.seed, args.cpu_reps)
         log(f"cpu baseline {cb['value']:.3e} unknowns/s on {cb['cores']} threads")
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        # cores: the threads used; the host's physical cores and nproc each under their own name
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "threads_used",
+                                                  "nproc", "physical_cores")}
         out["gpu_over_cpu"] = value / cb["value"]
     print(json.dumps(out), flush=True)
 
