@@ -147,6 +147,145 @@ __global__ void __launch_bounds__(256) k_conv(Geo g, const bf16 *__restrict__ in
         }
 }
 
+// The same convolution with the input staged in LDS.  A dilated 3x3x3 stencil
+// couples only voxels of one sub-grid (x mod d, y mod d, z mod d), where it is
+// an undilated stencil, so a workgroup takes an 8 x 8 x 4 tile of one sub-grid
+// and loads its 10 x 10 x 6 halo into LDS once (each input voxel then feeds its
+// 27 taps from LDS instead of 27 cache reads).  The K steps, MFMAs and their
+// order are those of k_conv, so the outputs are the same.
+constexpr int kTX = 8, kTY = 8, kTZ = 4, kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
+constexpr int kHalo = kHX * kHY * kHZ;
+
+struct TileGeo {
+    int nsx, nsy, nsz;   // sub-grid extents (the largest, at offset 0)
+    int ntx, nty, ntz;   // tiles per sub-grid axis
+};
+
+__host__ __device__ inline TileGeo tile_geo(const Geo &g)
+{
+    TileGeo t;
+    t.nsx = (g.nx + g.dil - 1) / g.dil;
+    t.nsy = (g.ny + g.dil - 1) / g.dil;
+    t.nsz = (g.nz + g.dil - 1) / g.dil;
+    t.ntx = (t.nsx + kTX - 1) / kTX;
+    t.nty = (t.nsy + kTY - 1) / kTY;
+    t.ntz = (t.nsz + kTZ - 1) / kTZ;
+    return t;
+}
+
+template <int CIP, int COP, int MODE>
+__global__ void __launch_bounds__(256) k_conv_t(Geo g, const bf16 *__restrict__ in, const bf16 *__restrict__ W,
+                                                const float *__restrict__ bias, bf16 *__restrict__ out_bf,
+                                                float *__restrict__ out_f, const bf16 *__restrict__ mask)
+{
+    constexpr int CH = CIP / 8, NCH = 27 * CH, KS = (NCH + 3) / 4, NT = COP / 16, MT = kConvMT;
+    // voxel stride in LDS: the channels plus 16 bytes, so that the 16 rows an
+    // MFMA operand reads start in different banks
+    constexpr int VS = CIP * 2 + 16;
+    static_assert(MT * 16 * 4 == kTX * kTY * kTZ, "a workgroup's voxels are one tile");
+    extern __shared__ __align__(16) unsigned char halo[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, grp = lane >> 4;
+    const TileGeo tg = tile_geo(g);
+    // workgroup -> (batch, sub-grid offset, tile)
+    long long q = blockIdx.x;
+    const int tx = (int)(q % tg.ntx); q /= tg.ntx;
+    const int ty = (int)(q % tg.nty); q /= tg.nty;
+    const int tz = (int)(q % tg.ntz); q /= tg.ntz;
+    const int d = g.dil;
+    const int sx = (int)(q % d); q /= d;
+    const int sy = (int)(q % d); q /= d;
+    const int sz = (int)(q % d); q /= d;
+    const long long b = q;
+    const long long plane = (long long)g.nx * g.ny, vol = plane * g.nz;
+    // halo: CH 16-byte chunks per voxel, zeros outside the grid
+    for (int c = threadIdx.x; c < kHalo * CH; c += 256) {
+        const int hv = c / CH, part = c - hv * CH;
+        const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+        const int x = sx + d * (tx * kTX + hx - 1), y = sy + d * (ty * kTY + hy - 1), z = sz + d * (tz * kTZ + hz - 1);
+        const bool ok = x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz;
+        const long long m = ok ? b * vol + (long long)z * plane + (long long)y * g.nx + x : 0;
+        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(in + m * CIP + part * 8);
+        const bf16x8 zero8 = {};
+        *reinterpret_cast<bf16x8 *>(halo + hv * VS + part * 16) = ok ? v : zero8;
+    }
+    __syncthreads();
+    // this lane's operand rows: voxel (i, j, k) = (r & 7, 2 t + (r >> 3), w) of the tile
+    int hbase[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        const int r = lane & 15;
+        const int i = r & 7, j = 2 * t + (r >> 3), k = w;
+        hbase[t] = ((k + 1) * kHY + (j + 1)) * kHX + (i + 1);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[i][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero8 = {};
+    for (int s = 0; s < KS; ++s) {
+        const int c = 4 * s + grp;
+        const bool cv = c < NCH;
+        const int cc = cv ? c : 0;
+        const int o = cc / CH;
+        const int ci0 = (cc - o * CH) * 8;
+        const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+        const int doff = (dz * kHY + dy) * kHX + dx;
+        bf16x8 a[MT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(halo + (hbase[i] + doff) * VS + ci0 * 2);
+            a[i] = cv ? v : zero8;
+        }
+        bf16x8 bw[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(W + ((long long)o * COP + nt * 16 + (lane & 15)) * CIP + ci0);
+            bw[nt] = cv ? v : zero8;
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int i = 0; i < MT; ++i) acc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bw[nt], acc[i][nt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int jc = nt * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                // output row 4 grp + r of operand tile t
+                const int v = 4 * grp + r;
+                const int i = v & 7, j = 2 * t + (v >> 3), k = w;
+                const int x = sx + d * (tx * kTX + i), y = sy + d * (ty * kTY + j), z = sz + d * (tz * kTZ + k);
+                if (x >= g.nx || y >= g.ny || z >= g.nz) continue;
+                const long long mo = b * vol + (long long)z * plane + (long long)y * g.nx + x;
+                float val = acc[t][nt][r];
+                if constexpr (MODE == 0) {
+                    val = fmaxf(val + bias[jc], 0.f);
+                    out_bf[mo * COP + jc] = (bf16)val;
+                } else if constexpr (MODE == 1) {
+                    if (jc == 0) out_f[mo] = val + bias[0];
+                } else {
+                    val = (float)mask[mo * COP + jc] > 0.f ? val : 0.f;
+                    out_bf[mo * COP + jc] = (bf16)val;
+                }
+            }
+        }
+}
+
+template <int CIP, int COP, int MODE>
+void launch_conv(const Geo &g, const bf16 *in, const bf16 *W, const float *bias, bf16 *out_bf, float *out_f,
+                 const bf16 *mask, hipStream_t s)
+{
+    const TileGeo tg = tile_geo(g);
+    const long long grid = (long long)g.B * g.dil * g.dil * g.dil * tg.ntx * tg.nty * tg.ntz;
+    const size_t lds = (size_t)kHalo * (CIP * 2 + 16);
+    hipLaunchKernelGGL((k_conv_t<CIP, COP, MODE>), dim3((unsigned)grid), dim3(256), lds, s, g, in, W, bias, out_bf,
+                       out_f, mask);
+}
+
 // ---------------------------------------------------------------------------
 // weight gradient partials: part[plane][27][COP][CIT]
 // ---------------------------------------------------------------------------
@@ -523,17 +662,11 @@ Geo layer_geo(const allium_cnn *h, const Geo &g, int l)
 
 int forward_packed(allium_cnn *h, const Geo &g, hipStream_t s)
 {
-    const int grid = (int)((g.T + kConvVoxels - 1) / kConvVoxels);
-    hipLaunchKernelGGL((k_conv<8, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 0), h->xin, h->Wf[0],
-                       h->bias[0], h->act[0], nullptr, nullptr);
-    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 1), h->act[0], h->Wf[1],
-                       h->bias[1], h->act[1], nullptr, nullptr);
-    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 2), h->act[1], h->Wf[2],
-                       h->bias[2], h->act[2], nullptr, nullptr);
-    hipLaunchKernelGGL((k_conv<32, 32, 0>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 3), h->act[2], h->Wf[3],
-                       h->bias[3], h->act[3], nullptr, nullptr);
-    hipLaunchKernelGGL((k_conv<32, 16, 1>), dim3(grid), dim3(256), 0, s, layer_geo(h, g, 4), h->act[3], h->Wf[4],
-                       h->bias[4], nullptr, h->out, nullptr);
+    launch_conv<8, 32, 0>(layer_geo(h, g, 0), h->xin, h->Wf[0], h->bias[0], h->act[0], nullptr, nullptr, s);
+    launch_conv<32, 32, 0>(layer_geo(h, g, 1), h->act[0], h->Wf[1], h->bias[1], h->act[1], nullptr, nullptr, s);
+    launch_conv<32, 32, 0>(layer_geo(h, g, 2), h->act[1], h->Wf[2], h->bias[2], h->act[2], nullptr, nullptr, s);
+    launch_conv<32, 32, 0>(layer_geo(h, g, 3), h->act[2], h->Wf[3], h->bias[3], h->act[3], nullptr, nullptr, s);
+    launch_conv<32, 16, 1>(layer_geo(h, g, 4), h->act[3], h->Wf[4], h->bias[4], nullptr, h->out, nullptr, s);
     CNN_TRY(hipGetLastError());
     return 0;
 }
@@ -588,11 +721,9 @@ int gradient(allium_cnn *h, const Geo &g, const float *d_x, const float *d_t, do
         if (l > 0) {
             // gradient of the previous pre-activation: conv with flipped weights, ReLU mask of its output
             if (l == NL - 1)
-                hipLaunchKernelGGL((k_conv<16, 32, 2>), dim3(grid), dim3(256), 0, s, gl, gz, h->Wb[l], nullptr, gnext,
-                                   nullptr, h->act[l - 1]);
+                launch_conv<16, 32, 2>(gl, gz, h->Wb[l], nullptr, gnext, nullptr, h->act[l - 1], s);
             else
-                hipLaunchKernelGGL((k_conv<32, 32, 2>), dim3(grid), dim3(256), 0, s, gl, gz, h->Wb[l], nullptr, gnext,
-                                   nullptr, h->act[l - 1]);
+                launch_conv<32, 32, 2>(gl, gz, h->Wb[l], nullptr, gnext, nullptr, h->act[l - 1], s);
             const bf16 *t = gnext;
             gnext = const_cast<bf16 *>(gz);
             gz = t;
