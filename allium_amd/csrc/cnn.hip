@@ -54,105 +54,15 @@ constexpr int kMaxDil = 8;
 // MODE 1: + bias, FP32 out [T] (channel 0)
 // MODE 2: masked by (mask[T][COP] > 0), bf16 out [T][COP]
 // ---------------------------------------------------------------------------
-// A wave owns MT consecutive 16-voxel row tiles: every B operand it loads (a
-// 16-byte weight row) feeds MT MFMAs, and each K step issues MT + NT loads for
-// MT * NT MFMAs.  Each output element still accumulates its K steps in order.
+// A workgroup computes one 8 x 8 x 4 tile of voxels with four waves; a wave
+// owns MT = 4 16-voxel row tiles (two y rows of one z plane each), so every B
+// operand it loads (a 16-byte weight row) feeds MT MFMAs.  A dilated 3x3x3
+// stencil couples only voxels of one sub-grid (x mod d, y mod d, z mod d),
+// where it is an undilated stencil: the tile is taken in one sub-grid and its
+// 10 x 10 x 6 halo is loaded into LDS once, so each input voxel feeds its 27
+// taps from LDS instead of 27 cache reads.  Each output element accumulates
+// its K steps (tap-major, then 8-channel chunks) in order.
 constexpr int kConvMT = 4;
-constexpr int kConvVoxels = 4 * 16 * kConvMT;   // per 256-thread workgroup
-
-template <int CIP, int COP, int MODE>
-__global__ void __launch_bounds__(256) k_conv(Geo g, const bf16 *__restrict__ in, const bf16 *__restrict__ W,
-                                              const float *__restrict__ bias, bf16 *__restrict__ out_bf,
-                                              float *__restrict__ out_f, const bf16 *__restrict__ mask)
-{
-    constexpr int CH = CIP / 8, NCH = 27 * CH, KS = (NCH + 3) / 4, NT = COP / 16, MT = kConvMT;
-    static_assert(CIP % 8 == 0 && COP % 16 == 0, "padded channel counts");
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, grp = lane >> 4;
-    const long long m0 = ((long long)blockIdx.x * 4 + w) * (16 * MT);
-    if (m0 >= g.T) return;   // wave-uniform; this kernel has no barriers
-    long long m[MT];
-    bool mv[MT];
-    int x[MT], y[MT], z[MT];
-#pragma unroll
-    for (int i = 0; i < MT; ++i) {
-        m[i] = m0 + 16 * i + (lane & 15);
-        mv[i] = m[i] < g.T;
-        x[i] = y[i] = z[i] = 0;
-        if (mv[i]) {
-            long long r = m[i];
-            x[i] = (int)(r % g.nx);
-            r /= g.nx;
-            y[i] = (int)(r % g.ny);
-            r /= g.ny;
-            z[i] = (int)(r % g.nz);
-        }
-    }
-    f32x4 acc[MT][NT];
-#pragma unroll
-    for (int i = 0; i < MT; ++i)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[i][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // Every load is issued on every path, from an in-bounds address (voxel 0 or
-    // weight row 0 when the operand is padding), and the padding is zeroed by a
-    // select: loads behind branches made the compiler wait for each pair of them
-    // before the next was issued.
-    const bf16x8 zero8 = {};
-    for (int s = 0; s < KS; ++s) {
-        const int c = 4 * s + grp;
-        const bool cv = c < NCH;
-        const int cc = cv ? c : 0;
-        const int o = cc / CH;
-        const int ci0 = (cc - o * CH) * 8;
-        const int dz = (o / 9 - 1) * g.dil, dy = ((o / 3) % 3 - 1) * g.dil, dx = (o % 3 - 1) * g.dil;
-        bf16x8 a[MT];
-#pragma unroll
-        for (int i = 0; i < MT; ++i) {
-            const int xx = x[i] + dx, yy = y[i] + dy, zz = z[i] + dz;
-            const bool ok = cv && mv[i] && xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
-            const long long mn = ok ? m[i] + dx + (long long)g.nx * (dy + (long long)g.ny * dz) : 0;
-            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(in + mn * CIP + ci0);
-            a[i] = ok ? v : zero8;
-        }
-        bf16x8 b[NT];
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(W + ((long long)o * COP + nt * 16 + (lane & 15)) * CIP + ci0);
-            b[nt] = cv ? v : zero8;
-        }
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-            for (int i = 0; i < MT; ++i) acc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[nt], acc[i][nt], 0, 0, 0);
-    }
-#pragma unroll
-    for (int i = 0; i < MT; ++i)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const int j = nt * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const long long mo = m0 + 16 * i + 4 * grp + r;
-                if (mo >= g.T) continue;
-                float v = acc[i][nt][r];
-                if constexpr (MODE == 0) {
-                    v = fmaxf(v + bias[j], 0.f);
-                    out_bf[mo * COP + j] = (bf16)v;
-                } else if constexpr (MODE == 1) {
-                    if (j == 0) out_f[mo] = v + bias[0];
-                } else {
-                    v = (float)mask[mo * COP + j] > 0.f ? v : 0.f;
-                    out_bf[mo * COP + j] = (bf16)v;
-                }
-            }
-        }
-}
-
-// The same convolution with the input staged in LDS.  A dilated 3x3x3 stencil
-// couples only voxels of one sub-grid (x mod d, y mod d, z mod d), where it is
-// an undilated stencil, so a workgroup takes an 8 x 8 x 4 tile of one sub-grid
-// and loads its 10 x 10 x 6 halo into LDS once (each input voxel then feeds its
-// 27 taps from LDS instead of 27 cache reads).  The K steps, MFMAs and their
-// order are those of k_conv, so the outputs are the same.
 constexpr int kTX = 8, kTY = 8, kTZ = 4, kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
 
@@ -179,6 +89,7 @@ __global__ void __launch_bounds__(256) k_conv_t(Geo g, const bf16 *__restrict__ 
                                                 float *__restrict__ out_f, const bf16 *__restrict__ mask)
 {
     constexpr int CH = CIP / 8, NCH = 27 * CH, KS = (NCH + 3) / 4, NT = COP / 16, MT = kConvMT;
+    static_assert(CIP % 8 == 0 && COP % 16 == 0, "padded channel counts");
     // voxel stride in LDS: the channels plus 16 bytes, so that the 16 rows an
     // MFMA operand reads start in different banks
     constexpr int VS = CIP * 2 + 16;
@@ -694,7 +605,6 @@ int gradient(allium_cnn *h, const Geo &g, const float *d_x, const float *d_t, do
     rc = forward_packed(h, g, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, s, g.T, h->out, d_t, h->ga, h->lpart);
-    const int grid = (int)((g.T + kConvVoxels - 1) / kConvVoxels);
     const int planes = g.B * g.nz;
     // layer 5: gz = ga [T][16]
     const bf16 *gz = h->ga;
