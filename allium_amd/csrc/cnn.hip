@@ -200,60 +200,100 @@ void launch_conv(const Geo &g, const bf16 *in, const bf16 *W, const float *bias,
 // ---------------------------------------------------------------------------
 // weight gradient partials: part[plane][27][COP][CIT]
 // ---------------------------------------------------------------------------
+// One workgroup per (batch, z) plane; K steps of 32 voxels along x, row by row.
+// Each step stages the step's gradient rows transposed (gT[co][x]) and the
+// nine (dz, dy) input rows with a dil-voxel halo each side (aT[r9][ci][x]) in
+// LDS.  The global reads are 16-byte channel chunks; those of the next step are
+// issued into registers before this step's MFMAs, so their latency hides
+// behind them.  Row strides are padded: gT's keeps 16-byte operand reads
+// aligned and spreads its 16 rows over the banks, aT's is an odd number of
+// dwords.
 template <int CIP, int COP>
 __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ gz, const bf16 *__restrict__ a,
                                                float *__restrict__ part)
 {
     constexpr int NTO = COP / 16, NTI = (CIP + 15) / 16, CIT = NTI * 16, NPAIR = NTO * NTI;
-    constexpr int XW = 32;                // voxels per K step
-    constexpr int XS = XW + 2 * kMaxDil;  // staged x range (dil-voxel halo each side)
+    constexpr int XW = 32;                        // voxels per K step
+    constexpr int XSMAX = XW + 2 * kMaxDil;       // staged x range (dil-voxel halo each side)
+    constexpr int GS = XW + 8, AS = XSMAX + 2;    // LDS row strides (bf16)
+    constexpr int CHI = CIP / 8, CHO = COP / 8;   // 16-byte chunks per voxel
+    constexpr int NA = (9 * XSMAX * CHI + 255) / 256, NG = (XW * CHO + 255) / 256;
     static_assert(NPAIR <= 4, "one tile pair per wave");
-    __shared__ __attribute__((aligned(16))) bf16 gT[COP][XW];
-    __shared__ __attribute__((aligned(16))) bf16 aT[9][CIP][XS];
+    __shared__ __attribute__((aligned(16))) bf16 gT[COP][GS];
+    __shared__ __attribute__((aligned(16))) bf16 aT[9][CIP][AS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, grp = lane >> 4;
     const int plane = blockIdx.x;
     const int b = plane / g.nz, z = plane - b * g.nz;
     const bool active = w < NPAIR;
     const int cot = active ? w / NTI : 0, cit = active ? w % NTI : 0;
+    const int dl = g.dil, xr = XW + 2 * dl;
+    const int nxs = (g.nx + XW - 1) / XW, nsteps = g.ny * nxs;
+    const long long plane_base = ((long long)b * g.nz + z) * g.ny * g.nx;
+    const bf16x8 zero8 = {};
+    bf16x8 ra[NA], rg[NG];
+    // the step's global reads into registers; out-of-grid and padding chunks read
+    // voxel 0 and are zeroed by a select
+    auto load = [&](int st) {
+        const int y = st / nxs, x0 = (st - y * nxs) * XW;
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            const int xi = i / CHO, c = i - xi * CHO;
+            const bool ok = i < XW * CHO && x0 + xi < g.nx;
+            const long long v = ok ? plane_base + (long long)y * g.nx + x0 + xi : 0;
+            const bf16x8 r = *reinterpret_cast<const bf16x8 *>(gz + v * COP + 8 * c);
+            rg[k] = ok ? r : zero8;
+        }
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            const int c = i % CHI, rest = i / CHI;
+            const int xi = rest % xr, r9 = rest / xr;
+            const int xx = x0 - dl + xi, yy = y + (r9 % 3 - 1) * dl, zz = z + (r9 / 3 - 1) * dl;
+            const bool ok = r9 < 9 && xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
+            const long long v = ok ? (((long long)b * g.nz + zz) * g.ny + yy) * g.nx + xx : 0;
+            const bf16x8 r = *reinterpret_cast<const bf16x8 *>(a + v * CIP + 8 * c);
+            ra[k] = ok ? r : zero8;
+        }
+    };
     f32x4 acc[27];
 #pragma unroll
     for (int o = 0; o < 27; ++o) acc[o] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const long long plane_base = ((long long)b * g.nz + z) * g.ny * g.nx;
-    for (int y = 0; y < g.ny; ++y) {
-        for (int x0 = 0; x0 < g.nx; x0 += XW) {
-            __syncthreads();
-            const long long row = plane_base + (long long)y * g.nx;
-            for (int i = threadIdx.x; i < XW * COP; i += blockDim.x) {
-                const int xi = i / COP, co = i - xi * COP;
-                const int xx = x0 + xi;
-                gT[co][xi] = xx < g.nx ? gz[(row + xx) * COP + co] : (bf16)0.f;
-            }
-            const int dl = g.dil, xr = XW + 2 * dl;
-            for (int i = threadIdx.x; i < 9 * xr * CIP; i += blockDim.x) {
-                const int ci = i % CIP;
-                const int rest = i / CIP;
-                const int xi = rest % xr, r9 = rest / xr;
-                const int dz = (r9 / 3 - 1) * dl, dy = (r9 % 3 - 1) * dl;
-                const int xx = x0 - dl + xi, yy = y + dy, zz = z + dz;
-                bf16 v = (bf16)0.f;
-                if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
-                    v = a[((((long long)b * g.nz + zz) * g.ny + yy) * g.nx + xx) * CIP + ci];
-                aT[r9][ci][xi] = v;
-            }
-            __syncthreads();
-            if (active) {
-                const bf16x8 av = *reinterpret_cast<const bf16x8 *>(&gT[cot * 16 + (lane & 15)][8 * grp]);
-                const int ci = cit * 16 + (lane & 15);
+    if (nsteps > 0) load(0);
+    for (int st = 0; st < nsteps; ++st) {
+        __syncthreads();   // the previous step's operand reads are done
 #pragma unroll
-                for (int o = 0; o < 27; ++o) {
-                    const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
-                    const int r9 = (dz + 1) * 3 + (dy + 1);
-                    bf16x8 bv;
+        for (int k = 0; k < NG; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            if (i < XW * CHO) {
+                const int xi = i / CHO, c = i - xi * CHO;
 #pragma unroll
-                    for (int t = 0; t < 8; ++t)
-                        bv[t] = ci < CIP ? aT[r9][ci][g.dil * (1 + dx) + 8 * grp + t] : (bf16)0.f;
-                    acc[o] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[o], 0, 0, 0);
-                }
+                for (int t = 0; t < 8; ++t) gT[8 * c + t][xi] = rg[k][t];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            const int c = i % CHI, rest = i / CHI;
+            const int xi = rest % xr, r9 = rest / xr;
+            if (r9 < 9) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) aT[r9][8 * c + t][xi] = ra[k][t];
+            }
+        }
+        __syncthreads();
+        if (st + 1 < nsteps) load(st + 1);
+        if (active) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8 *>(&gT[cot * 16 + (lane & 15)][8 * grp]);
+            const int ci = cit * 16 + (lane & 15);
+#pragma unroll
+            for (int o = 0; o < 27; ++o) {
+                const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+                const int r9 = (dz + 1) * 3 + (dy + 1);
+                bf16x8 bv;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) bv[t] = ci < CIP ? aT[r9][ci][dl * (1 + dx) + 8 * grp + t] : (bf16)0.f;
+                acc[o] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[o], 0, 0, 0);
             }
         }
     }
