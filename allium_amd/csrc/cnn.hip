@@ -307,45 +307,78 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
         }
 }
 
-// grad[o][co][ci] (packed, real sizes) = sum over planes of the partials
-__global__ void k_wgrad_reduce(const float *__restrict__ part, int nplane, int COP, int CIT, int CO, int CI,
-                               float *__restrict__ grad)
+// grad[o][co][ci] (packed, real sizes) = sum over planes of the partials.  A
+// block sums 64 outputs: each of its four waves adds every fourth plane (loads
+// unrolled eight deep) and the four sums are added in wave order, so the result
+// does not depend on timing.
+__global__ void __launch_bounds__(256) k_wgrad_reduce(const float *__restrict__ part, int nplane, int COP, int CIT,
+                                                      int CO, int CI, float *__restrict__ grad)
 {
+    __shared__ float red[4][64];
     const int n = 27 * CO * CI;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int jl = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + jl;
+    float s = 0.f;
+    if (i < n) {
         const int o = i / (CO * CI), rem = i - o * CO * CI, co = rem / CI, ci = rem - co * CI;
-        float s = 0.f;
-        for (int p = 0; p < nplane; ++p) s += part[(((long long)p * 27 + o) * COP + co) * CIT + ci];
-        grad[i] = s;
+        const long long stride = 27LL * COP * CIT;
+        const float *p0 = part + ((long long)o * COP + co) * CIT + ci;
+#pragma unroll 8
+        for (int p = sl; p < nplane; p += 4) s += p0[p * stride];
     }
+    red[sl][jl] = s;
+    __syncthreads();
+    if (sl == 0 && i < n) grad[i] = ((red[0][jl] + red[1][jl]) + red[2][jl]) + red[3][jl];
 }
 
-// bias gradient partials: bpart[block][COP] = sum over the block's voxels
+// bias gradient partials: bpart[block][COP] = sum over the block's voxels.  A
+// thread reads one 16-byte chunk (8 channels) of a voxel per step; the block's
+// per-thread sums are added in a fixed order.
 template <int COP>
 __global__ void __launch_bounds__(256) k_bias_grad(long long T, const bf16 *__restrict__ gz, float *__restrict__ bpart)
 {
-    constexpr int VPB = 256 / COP;
-    __shared__ float red[256];
-    const int co = threadIdx.x % COP, vi = threadIdx.x / COP;
-    float s = 0.f;
-    for (long long v = (long long)blockIdx.x * VPB + vi; v < T; v += (long long)gridDim.x * VPB)
-        s += (float)gz[v * COP + co];
-    red[threadIdx.x] = s;
+    constexpr int CHO = COP / 8, VPB = 256 / CHO;
+    __shared__ float red[256][9];
+    const int c = threadIdx.x % CHO, vi = threadIdx.x / CHO;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (long long v = (long long)blockIdx.x * VPB + vi; v < T; v += (long long)gridDim.x * VPB) {
+        const bf16x8 r = *reinterpret_cast<const bf16x8 *>(gz + v * COP + 8 * c);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) s[t] += (float)r[t];
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) red[threadIdx.x][t] = s[t];
     __syncthreads();
     if (threadIdx.x < COP) {
-        float t = 0.f;
-        for (int k = 0; k < VPB; ++k) t += red[k * COP + threadIdx.x];
-        bpart[(long long)blockIdx.x * COP + threadIdx.x] = t;
+        const int cc = threadIdx.x >> 3, t = threadIdx.x & 7;
+        float a = 0.f;
+        for (int k = 0; k < VPB; ++k) a += red[k * CHO + cc][t];
+        bpart[(long long)blockIdx.x * COP + threadIdx.x] = a;
     }
 }
 
-__global__ void k_bias_reduce(const float *__restrict__ bpart, int nblk, int COP, int CO, float *__restrict__ grad)
+// grad[co] = sum over the partials of k_bias_grad: eight slices of 32 threads
+// (one per channel) add every eighth partial, then the slices in order
+__global__ void __launch_bounds__(256) k_bias_reduce(const float *__restrict__ bpart, int nblk, int COP, int CO,
+                                                     float *__restrict__ grad)
 {
-    const int co = threadIdx.x;
-    if (co >= CO) return;
+    static_assert(HID <= 32, "one thread per channel in a 32-thread slice");
+    __shared__ float red[8][32];
+    const int co = threadIdx.x & 31, sl = threadIdx.x >> 5;
     float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += bpart[(long long)b * COP + co];
-    grad[co] = s;
+    if (co < CO) {
+#pragma unroll 8
+        for (int b = sl; b < nblk; b += 8) s += bpart[(long long)b * COP + co];
+    }
+    red[sl][co] = s;
+    __syncthreads();
+    if (sl == 0 && co < CO) {
+        float a = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a += red[k][co];
+        grad[co] = a;
+    }
 }
 
 // x [T][3] float -> [T][8] bf16
@@ -665,9 +698,9 @@ int gradient(allium_cnn *h, const Geo &g, const float *d_x, const float *d_t, do
             hipLaunchKernelGGL((k_wgrad<32, 32>), dim3(planes), dim3(256), 0, s, gl, gz, ain, h->part);
             hipLaunchKernelGGL(k_bias_grad<32>, dim3(kBiasBlocks), dim3(256), 0, s, g.T, gz, h->bpart);
         }
-        hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(27L * CO_[l] * CI_[l])), dim3(256), 0, s, h->part, planes,
-                           COP_[l], CIT, CO_[l], CI_[l], gW);
-        hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(64), 0, s, h->bpart, kBiasBlocks, COP_[l], CO_[l], gB);
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((27L * CO_[l] * CI_[l] + 63) / 64)), dim3(256), 0, s,
+                           h->part, planes, COP_[l], CIT, CO_[l], CI_[l], gW);
+        hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(256), 0, s, h->bpart, kBiasBlocks, COP_[l], CO_[l], gB);
         if (l > 0) {
             // gradient of the previous pre-activation: conv with flipped weights, ReLU mask of its output
             if (l == NL - 1)
