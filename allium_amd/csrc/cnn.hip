@@ -218,14 +218,20 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
     constexpr int GS = XW + 8, AS = XSMAX + 2;    // LDS row strides (bf16)
     constexpr int CHI = CIP / 8, CHO = COP / 8;   // 16-byte chunks per voxel
     constexpr int NA = (9 * XSMAX * CHI + 255) / 256, NG = (XW * CHO + 255) / 256;
-    static_assert(NPAIR <= 4, "one tile pair per wave");
+    static_assert(NPAIR == 1 || NPAIR == 2 || NPAIR == 4, "tile pairs divide the four waves");
+    // The waves take the NPAIR (16 co x 16 ci) tile pairs.  With 8 input
+    // channels, the waves left over share a pair and split its 27 taps (input
+    // layer: 126 -> 86 us on 8 x 32^3); with 32 they stay idle, which timed
+    // faster than splitting.
+    constexpr int WPP = CIP <= 8 ? 4 / NPAIR : 1;
     __shared__ __attribute__((aligned(16))) bf16 gT[COP][GS];
     __shared__ __attribute__((aligned(16))) bf16 aT[9][CIP][AS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, grp = lane >> 4;
     const int plane = blockIdx.x;
     const int b = plane / g.nz, z = plane - b * g.nz;
-    const bool active = w < NPAIR;
-    const int cot = active ? w / NTI : 0, cit = active ? w % NTI : 0;
+    const bool active = w < NPAIR * WPP;
+    const int pw = WPP == 1 ? w : w % NPAIR, tap0 = WPP == 1 ? 0 : w / NPAIR;   // taps o = tap0 mod WPP
+    const int cot = active ? pw / NTI : 0, cit = active ? pw % NTI : 0;
     const int dl = g.dil, xr = XW + 2 * dl;
     const int nxs = (g.nx + XW - 1) / XW, nsteps = g.ny * nxs;
     const long long plane_base = ((long long)b * g.nz + z) * g.ny * g.nx;
@@ -288,6 +294,7 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
             const int ci = cit * 16 + (lane & 15);
 #pragma unroll
             for (int o = 0; o < 27; ++o) {
+                if (WPP > 1 && o % WPP != tap0) continue;
                 const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
                 const int r9 = (dz + 1) * 3 + (dy + 1);
                 bf16x8 bv;
@@ -299,12 +306,14 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
     }
     if (!active) return;
 #pragma unroll
-    for (int o = 0; o < 27; ++o)
+    for (int o = 0; o < 27; ++o) {
+        if (WPP > 1 && o % WPP != tap0) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int co = cot * 16 + 4 * grp + r, ci = cit * 16 + (lane & 15);
             part[(((long long)plane * 27 + o) * COP + co) * CIT + ci] = acc[o][r];
         }
+    }
 }
 
 // grad[o][co][ci] (packed, real sizes) = sum over planes of the partials.  A
