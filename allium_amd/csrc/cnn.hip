@@ -32,6 +32,7 @@ namespace {
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int NL = 5;
@@ -205,9 +206,10 @@ void launch_conv(const Geo &g, const bf16 *in, const bf16 *W, const float *bias,
 // nine (dz, dy) input rows with a dil-voxel halo each side (aT[r9][ci][x]) in
 // LDS.  The global reads are 16-byte channel chunks; those of the next step are
 // issued into registers before this step's MFMAs, so their latency hides
-// behind them.  Row strides are padded: gT's keeps 16-byte operand reads
-// aligned and spreads its 16 rows over the banks, aT's is an odd number of
-// dwords.
+// behind them.  A thread stages two neighbouring input voxels, so each channel
+// pair goes to LDS as one 4-byte store.  Row strides are padded: gT's keeps
+// 16-byte operand reads aligned and spreads its 16 rows over the banks, aT's is
+// an odd number of dwords.
 template <int CIP, int COP>
 __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ gz, const bf16 *__restrict__ a,
                                                float *__restrict__ part)
@@ -217,7 +219,7 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
     constexpr int XSMAX = XW + 2 * kMaxDil;       // staged x range (dil-voxel halo each side)
     constexpr int GS = XW + 8, AS = XSMAX + 2;    // LDS row strides (bf16)
     constexpr int CHI = CIP / 8, CHO = COP / 8;   // 16-byte chunks per voxel
-    constexpr int NA = (9 * XSMAX * CHI + 255) / 256, NG = (XW * CHO + 255) / 256;
+    constexpr int NA = (9 * (XSMAX / 2) * CHI + 255) / 256, NG = (XW * CHO + 255) / 256;
     static_assert(NPAIR == 1 || NPAIR == 2 || NPAIR == 4, "tile pairs divide the four waves");
     // The waves take the NPAIR (16 co x 16 ci) tile pairs.  With 8 input
     // channels, the waves left over share a pair and split its 27 taps (input
@@ -236,7 +238,8 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
     const int nxs = (g.nx + XW - 1) / XW, nsteps = g.ny * nxs;
     const long long plane_base = ((long long)b * g.nz + z) * g.ny * g.nx;
     const bf16x8 zero8 = {};
-    bf16x8 ra[NA], rg[NG];
+    bf16x8 ra[NA][2], rg[NG];
+    const int xh = xr / 2;
     // the step's global reads into registers; out-of-grid and padding chunks read
     // voxel 0 and are zeroed by a select
     auto load = [&](int st) {
@@ -254,12 +257,16 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
         for (int k = 0; k < NA; ++k) {
             const int i = threadIdx.x + 256 * k;
             const int c = i % CHI, rest = i / CHI;
-            const int xi = rest % xr, r9 = rest / xr;
-            const int xx = x0 - dl + xi, yy = y + (r9 % 3 - 1) * dl, zz = z + (r9 / 3 - 1) * dl;
-            const bool ok = r9 < 9 && xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
-            const long long v = ok ? (((long long)b * g.nz + zz) * g.ny + yy) * g.nx + xx : 0;
-            const bf16x8 r = *reinterpret_cast<const bf16x8 *>(a + v * CIP + 8 * c);
-            ra[k] = ok ? r : zero8;
+            const int xp = rest % xh, r9 = rest / xh;
+            const int yy = y + (r9 % 3 - 1) * dl, zz = z + (r9 / 3 - 1) * dl;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int xx = x0 - dl + 2 * xp + h;
+                const bool ok = r9 < 9 && xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
+                const long long v = ok ? (((long long)b * g.nz + zz) * g.ny + yy) * g.nx + xx : 0;
+                const bf16x8 r = *reinterpret_cast<const bf16x8 *>(a + v * CIP + 8 * c);
+                ra[k][h] = ok ? r : zero8;
+            }
         }
     };
     f32x4 acc[27];
@@ -281,10 +288,11 @@ __global__ void __launch_bounds__(256) k_wgrad(Geo g, const bf16 *__restrict__ g
         for (int k = 0; k < NA; ++k) {
             const int i = threadIdx.x + 256 * k;
             const int c = i % CHI, rest = i / CHI;
-            const int xi = rest % xr, r9 = rest / xr;
+            const int xp = rest % xh, r9 = rest / xh;
             if (r9 < 9) {
 #pragma unroll
-                for (int t = 0; t < 8; ++t) aT[r9][8 * c + t][xi] = ra[k][t];
+                for (int t = 0; t < 8; ++t)
+                    *reinterpret_cast<bf16x2 *>(&aT[r9][8 * c + t][2 * xp]) = bf16x2{ra[k][0][t], ra[k][1][t]};
             }
         }
         __syncthreads();
