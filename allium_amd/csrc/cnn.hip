@@ -55,16 +55,16 @@ constexpr int kMaxDil = 8;
 // MODE 1: + bias, FP32 out [T] (channel 0)
 // MODE 2: masked by (mask[T][COP] > 0), bf16 out [T][COP]
 // ---------------------------------------------------------------------------
-// A workgroup computes one 8 x 8 x 4 tile of voxels with four waves; a wave
+// A workgroup computes one 8 x 8 x 8 tile of voxels with eight waves; a wave
 // owns MT = 4 16-voxel row tiles (two y rows of one z plane each), so every B
 // operand it loads (a 16-byte weight row) feeds MT MFMAs.  A dilated 3x3x3
 // stencil couples only voxels of one sub-grid (x mod d, y mod d, z mod d),
 // where it is an undilated stencil: the tile is taken in one sub-grid and its
-// 10 x 10 x 6 halo is loaded into LDS once, so each input voxel feeds its 27
+// 10 x 10 x 10 halo is loaded into LDS once, so each input voxel feeds its 27
 // taps from LDS instead of 27 cache reads.  Each output element accumulates
 // its K steps (tap-major, then 8-channel chunks) in order.
 constexpr int kConvMT = 4;
-constexpr int kTX = 8, kTY = 8, kTZ = 4, kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
+constexpr int kTX = 8, kTY = 8, kTZ = 8, kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
 
 struct TileGeo {
@@ -85,7 +85,7 @@ __host__ __device__ inline TileGeo tile_geo(const Geo &g)
 }
 
 template <int CIP, int COP, int MODE>
-__global__ void __launch_bounds__(256) k_conv_t(Geo g, const bf16 *__restrict__ in, const bf16 *__restrict__ W,
+__global__ void __launch_bounds__(512) k_conv_t(Geo g, const bf16 *__restrict__ in, const bf16 *__restrict__ W,
                                                 const float *__restrict__ bias, bf16 *__restrict__ out_bf,
                                                 float *__restrict__ out_f, const bf16 *__restrict__ mask)
 {
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) k_conv_t(Geo g, const bf16 *__restrict__ 
     // voxel stride in LDS: the channels, unpadded (timed against 16, 32 and 48
     // bytes of padding per voxel, which were 28 % slower in a 64^3 inference)
     constexpr int VS = CIP * 2;
-    static_assert(MT * 16 * 4 == kTX * kTY * kTZ, "a workgroup's voxels are one tile");
+    static_assert(MT * 16 * 8 == kTX * kTY * kTZ, "a workgroup's voxels are one tile");
     extern __shared__ __align__(16) unsigned char halo[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, grp = lane >> 4;
     const TileGeo tg = tile_geo(g);
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256) k_conv_t(Geo g, const bf16 *__restrict__ 
     const long long b = q;
     const long long plane = (long long)g.nx * g.ny, vol = plane * g.nz;
     // halo: CH 16-byte chunks per voxel, zeros outside the grid
-    for (int c = threadIdx.x; c < kHalo * CH; c += 256) {
+    for (int c = threadIdx.x; c < kHalo * CH; c += 512) {
         const int hv = c / CH, part = c - hv * CH;
         const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
         const int x = sx + d * (tx * kTX + hx - 1), y = sy + d * (ty * kTY + hy - 1), z = sz + d * (tz * kTZ + hz - 1);
@@ -194,7 +194,7 @@ void launch_conv(const Geo &g, const bf16 *in, const bf16 *W, const float *bias,
     const TileGeo tg = tile_geo(g);
     const long long grid = (long long)g.B * g.dil * g.dil * g.dil * tg.ntx * tg.nty * tg.ntz;
     const size_t lds = (size_t)kHalo * CIP * 2;
-    hipLaunchKernelGGL((k_conv_t<CIP, COP, MODE>), dim3((unsigned)grid), dim3(256), lds, s, g, in, W, bias, out_bf,
+    hipLaunchKernelGGL((k_conv_t<CIP, COP, MODE>), dim3((unsigned)grid), dim3(512), lds, s, g, in, W, bias, out_bf,
                        out_f, mask);
 }
 
