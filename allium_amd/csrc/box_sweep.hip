@@ -1004,7 +1004,10 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
             o.y = (double)v1;
         }
     }
-    *reinterpret_cast<d2v *>(p.psi + in.off) = o;
+    // psi leaves through nontemporal stores: it is read back once (by the moment
+    // kernel or the next box level's face reads) and would otherwise displace the
+    // L2 and MALL lines holding this sweep's sources and traces
+    __builtin_nontemporal_store(o, reinterpret_cast<d2v *>(p.psi + in.off));
     // outflow traces
     slot[0][r] = (OCT & 1) ? o.x : o.y;
     if (by == (((OCT >> 1) & 1) ? 0 : 1)) {
