@@ -21,6 +21,7 @@
 #include "kernel_common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace allium {
 
@@ -933,7 +934,7 @@ __device__ __forceinline__ void split_fetch(const SweepParams &p, const BoxPlan 
     }
 }
 
-template <int DIM, typename T, int OCT, bool REF = false, bool ONE = false>
+template <int DIM, typename T, int OCT, bool REF = false, bool ONE = false, bool NT = false>
 __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan &bp, const BoxCtx &cx, int r,
                                             const SplitIn &in, const DirConst<T> &K)
 {
@@ -1004,10 +1005,16 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
             o.y = (double)v1;
         }
     }
-    // psi leaves through nontemporal stores: it is read back once (by the moment
-    // kernel or the next box level's face reads) and would otherwise displace the
-    // L2 and MALL lines holding this sweep's sources and traces
-    __builtin_nontemporal_store(o, reinterpret_cast<d2v *>(p.psi + in.off));
+    // NT: the moment kernel reads psi once after the sweep (source iteration,
+    // the Krylov solvers, sweeps with moments), and psi leaves through
+    // nontemporal stores.  At 64^3 S8 source iteration the box levels took 90.8
+    // instead of 96.2 us and the moment kernel 203 instead of 251 us.
+    // Back-to-back standalone sweeps into one psi buffer were slower with them
+    // (1.346 against 1.280 ms), so other sweeps keep plain stores.
+    if constexpr (NT)
+        __builtin_nontemporal_store(o, reinterpret_cast<d2v *>(p.psi + in.off));
+    else
+        *reinterpret_cast<d2v *>(p.psi + in.off) = o;
     // outflow traces
     slot[0][r] = (OCT & 1) ? o.x : o.y;
     if (by == (((OCT >> 1) & 1) ? 0 : 1)) {
@@ -1026,7 +1033,7 @@ __device__ __forceinline__ void split_solve(const SweepParams &p, const BoxPlan 
 // pass of every level is prefetched during the level before (two input
 // buffers with alternating roles, so no register copy waits for loads in
 // flight; the LDS-only barrier lets them cross it), at the price of registers.
-template <int DIM, typename T, int OCT, int PF, bool REF, bool ONE>
+template <int DIM, typename T, int OCT, int PF, bool REF, bool ONE, bool NT>
 __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPlan &bp, int box_base, double *lds)
 {
     constexpr int LPC = 1 << (DIM - 1);
@@ -1059,7 +1066,7 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
             for (int it = qd; it < items; it += nq) {   // uniform over the lanes of a cell
                 SplitIn in;
                 split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l, it, r, in);
-                if (in.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, in, K);
+                if (in.ok) split_solve<DIM, T, OCT, REF, ONE, NT>(p, bp, cx, r, in, K);
             }
             lds_barrier();
         }
@@ -1071,11 +1078,11 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     auto level = [&](int l, SplitIn &cur, SplitIn &nxt) {
         const int items = (cx.L.lpl[l + 1] - cx.L.lpl[l]) * (ONE ? 1 : cx.ndir);
         split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l + AHEAD, qd, r, nxt);
-        if (cur.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, cur, K);
+        if (cur.ok) split_solve<DIM, T, OCT, REF, ONE, NT>(p, bp, cx, r, cur, K);
         for (int it = qd + nq; it < items; it += nq) {   // uniform over the lanes of a cell
             SplitIn in;
             split_fetch<DIM, OCT, REF, ONE>(p, bp, cx, l, it, r, in);
-            if (in.ok) split_solve<DIM, T, OCT, REF, ONE>(p, bp, cx, r, in, K);
+            if (in.ok) split_solve<DIM, T, OCT, REF, ONE, NT>(p, bp, cx, r, in, K);
         }
         lds_barrier();
     };
@@ -1098,20 +1105,20 @@ __device__ __forceinline__ void box_task_split(const SweepParams &p, const BoxPl
     }
 }
 
-template <int DIM, typename T, int PF, bool REF, bool ONE>
+template <int DIM, typename T, int PF, bool REF, bool ONE, bool NT = false>
 __global__ void __launch_bounds__(1024) k_sweep_box_split(SweepParams p, BoxPlan bp, int box_base)
 {
     extern __shared__ double lds[];
     const int oct = blockIdx.x % bp.noct;   // workgroup-uniform
     switch (oct) {
-    case 0: box_task_split<DIM, T, 0, PF, REF, ONE>(p, bp, box_base, lds); break;
-    case 1: box_task_split<DIM, T, 1, PF, REF, ONE>(p, bp, box_base, lds); break;
-    case 2: box_task_split<DIM, T, 2, PF, REF, ONE>(p, bp, box_base, lds); break;
-    case 3: box_task_split<DIM, T, 3, PF, REF, ONE>(p, bp, box_base, lds); break;
-    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4, PF, REF, ONE>(p, bp, box_base, lds); break;
-    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5, PF, REF, ONE>(p, bp, box_base, lds); break;
-    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6, PF, REF, ONE>(p, bp, box_base, lds); break;
-    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7, PF, REF, ONE>(p, bp, box_base, lds); break;
+    case 0: box_task_split<DIM, T, 0, PF, REF, ONE, NT>(p, bp, box_base, lds); break;
+    case 1: box_task_split<DIM, T, 1, PF, REF, ONE, NT>(p, bp, box_base, lds); break;
+    case 2: box_task_split<DIM, T, 2, PF, REF, ONE, NT>(p, bp, box_base, lds); break;
+    case 3: box_task_split<DIM, T, 3, PF, REF, ONE, NT>(p, bp, box_base, lds); break;
+    case 4: if constexpr (DIM == 3) box_task_split<DIM, T, 4, PF, REF, ONE, NT>(p, bp, box_base, lds); break;
+    case 5: if constexpr (DIM == 3) box_task_split<DIM, T, 5, PF, REF, ONE, NT>(p, bp, box_base, lds); break;
+    case 6: if constexpr (DIM == 3) box_task_split<DIM, T, 6, PF, REF, ONE, NT>(p, bp, box_base, lds); break;
+    default: if constexpr (DIM == 3) box_task_split<DIM, T, 7, PF, REF, ONE, NT>(p, bp, box_base, lds); break;
     }
 }
 
@@ -1130,6 +1137,16 @@ static hipError_t run_boxes_split(const DeviceSweep &ds, const SweepParams &prm,
         const int th = (wgs <= ds.box_ncu && ds.box_threads <= 512) ? 2 * ds.box_threads : ds.box_threads;
         // one direction per workgroup (the tiling the autotuner picks at 64^3) has its own
         // instantiation without the item -> (direction, cell) division
+        // nontemporal psi stores (prm.nt_psi) have their own instantiation, for
+        // the FP64 one-direction tiling the solvers run
+        if constexpr (std::is_same<T, double>::value && !REF) {
+            if (bp.nd == 1 && prm.nt_psi) {
+                hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF, REF, true, true>),
+                                   dim3(nbox * bp.noct, prm.G * bp.nchunk), dim3(th), bp.lds_bytes, s, prm, bp,
+                                   ds.box_ptr[k]);
+                continue;
+            }
+        }
         if (bp.nd == 1)
             hipLaunchKernelGGL((k_sweep_box_split<DIM, T, PF, REF, true>), dim3(nbox * bp.noct, prm.G * bp.nchunk),
                                dim3(th), bp.lds_bytes, s, prm, bp, ds.box_ptr[k]);

@@ -49,6 +49,9 @@ struct SweepParams {
     double fix_min;
     unsigned long long *fix_counts;   // applied, failed, ryosuke beyond 3 positive unknowns
     double *fix_monitor;              // G x Ne application counts, or null
+    // 1: the box sweep writes psi with nontemporal stores (psi is read once,
+    // by the moment kernel, before the next sweep)
+    int nt_psi;
 };
 
 // One level of the direction-uniform wave plan: offset of the level in the
@@ -118,6 +121,7 @@ struct DeviceSweep {
     int box_threads = 256;
     int box_ncu = 0;             // CUs of the device (box launches of at most one workgroup per CU)
     int refine = 0;              // box sweeps at p = 1: in-place defect correction with FP32 cell solves
+    int stream_psi = 0;          // set by the solvers whose every sweep is followed by the moment kernel
     int box_prefetch = 2;        // 0 none, 2 next level's inputs loaded during the current solve
     void *box_buf = nullptr;     // device memory of the box plan tables
     // host copies of the plan

@@ -259,6 +259,13 @@ int krylov_solve(allium_sweep_t h, const double *sigma_s, const double *source, 
         ~PrecisionGuard() { h->ds.precision = saved; }
     } guard{h, h->ds.precision};
     h->ds.precision = 0;
+    // every sweep here is followed by the moment kernel's read of psi
+    struct StreamGuard {
+        allium_sweep *h;
+        int saved;
+        ~StreamGuard() { h->ds.stream_psi = saved; }
+    } sguard{h, h->ds.stream_psi};
+    h->ds.stream_psi = 1;
 
     // b = D L^{-1} q ; x = 0
     int rc = allium_sweep_internal::sweep(h, dq, nullptr, h->a.d(), s, h->b.d());

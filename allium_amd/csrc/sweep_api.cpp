@@ -1033,6 +1033,13 @@ static int source_iteration(allium_sweep_t h, const double *sigma_s, const doubl
     HIP_TRY(hipMemcpyAsync(h->coef.p, sigma_s, sizeof(double) * h->mesh.Ne, hipMemcpyHostToDevice, s));
     const double inv_w = 1.0 / h->quad.wsum;
     const int prec_save = h->ds.precision;
+    // every sweep here is followed by the moment kernel's read of psi
+    struct StreamGuard {
+        allium_sweep *h;
+        int saved;
+        ~StreamGuard() { h->ds.stream_psi = saved; }
+    } sguard{h, h->ds.stream_psi};
+    h->ds.stream_psi = 1;
     int rc = ALLIUM_OK;
     std::vector<double> hp(nb);
     double r0 = 0.0;

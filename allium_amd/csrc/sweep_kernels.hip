@@ -890,6 +890,7 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
     prm.src = src;
     prm.iso = iso;
     prm.psi = psi;
+    prm.nt_psi = (ds.stream_psi || phi) ? 1 : 0;
     if (phi && (phi == iso || (const double *)phi == src || phi == psi)) return hipErrorInvalidValue;
     hipError_t err = hipSuccess;
     if (ds.timing) hipEventRecord(ds.ev0, stream);
@@ -898,7 +899,7 @@ hipError_t launch_sweep(DeviceSweep &ds, const double *src, const double *iso, d
     } else if (ds.launch_mode == 1 || ds.launch_mode == 5) {
         // hipGraph of the level (1) or box-level (5) launches, re-captured when
         // the vectors change
-        const int variant = ds.precision + 2 * ds.refine;
+        const int variant = ds.precision + 2 * ds.refine + 4 * prm.nt_psi;
         if (!ds.graph_exec || ds.graph_src != src || ds.graph_iso != iso || ds.graph_psi != psi ||
             ds.graph_prec != variant) {
             if (ds.graph_exec) { hipGraphExecDestroy(ds.graph_exec); ds.graph_exec = nullptr; }
